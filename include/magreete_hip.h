@@ -1,0 +1,115 @@
+/* C ABI of libmagreete_hip.so: the coupled-dipole hot path on MI355X (gfx950).
+ *
+ * Conventions
+ *  - All buffers are device pointers allocated by the caller (torch owns
+ *    them); matrices are row-major complex (interleaved re, im) with a
+ *    leading dimension in elements.  Points are row-major float64 (x, y[, z]).
+ *  - Row d*m + a, column d*j + b: the reference's (M, N, d, d) Green tensor
+ *    transposed to (d M, d N) (Transmission3D.py:267, Transmission2D.py:279).
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = default
+ *    stream) and returns an int status: 0 ok, < 0 invalid argument / runtime
+ *    failure (text from mg_last_error()).  LU info (first exactly-zero pivot,
+ *    1-based) is written to a device int, as LAPACK's info.
+ *  - A handle owns the LU workspace; one handle must not be used from two
+ *    streams at once.
+ */
+#ifndef MAGREETE_HIP_H
+#define MAGREETE_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mg_handle mg_handle;
+
+/* Green's-function kinds (one per reference class) */
+enum { MG_2D_TM = 0, MG_2D_TE = 1, MG_3D_VECTOR = 2, MG_3D_SCALAR = 3 };
+/* element precisions: complex128; complex64; complex64 with bf16x3 products */
+enum { MG_C128 = 0, MG_C64 = 1, MG_BF16X3 = 2 };
+/* assembly flags */
+enum {
+  MG_F_REGULARIZE = 1, MG_F_ZERO_SELF = 2, MG_F_DIAG = 4, MG_F_NANFILL = 8,
+  MG_F_PERIODIC_X = 16, MG_F_PERIODIC_Y = 32, MG_F_PERIODIC_Z = 64,
+  MG_F_BESSEL_EXACT = 128 /* 2D: accurate J/Y instead of the reference's torch values */
+};
+/* status codes */
+enum { MG_OK = 0, MG_ERR_ARG = -1, MG_ERR_HIP = -2, MG_ERR_KIND = -3, MG_ERR_NOMEM = -4, MG_ERR_SYNC = -5,
+       MG_ERR_UNSUPPORTED = -6 };
+
+int mg_version(void);
+const char* mg_last_error(void);
+int mg_create(mg_handle** out);
+int mg_destroy(mg_handle* h);
+
+/* Green's assembly: out[d i + a, d j + b] = scale * G_ab(rows[i] - cols[j]).
+ * Replaces the Green's functions and G0 of the reference
+ * (Transmission2D.py:63-91, 263-280, 595-618, 777-785;
+ *  Transmission3D.py:47-77, 240-268, 573-601, 751-777) and the construction
+ * of M = I - k0^2 alpha G with its self-interaction shift
+ * (Transmission2D.py:222-228, 733-739; Transmission3D.py:197-203, 709-715)
+ * through the flags: ZERO_SELF zeroes block (i, i); DIAG sets entry (r, r)
+ * to diag; NANFILL replaces NaN entries by fill (LDOS fix-up,
+ * Transmission3D.py:353-362); REGULARIZE and PERIODIC_* as the reference's
+ * regularize / periodic arguments.  Either output may be NULL (c128 and/or a
+ * complex64 copy written in the same pass). */
+int mg_assemble(int kind, const double* rows, int64_t nrows, const double* cols, int64_t ncols, double k0,
+                double scale_re, double scale_im, double diag_re, double diag_im, double fill_re, double fill_im,
+                int flags, double radius, void* out_c128, int64_t ld_c128, void* out_c64, int64_t ld_c64,
+                void* stream);
+
+/* C = alpha A B + beta C, row-major complex (prec MG_C128, MG_C64, MG_BF16X3). */
+int mg_cgemm(int prec, int64_t m, int64_t n, int64_t k, double alpha_re, double alpha_im, const void* A,
+             int64_t lda, const void* B, int64_t ldb, double beta_re, double beta_im, void* C, int64_t ldc,
+             void* stream);
+
+/* LU with partial pivoting in place; the factor step of torch.linalg.solve
+ * (Transmission2D.py:230, 742; Transmission3D.py:205, 717).  ipiv: n device
+ * ints (0-based row swapped with row k); info: one device int. */
+int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv, int* info, void* stream);
+/* Multi-RHS solve from getrf's factors, B (n x nrhs) overwritten by X: the
+ * solve step of torch.linalg.solve at the same call sites. */
+int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb,
+             int64_t nrhs, void* stream);
+/* Waits for `stream`; MG_ERR_SYNC if a grid barrier of an LU leaf failed. */
+int mg_check(mg_handle* h, void* stream);
+/* Measurement hooks with no reference counterpart: HIP-event timing of the
+ * LU trailing-update launches (kernel k_cgemm_trail) on the launching stream.
+ * read returns launches, algorithmic flops (8 m n k each) and summed ms since
+ * the last read, and synchronises the stream. */
+int mg_profile_enable(mg_handle* h, int on);
+int mg_profile_read(mg_handle* h, void* stream, long long* launches, double* flops, double* ms);
+/* Element conversion of a rows x cols block; dir 0: c128 -> c64, 1: c64 -> c128. */
+int mg_convert(int dir, const void* src, int64_t lds, void* dst, int64_t ldd, int64_t rows, int64_t cols,
+               void* stream);
+
+/* Fused propagation E = alpha G(meas, scat) Ek + beta E with G evaluated in
+ * kernel (the matmul of propagate / propagate_ss: Transmission2D.py:190,
+ * 245, 701, 760; Transmission3D.py:163, 220, 675, 733).  Only REGULARIZE and
+ * PERIODIC_* flags apply. */
+int mg_propagate(int kind, const double* meas, int64_t n_meas, const double* scat, int64_t n_scat, double k0,
+                 double alpha_re, double alpha_im, int flags, double radius, const void* Ek, int64_t ld_ek,
+                 int64_t nrhs, double beta_re, double beta_im, void* E, int64_t ld_e, void* stream);
+
+/* H_nu^(1)(x) for nu in {0, 1} (torch_hankel1, Transmission2D.py:46-61, 578-593);
+ * exact = 0: the reference's torch values, 1: accurate to ~1e-15. */
+int mg_hankel1(int nu, int exact, const double* x, int64_t n, void* out, void* stream);
+
+/* out[c] = sum_r Y[r, c] X[r, c] without conjugation: the diagonal of
+ * G_p W G_p^T once X = M^{-1} G_p^T (einsum 'ij,ji->i' at
+ * Transmission3D.py:365 and siblings). */
+int mg_coldot(const void* Y, int64_t ldy, const void* X, int64_t ldx, int64_t n, int64_t ncols, void* out,
+              void* stream);
+
+/* Plane waves (type 0) and Gaussian beams (type 1) at points
+ * (generate_source, Transmission2D.py:93-143, 620-652;
+ *  Transmission3D.py:86-120, 610-633).  space 2: dirs[t] = (cos(-theta),
+ * sin(-theta)); space 3: dirs[t] = u_t.  pol NULL (d = 1): out is
+ * npts x T; else out[(m d + a) T + t] = f_t(x_m) pol[t d + a]. */
+int mg_sources(int space, int type, const double* points, int64_t npts, const double* dirs, int64_t T,
+               const double* pol, int64_t d, double k0, double w, void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,246 @@
+"""Shared implementation of the four drop-in solver classes.
+
+The classes keep the constructor, attributes (r, N, source) and method
+signatures of the reference's Transmission2D_{vector,scalar} and
+Transmission3D_{vector,scalar} (Transmission2D.py:38-453, 570-953;
+Transmission3D.py:39-439, 565-949).  The compute runs on the GPU through
+libmagreete_hip: Green's assembly straight into M, blocked LU + multi-RHS
+solve, fused propagation, LDOS through solves against G_p^T.  Results come
+back on the device of the caller's tensors (CPU tensors in, CPU tensors out).
+Progress prints of the reference are not emitted.
+"""
+import cmath
+import math
+
+import numpy as onp
+import torch
+
+from . import _lib as L
+from . import engine as E
+
+
+def _like(t, ref):
+    """Move a result to the device of the caller's tensor."""
+    if ref is not None and not ref.is_cuda:
+        return t.cpu()
+    return t
+
+
+class _Base:
+    kind = None          # engine kind name
+    d = 1                # field components per site
+    space = 2            # coordinates per point
+    # 2D Bessel functions: "reference" reproduces torch.special's values as
+    # the reference computes them (parity); "exact" evaluates them to ~1e-15.
+    bessel = "reference"
+
+    def __init__(self, points, source="beam"):
+        self.r = points.reshape(-1, self.space)
+        self.N = self.r.shape[0]
+        self.source = source
+        self._rdev = None
+
+    # -- helpers ------------------------------------------------------------
+    def _bf(self):
+        """Bessel-mode flag added to every Green's evaluation."""
+        if self.bessel not in ("reference", "exact"):
+            raise ValueError("bessel must be 'reference' or 'exact'")
+        return L.MG_F_BESSEL_EXACT if self.bessel == "exact" else 0
+
+    def _dev(self):
+        return E.device_of(self.r)
+
+    def _r(self):
+        dev = self._dev()
+        if self._rdev is None or self._rdev.device != dev:
+            self._rdev = self.r.to(device=dev, dtype=torch.float64).contiguous()
+        return self._rdev
+
+    def _volume(self, radius):
+        raise NotImplementedError
+
+    def _self_int(self, k0, radius, kind):
+        raise NotImplementedError
+
+    def _shift(self, k0, alpha, radius, kind):
+        """The scalar subtracted on the diagonal of M by the self-interaction."""
+        raise NotImplementedError
+
+    def _system(self, k0, alpha, radius, self_interaction, si_type, want64=False, store128=True):
+        """M = -alpha k0^2 G0(None), unit diagonal, optional self-interaction shift,
+        assembled in one pass (Transmission3D.py:197-203 and siblings)."""
+        r = self._r()
+        scale = -alpha * k0 * k0
+        diag = 1.0
+        bad = False
+        if self_interaction:
+            c = complex(self._shift(k0, alpha, radius, si_type))
+            if not (cmath.isfinite(c)):
+                bad = True   # c * eye has NaN off the diagonal: the whole M is NaN
+            # the reference multiplies c by torch.eye(n), whose default dtype is
+            # float32, so the shift it subtracts is c rounded to complex64
+            diag = 1.0 - complex(onp.complex64(c))
+        flags = L.MG_F_ZERO_SELF | L.MG_F_DIAG | self._bf()
+        M, M64 = E.assemble(self.kind, r, r, k0, scale=scale, flags=flags, diag=diag, want64=want64,
+                            out=None if store128 else None)
+        if bad:
+            if M is not None:
+                M.fill_(complex(math.nan, math.nan))
+            if M64 is not None:
+                M64.fill_(complex(math.nan, math.nan))
+        return M, M64
+
+    def _raise_singular(self, info):
+        i = int(info.item())
+        if i > 0:
+            raise torch.linalg.LinAlgError(
+                "linalg.solve: The solver failed because the input matrix is singular "
+                "(U[%d,%d] is exactly zero)." % (i, i))
+
+    def _factor(self, k0, alpha, radius, self_interaction, si_type):
+        M, _ = self._system(k0, alpha, radius, self_interaction, si_type)
+        ipiv, info = E.getrf(M)
+        return M, ipiv, info
+
+    # -- G0 -------------------------------------------------------------------
+    def _G0(self, points, k0, regularize=False, radius=0.0):
+        r = self._r()
+        if points is None:
+            G, _ = E.assemble(self.kind, r, r, k0, flags=L.MG_F_ZERO_SELF | self._bf())
+            return G, self.r
+        flags = (L.MG_F_REGULARIZE if regularize else 0) | self._bf()
+        G, _ = E.assemble(self.kind, points.to(r.device), r, k0, flags=flags, radius=radius)
+        return G, points
+
+    def _green_raw(self, r, k0, periodic, regularize, radius):
+        """Green's function of a (M, N, space) tensor of difference vectors."""
+        m, n = r.shape[0], r.shape[1]
+        self._wrap(r, periodic)
+        diffs = r.reshape(-1, self.space)
+        dev = E.device_of(diffs)
+        origin = torch.zeros((1, self.space), dtype=torch.float64, device=dev)
+        flags = (L.MG_F_REGULARIZE if regularize else 0) | self._bf()
+        G, _ = E.assemble(self.kind, diffs.to(dev), origin, k0, flags=flags, radius=radius)
+        if self.d == 1:
+            return _like(G.reshape(m, n), r)
+        return _like(G.reshape(m, n, self.d, self.d), r)
+
+    def _wrap(self, r, periodic):
+        raise NotImplementedError
+
+    # -- solve / propagate -----------------------------------------------------
+    def _solve(self, k0, alpha, radius, E0, self_interaction, si_type, flat):
+        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
+        dev = M.device
+        vec = E0.dim() == 1 and flat
+        B = E0.reshape(self.d * self.N, -1).to(device=dev, dtype=torch.complex128).contiguous().clone()
+        E.getrs(M, ipiv, B)
+        E.check_barriers(dev)
+        self._raise_singular(info)
+        if vec:
+            B = B.reshape(-1)
+        return _like(B, E0)
+
+    def _propagate(self, points, src, k0, alpha, E0, regularize, radius):
+        dev = self._dev()
+        npts = points.shape[0]
+        T = E0.shape[-1] if E0.dim() > 1 else 1
+        out = E0.to(device=dev, dtype=torch.complex128).reshape(self.d * npts, -1).contiguous().clone()
+        Ek = src.to(device=dev, dtype=torch.complex128).reshape(self.d * self.N, -1).contiguous()
+        flags = (L.MG_F_REGULARIZE if regularize else 0) | self._bf()
+        E.propagate(self.kind, points.to(dev), self._r(), k0, alpha * k0 * k0, Ek, out, flags=flags,
+                    radius=radius)
+        if self.d == 1:
+            return out.reshape(npts, -1) if E0.dim() > 1 else out.reshape(npts), T
+        return out.reshape(npts, self.d, -1), T
+
+    def _fix_rows(self, Ek_, points, flat_src, reg_src, regularize, radius):
+        """Rows whose first entry is NaN take a source row (Transmission3D.py:166-177)."""
+        probe = Ek_[:, 0] if Ek_.dim() == 2 else Ek_[:, 0, 0]
+        bad = torch.argwhere(torch.isnan(probe))
+        if bad.numel() == 0:
+            return Ek_
+        r = self.r.to(Ek_.device)
+        points = points.to(Ek_.device)
+        for j in bad:
+            if regularize:
+                dist = torch.linalg.norm(r - points[j], axis=-1)
+                possible = torch.nonzero(dist <= radius)
+                idx = torch.argmin(dist) if possible.shape[0] > 1 else possible
+                Ek_[j] = reg_src.to(Ek_.device)[idx]
+            else:
+                Ek_[j] = flat_src.to(Ek_.device)[torch.nonzero(torch.prod(r - points[j] == 0, axis=-1))]
+        return Ek_
+
+    # -- LDOS ------------------------------------------------------------------
+    def _ldos_dots(self, M, ipiv, pts, k0, radius, self_interaction, si_type, regularize, chunk=None):
+        """Row sums diag(G_p W G_p^T) for the measurement points, by solving
+        M X = G_p^T and contracting X with G_p^T column by column."""
+        dev = M.device
+        pts = pts.to(device=dev, dtype=torch.float64).contiguous()
+        fill = self._self_int(k0, radius, si_type) / self._volume(radius) if self_interaction else 0.0
+        flags = L.MG_F_NANFILL | (L.MG_F_REGULARIZE if regularize else 0) | self._bf()
+        n = self.d * self.N
+        if chunk is None:
+            # keep G_p^T and X within ~4 GiB per chunk
+            chunk = max(1, int((4 << 30) // (2 * 16 * n * self.d)))
+        out = []
+        for s in range(0, pts.shape[0], chunk):
+            p = pts[s:s + chunk]
+            GT, _ = E.assemble(self.kind, self._r(), p, k0, flags=flags, fill=fill, radius=radius)
+            X = GT.clone()
+            E.getrs(M, ipiv, X)
+            out.append(E.coldot(GT, X))
+            del GT, X
+        return torch.cat(out) if out else torch.zeros(0, dtype=torch.complex128, device=dev)
+
+    def _ldos_finish(self, dots, npts, pref, alpha, discard_absorption):
+        alpha_ = onp.real(alpha) if discard_absorption else alpha
+        ldos = torch.imag(dots * (pref * alpha_))
+        if self.kind == "2d_tm":
+            return ldos
+        if self.d == 1:
+            return torch.sum(ldos.reshape(npts, -1), 1)
+        return torch.sum(ldos.reshape(npts, self.d, -1), 1)
+
+    def _ldos(self, measure_points, k0, alpha, radius, self_interaction, si_type, regularize, discard_absorption,
+              pref):
+        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
+        dots = self._ldos_dots(M, ipiv, measure_points, k0, radius, self_interaction, si_type, regularize)
+        E.check_barriers(M.device)
+        self._raise_singular(info)
+        return _like(self._ldos_finish(dots, measure_points.shape[0], pref, alpha, discard_absorption),
+                     measure_points)
+
+    def _batch_ldos(self, batches, k0, alpha, radius, self_interaction, si_type, regularize, discard_absorption,
+                    pref):
+        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
+        outs = []
+        for b in range(len(batches)):
+            pts = batches[b]
+            dots = self._ldos_dots(M, ipiv, pts, k0, radius, self_interaction, si_type, regularize)
+            outs.append(self._ldos_finish(dots, pts.shape[0], pref, alpha, discard_absorption))
+        E.check_barriers(M.device)
+        self._raise_singular(info)
+        return _like(torch.cat(outs), batches[0] if len(batches) else None)
+
+    def _mean_dos(self, measure_points, k0, alpha, radius, self_interaction, si_type, regularize,
+                  discard_absorption, pref):
+        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
+        dots = self._ldos_dots(M, ipiv, measure_points, k0, radius, self_interaction, si_type, regularize)
+        E.check_barriers(M.device)
+        self._raise_singular(info)
+        alpha_ = onp.real(alpha) if discard_absorption else alpha
+        dos = dots.sum() / measure_points.shape[0]
+        dos = dos * (pref * alpha_)
+        return _like(torch.imag(dos), measure_points)
+
+    # -- out of the accelerated path ---------------------------------------------
+    def compute_hamiltonian_DOS(self, *args, **kwargs):
+        raise NotImplementedError("spectral methods (eigvals of G or M) are outside the accelerated path")
+
+    def compute_eigenvalues_and_scatterer_LDOS(self, *args, **kwargs):
+        raise NotImplementedError("spectral methods (eigvals of G or M) are outside the accelerated path")
+
+    def compute_eigenmodes_IPR(self, *args, **kwargs):
+        raise NotImplementedError("spectral methods (eig of M) are outside the accelerated path")

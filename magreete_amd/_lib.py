@@ -1,0 +1,77 @@
+"""ctypes binding of libmagreete_hip.so (the C ABI declared in include/magreete_hip.h).
+
+The library is required: importing the product without it raises, there is
+no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmagreete_hip.so")
+
+# enums (include/magreete_hip.h)
+MG_OK = 0
+MG_2D_TM, MG_2D_TE, MG_3D_VECTOR, MG_3D_SCALAR = 0, 1, 2, 3
+MG_C128, MG_C64, MG_BF16X3 = 0, 1, 2
+MG_F_REGULARIZE, MG_F_ZERO_SELF, MG_F_DIAG, MG_F_NANFILL = 1, 2, 4, 8
+MG_F_PERIODIC_X, MG_F_PERIODIC_Y, MG_F_PERIODIC_Z = 16, 32, 64
+MG_F_BESSEL_EXACT = 128
+
+_c = ctypes
+_i64 = _c.c_int64
+_dbl = _c.c_double
+_vp = _c.c_void_p
+_int = _c.c_int
+
+# name -> argtypes (every symbol returns int unless listed in _RESTYPES)
+SIGNATURES = {
+    "mg_version": [],
+    "mg_last_error": [],
+    "mg_create": [_c.POINTER(_vp)],
+    "mg_destroy": [_vp],
+    "mg_assemble": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _dbl,
+                    _vp, _i64, _vp, _i64, _vp],
+    "mg_cgemm": [_int, _i64, _i64, _i64, _dbl, _dbl, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _i64, _vp],
+    "mg_getrf": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp],
+    "mg_getrs": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp],
+    "mg_check": [_vp, _vp],
+    "mg_profile_enable": [_vp, _int],
+    "mg_profile_read": [_vp, _vp, _c.POINTER(_c.c_longlong), _c.POINTER(_dbl), _c.POINTER(_dbl)],
+    "mg_convert": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _vp],
+    "mg_propagate": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
+                     _vp, _i64, _vp],
+    "mg_hankel1": [_int, _int, _vp, _i64, _vp, _vp],
+    "mg_coldot": [_vp, _i64, _vp, _i64, _i64, _i64, _vp, _vp],
+    "mg_sources": [_int, _int, _vp, _i64, _vp, _vp, _i64, _dbl, _dbl, _vp, _i64, _vp],
+}
+_RESTYPES = {"mg_last_error": _c.c_char_p}
+
+_lib = None
+
+
+def load():
+    """Load the library once; raise if it is missing or lacks a symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmagreete_hip.so is not built (run magreete_amd/build.py or __graft_entry__.build())")
+    lib = _c.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, _int)
+    _lib = lib
+    return lib
+
+
+class MGError(RuntimeError):
+    def __init__(self, status, where):
+        msg = load().mg_last_error().decode(errors="replace")
+        super().__init__("%s failed with status %d: %s" % (where, status, msg))
+        self.status = status
+
+
+def check(status, where):
+    if status != MG_OK:
+        raise MGError(status, where)

@@ -1,0 +1,285 @@
+// Complex GEMM on CDNA MFMA: C = alpha * A * B + beta * C (row-major,
+// interleaved complex).  Complex products run as four real MFMA products
+// (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br) over LDS tiles that hold the
+// real and imaginary planes separately.
+//
+// The A tile comes from a loader policy: MemA reads a row-major matrix,
+// GreenA<KIND> evaluates the Green's function of (row point, column point)
+// pairs in FP64 straight into LDS, so the propagation operator is never
+// stored.
+#pragma once
+#include "mg_common.h"
+#include "green.h"
+#include <type_traits>
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Cplx;
+template <> struct Cplx<double> { typedef double2 t; typedef v4d acc; };
+template <> struct Cplx<float> { typedef float2 t; typedef v4f acc; };
+
+__device__ inline v4d mg_mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline v4f mg_mfma(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Map a linear workgroup id to (tile_m, tile_n) so that the tiles one XCD
+// works on are contiguous groups of rows: ids are dealt round-robin over the
+// 8 XCDs, so undo that first, then walk the tile grid in row groups of 8.
+__device__ inline void mg_tile_order(int bid, int nbm, int nbn, int& tm, int& tn) {
+  const int total = nbm * nbn;
+  int id = bid;
+  if (total >= 64 && (total % 8) == 0) {
+    const int per = total / 8;
+    id = (bid % 8) * per + bid / 8;
+  }
+  const int group = 8;
+  const int in_group = group * nbn;
+  const int g = id / in_group;
+  const int first_m = g * group;
+  const int gsize = min(nbm - first_m, group);
+  tm = first_m + (id % in_group) % gsize;
+  tn = (id % in_group) / gsize;
+}
+
+struct MemA {
+  const void* A;
+  int64_t lda;
+};
+
+template <int KIND>
+struct GreenA {
+  static constexpr int kind = KIND;
+  const double* rows;   // row points (space dim S each)
+  const double* cols;   // column points
+  int64_t nrows_pts, ncols_pts;
+  double k0;
+  int flags;
+  double radius;
+};
+
+// LDS planes for one stage: [re/im][BK][BM + PAD]
+template <typename T, int BM, int BN, int BK>
+struct GemmSmem {
+  T a[2][BK][BM + 1];
+  T b[2][BK][BN + 1];
+};
+
+template <typename T, int BM, int BK, int NT>
+struct ARegs {
+  static constexpr int PER = (BM * BK + NT - 1) / NT;
+  typename Cplx<T>::t v[PER];
+};
+
+// ---- A-tile staging: memory ----------------------------------------------
+template <typename T, int BM, int BK, int NT>
+__device__ inline void a_fetch(const MemA& src, int64_t M, int64_t K, int64_t i0, int64_t k0,
+                               ARegs<T, BM, BK, NT>& r) {
+  typedef typename Cplx<T>::t T2;
+  const T2* A = reinterpret_cast<const T2*>(src.A);
+#pragma unroll
+  for (int p = 0; p < r.PER; ++p) {
+    const int e = p * NT + threadIdx.x;
+    const int row = e / BK, kk = e % BK;
+    T2 v;
+    v.x = 0; v.y = 0;
+    if (e < BM * BK && i0 + row < M && k0 + kk < K) v = A[(i0 + row) * src.lda + k0 + kk];
+    r.v[p] = v;
+  }
+}
+
+template <typename T, int BM, int BN, int BK, int NT>
+__device__ inline void a_store(GemmSmem<T, BM, BN, BK>& s, const ARegs<T, BM, BK, NT>& r) {
+#pragma unroll
+  for (int p = 0; p < r.PER; ++p) {
+    const int e = p * NT + threadIdx.x;
+    if (e < BM * BK) {
+      const int row = e / BK, kk = e % BK;
+      s.a[0][kk][row] = r.v[p].x;
+      s.a[1][kk][row] = r.v[p].y;
+    }
+  }
+}
+
+// ---- A-tile staging: Green's function evaluated in FP64 ------------------
+// Tile rows i0..i0+BM are d*point + a, tile columns k0..k0+BK are
+// d*scatterer + b; BM and BK are multiples of d.
+template <int KIND, typename T, int BM, int BN, int BK, int NT>
+__device__ inline void a_green(GemmSmem<T, BM, BN, BK>& s, const GreenA<KIND>& src, int64_t i0, int64_t k0) {
+  constexpr int D = (KIND == MG_2D_TE) ? 2 : (KIND == MG_3D_VECTOR ? 3 : 1);
+  constexpr int SP = (KIND == MG_2D_TM || KIND == MG_2D_TE) ? 2 : 3;
+  constexpr int PM = BM / D, PK = BK / D;
+  static_assert(BM % D == 0 && BK % D == 0, "tile must hold whole blocks");
+  const int64_t p0 = i0 / D, q0 = k0 / D;
+  for (int e = threadIdx.x; e < PM * PK; e += NT) {
+    const int pm = e / PK, pk = e % PK;
+    const int64_t pi = p0 + pm, qj = q0 + pk;
+    mg_c g[D * D];
+    if (pi < src.nrows_pts && qj < src.ncols_pts) {
+      mg_green<KIND>(src.rows + pi * SP, src.cols + qj * SP, src.k0, src.flags, src.radius, g);
+    } else {
+#pragma unroll
+      for (int t = 0; t < D * D; ++t) g[t] = {0.0, 0.0};
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        s.a[0][pk * D + b][pm * D + a] = (T)g[a * D + b].re;
+        s.a[1][pk * D + b][pm * D + a] = (T)g[a * D + b].im;
+      }
+    }
+  }
+}
+
+// ---- B tile ----------------------------------------------------------------
+template <typename T, int BN, int BK, int NT>
+struct BRegs {
+  static constexpr int PER = (BN * BK + NT - 1) / NT;
+  typename Cplx<T>::t v[PER];
+};
+
+template <typename T, int BN, int BK, int NT>
+__device__ inline void b_fetch(const void* Bp, int64_t ldb, int64_t K, int64_t N, int64_t k0, int64_t j0,
+                               BRegs<T, BN, BK, NT>& r) {
+  typedef typename Cplx<T>::t T2;
+  const T2* B = reinterpret_cast<const T2*>(Bp);
+#pragma unroll
+  for (int p = 0; p < r.PER; ++p) {
+    const int e = p * NT + threadIdx.x;
+    const int kk = e / BN, col = e % BN;
+    T2 v;
+    v.x = 0; v.y = 0;
+    if (e < BN * BK && k0 + kk < K && j0 + col < N) v = B[(k0 + kk) * ldb + j0 + col];
+    r.v[p] = v;
+  }
+}
+
+template <typename T, int BM, int BN, int BK, int NT>
+__device__ inline void b_store(GemmSmem<T, BM, BN, BK>& s, const BRegs<T, BN, BK, NT>& r) {
+#pragma unroll
+  for (int p = 0; p < r.PER; ++p) {
+    const int e = p * NT + threadIdx.x;
+    if (e < BN * BK) {
+      const int kk = e / BN, col = e % BN;
+      s.b[0][kk][col] = r.v[p].x;
+      s.b[1][kk][col] = r.v[p].y;
+    }
+  }
+}
+
+// ---- the kernel body -------------------------------------------------------
+// WM x WN waves of 64 lanes; each wave owns (BM/WM) x (BN/WN) of C as
+// 16x16 MFMA tiles.
+template <typename T, int BM, int BN, int BK, int WM, int WN, typename ASrc>
+__device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx<T>::t alpha, const ASrc& asrc,
+                                  const void* Bp, int64_t ldb, typename Cplx<T>::t beta, void* Cp, int64_t ldc,
+                                  int tm, int tn) {
+  typedef typename Cplx<T>::t T2;
+  typedef typename Cplx<T>::acc ACC;
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  static_assert(BM % (WM * 16) == 0 && BN % (WN * 16) == 0 && BK % 4 == 0, "tile shape");
+  __shared__ GemmSmem<T, BM, BN, BK> sm[2];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
+  const int li = lane & 15, lk = lane >> 4;
+
+  ACC cr[TM][TN], ci[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0; ci[a][b][r] = 0; }
+
+  const int64_t KT = (K + BK - 1) / BK;
+  constexpr bool kMem = std::is_same<ASrc, MemA>::value;
+  ARegs<T, BM, BK, NT> ar;
+  BRegs<T, BN, BK, NT> br;
+
+  // prologue
+  if constexpr (kMem) {
+    a_fetch<T, BM, BK, NT>(asrc, M, K, i0, 0, ar);
+    a_store<T, BM, BN, BK, NT>(sm[0], ar);
+  } else {
+    a_green<ASrc::kind, T, BM, BN, BK, NT>(sm[0], asrc, i0, 0);
+  }
+  b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, 0, j0, br);
+  b_store<T, BM, BN, BK, NT>(sm[0], br);
+  __syncthreads();
+
+  for (int64_t kt = 0; kt < KT; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < KT;
+    if (more) {
+      if constexpr (kMem) a_fetch<T, BM, BK, NT>(asrc, M, K, i0, (kt + 1) * BK, ar);
+      b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, (kt + 1) * BK, j0, br);
+    }
+    GemmSmem<T, BM, BN, BK>& s = sm[cur];
+#pragma unroll
+    for (int g = 0; g < BK / 4; ++g) {
+      T are[TM], aim[TM], bre[TN], bim[TN];
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        are[a] = s.a[0][g * 4 + lk][wm * (BM / WM) + a * 16 + li];
+        aim[a] = s.a[1][g * 4 + lk][wm * (BM / WM) + a * 16 + li];
+      }
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        bre[b] = s.b[0][g * 4 + lk][wn * (BN / WN) + b * 16 + li];
+        bim[b] = s.b[1][g * 4 + lk][wn * (BN / WN) + b * 16 + li];
+      }
+#pragma unroll
+      for (int a = 0; a < TM; ++a) {
+        const T naim = -aim[a];
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          cr[a][b] = mg_mfma(are[a], bre[b], cr[a][b]);
+          cr[a][b] = mg_mfma(naim, bim[b], cr[a][b]);
+          ci[a][b] = mg_mfma(are[a], bim[b], ci[a][b]);
+          ci[a][b] = mg_mfma(aim[a], bre[b], ci[a][b]);
+        }
+      }
+    }
+    if (more) {
+      GemmSmem<T, BM, BN, BK>& nx = sm[cur ^ 1];
+      if constexpr (kMem) a_store<T, BM, BN, BK, NT>(nx, ar);
+      else a_green<ASrc::kind, T, BM, BN, BK, NT>(nx, asrc, i0, (kt + 1) * BK);
+      b_store<T, BM, BN, BK, NT>(nx, br);
+    }
+    __syncthreads();
+  }
+
+  // epilogue: C = alpha*acc + beta*C
+  T2* C = reinterpret_cast<T2*>(Cp);
+  const bool beta0 = (beta.x == 0 && beta.y == 0);
+#pragma unroll
+  for (int a = 0; a < TM; ++a) {
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const int64_t col = j0 + wn * (BN / WN) + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = i0 + wm * (BM / WM) + a * 16 + lk * 4 + r;
+        if (row < M && col < N) {
+          const T xr = cr[a][b][r], xi = ci[a][b][r];
+          T2 out;
+          out.x = alpha.x * xr - alpha.y * xi;
+          out.y = alpha.x * xi + alpha.y * xr;
+          if (!beta0) {
+            const T2 c = C[row * ldc + col];
+            out.x += beta.x * c.x - beta.y * c.y;
+            out.y += beta.x * c.y + beta.y * c.x;
+          }
+          C[row * ldc + col] = out;
+        }
+      }
+    }
+  }
+}

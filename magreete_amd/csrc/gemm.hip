@@ -1,0 +1,100 @@
+// Complex GEMM launchers (FP64 and FP32 MFMA) and their C-ABI entry.
+#include "cgemm.h"
+#include "mg_internal.h"
+
+namespace {
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_cgemm(int64_t M, int64_t N, int64_t K,
+                                                        typename Cplx<T>::t alpha, MemA a, const void* B,
+                                                        int64_t ldb, typename Cplx<T>::t beta, void* C,
+                                                        int64_t ldc, int nbm, int nbn) {
+  int tm, tn;
+  mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA>(M, N, K, alpha, a, B, ldb, beta, C, ldc, tm, tn);
+}
+
+// The same GEMM under its own symbol for the LU trailing update, so profiles
+// separate it from the small GEMMs inside panels and triangular solves.
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_cgemm_trail(int64_t M, int64_t N, int64_t K,
+                                                              typename Cplx<T>::t alpha, MemA a, const void* B,
+                                                              int64_t ldb, typename Cplx<T>::t beta, void* C,
+                                                              int64_t ldc, int nbm, int nbn) {
+  int tm, tn;
+  mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA>(M, N, K, alpha, a, B, ldb, beta, C, ldc, tm, tn);
+}
+
+template <typename T>
+struct GemmCfg;
+template <>
+struct GemmCfg<double> {
+  static constexpr int BM = 64, BN = 64, BK = 8, WM = 2, WN = 2;
+};
+template <>
+struct GemmCfg<float> {
+  static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2;
+};
+
+}  // namespace
+
+namespace mg {
+
+template <typename T>
+int cgemm(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+          cd beta, void* C, int64_t ldc, hipStream_t s) {
+  if (M < 0 || N < 0 || K < 0) return MG_ERR_ARG;
+  if (M == 0 || N == 0) return MG_OK;
+  typedef GemmCfg<T> G;
+  const int nbm = (int)((M + G::BM - 1) / G::BM), nbn = (int)((N + G::BN - 1) / G::BN);
+  typename Cplx<T>::t al, be;
+  al.x = (T)alpha.re; al.y = (T)alpha.im;
+  be.x = (T)beta.re; be.y = (T)beta.im;
+  MemA a{A, lda};
+  hipLaunchKernelGGL((k_cgemm<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn), dim3(64 * G::WM * G::WN), 0,
+                     s, M, N, K, al, a, B, ldb, be, C, ldc, nbm, nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+template <typename T>
+int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                cd beta, void* C, int64_t ldc, hipStream_t s) {
+  if (M < 0 || N < 0 || K < 0) return MG_ERR_ARG;
+  if (M == 0 || N == 0) return MG_OK;
+  typedef GemmCfg<T> G;
+  const int nbm = (int)((M + G::BM - 1) / G::BM), nbn = (int)((N + G::BN - 1) / G::BN);
+  typename Cplx<T>::t al, be;
+  al.x = (T)alpha.re; al.y = (T)alpha.im;
+  be.x = (T)beta.re; be.y = (T)beta.im;
+  MemA a{A, lda};
+  hipLaunchKernelGGL((k_cgemm_trail<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn),
+                     dim3(64 * G::WM * G::WN), 0, s, M, N, K, al, a, B, ldb, be, C, ldc, nbm, nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+template int cgemm_trail<double>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd,
+                                 void*, int64_t, hipStream_t);
+template int cgemm_trail<float>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd,
+                                void*, int64_t, hipStream_t);
+template int cgemm<double>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,
+                           int64_t, hipStream_t);
+template int cgemm<float>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,
+                          int64_t, hipStream_t);
+
+}  // namespace mg
+
+extern "C" int mg_cgemm(int prec, int64_t m, int64_t n, int64_t k, double alpha_re, double alpha_im, const void* A,
+                        int64_t lda, const void* B, int64_t ldb, double beta_re, double beta_im, void* C,
+                        int64_t ldc, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  mg::cd al{alpha_re, alpha_im}, be{beta_re, beta_im};
+  switch (prec) {
+    case MG_C128: return mg::cgemm<double>(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
+    case MG_C64: return mg::cgemm<float>(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
+    case MG_BF16X3: return mg::cgemm_bf16x3(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
+    default: return MG_ERR_ARG;
+  }
+}

@@ -1,0 +1,67 @@
+// LU factorisation and solve drivers (internal).
+#pragma once
+#include "mg_internal.h"
+#include <vector>
+
+namespace mg {
+
+constexpr int kLeaf = 32;              // widest leaf factored by the cooperative kernel
+constexpr int kMaxGetf2Blocks = 256;   // workgroups of one leaf launch (<= one per CU)
+constexpr int kGetf2MinRows = 64;      // aim for at least this many rows per workgroup
+constexpr int kGetf2Lds = 48 * 1024;   // dynamic LDS per leaf workgroup (static use stays under 64 KB total)
+
+// Cross-workgroup exchange area of the leaf kernel (double-buffered by
+// column parity).
+struct Getf2Work {
+  double* cand_val = nullptr;   // [2][G] best |re|+|im| per workgroup
+  int* cand_row = nullptr;      // [2][G] its row
+  void* cand_data = nullptr;    // [2][G][kLeaf] the candidate rows
+  void* rowj = nullptr;         // [2][kLeaf] row j before the swap
+  unsigned* bar = nullptr;      // barrier counter
+  int* err = nullptr;           // barrier failure flag
+};
+
+// Optional timing of the LU trailing-update launches with HIP events on the
+// launching stream: algorithmic flops (8 m n k per complex GEMM) and elapsed
+// time per launch.
+struct UpdateProfile {
+  bool enabled = false;
+  std::vector<hipEvent_t> ev;   // start/stop pairs, reused
+  std::vector<double> flops;    // per recorded launch
+  size_t used = 0;
+};
+
+struct Workspace {
+  bool ready = false;
+  int cus = 0;
+  int coop_blocks = 0;
+  Getf2Work getf2;
+  UpdateProfile prof;
+};
+
+typedef int (*GemmFn)(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*, int64_t,
+                      hipStream_t);
+
+// Sum of recorded update launches since the last read (synchronises the stream).
+int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms);
+
+int workspace_init(Workspace& ws);
+void workspace_free(Workspace& ws);
+// Synchronises the stream and reports a failed grid barrier.
+int workspace_check(Workspace& ws, hipStream_t s);
+
+template <typename T>
+int laswp(void* A, int64_t lda, int64_t c0, int64_t c1, int64_t k1, int64_t k2, const int* ipiv, hipStream_t s);
+template <typename T>
+int trsm_lower_unit(const void* L, int64_t ldl, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s);
+template <typename T>
+int trsm_upper(const void* U, int64_t ldu, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s);
+// update: GEMM used for the trailing update (nullptr: the FP64/FP32 MFMA GEMM of T).
+template <typename T>
+int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, hipStream_t s, int nb,
+          GemmFn update);
+template <typename T>
+int getrs(const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb, int64_t nrhs,
+          hipStream_t s);
+
+}  // namespace mg

@@ -1,0 +1,494 @@
+// Blocked complex LU with partial pivoting (getrf) and the multi-RHS solve
+// (getrs), row-major, for complex double and complex float.
+//
+// getrf is right-looking with an outer block of NB columns.  Each NB-wide
+// panel is factored recursively (left half, swap + triangular solve + GEMM
+// on the right half, right half) down to leaves of at most 32 columns; a
+// leaf is factored by k_getf2, a cooperative kernel whose workgroups each
+// keep a slice of the leaf's rows in LDS and meet at one grid barrier per
+// column.  The trailing update is the MFMA complex GEMM.  Pivoting follows
+// LAPACK: the pivot of column j is the row of largest |re| + |im| on or
+// below the diagonal, lowest row on ties; info is the first zero pivot
+// (1-based); the factors overwrite A with unit-lower L and upper U, and
+// ipiv[k] is the (0-based) row swapped with row k.
+#include "mg_internal.h"
+#include "lu.h"
+#include <climits>
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr long long kSpinLimit = 1ll << 25;   // ~ seconds of s_sleep before a barrier gives up
+
+template <typename T> struct C2;
+template <> struct C2<double> { typedef double2 t; };
+template <> struct C2<float> { typedef float2 t; };
+
+template <typename T2>
+__device__ inline T2 crecip(T2 a) {
+  // Smith's method for 1/(x + iy)
+  T2 r;
+  if (fabs(a.x) >= fabs(a.y)) {
+    const auto q = a.y / a.x, d = a.x + a.y * q;
+    r.x = 1 / d; r.y = -q / d;
+  } else {
+    const auto q = a.x / a.y, d = a.y + a.x * q;
+    r.x = q / d; r.y = -1 / d;
+  }
+  return r;
+}
+
+template <typename T2>
+__device__ inline T2 cmul(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+__device__ inline bool better(double v, int r, double bv, int br) { return v > bv || (v == bv && r < br); }
+
+// (value, row) arg-max over the workgroup; result valid in every thread.
+__device__ inline void block_argmax(double& v, int& r, double* sv, int* sr) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const int orow = __shfl_xor(r, off);
+    if (better(ov, orow, v, r)) { v = ov; r = orow; }
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { sv[wave] = v; sr[wave] = r; }
+  __syncthreads();
+  v = sv[0]; r = sr[0];
+#pragma unroll
+  for (int w = 1; w < kThreads / 64; ++w)
+    if (better(sv[w], sr[w], v, r)) { v = sv[w]; r = sr[w]; }
+  __syncthreads();
+}
+
+// Grid barrier over all workgroups of a cooperative launch.  The counter
+// starts at 0; barrier number j waits for (j+1)*G arrivals.  A wait that
+// exceeds the spin limit, or sees another workgroup's failure flag, sets the
+// flag and returns, so every workgroup always reaches the end of the kernel.
+__device__ inline void grid_barrier(unsigned* bar, unsigned target, int* err) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    long long spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+// Leaf LU of an m x w panel (w <= kLeaf) at A (row-major, lda).  Workgroup g
+// owns panel rows [g R, g R + R).  ipiv and info use global indices (row0 is
+// the panel's global row, which equals its global column).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restrict__ A, int64_t lda, int m, int w,
+                                                   int R, int64_t row0, int* __restrict__ ipiv,
+                                                   int* __restrict__ info, mg::Getf2Work wk) {
+  typedef typename C2<T>::t T2;
+  extern __shared__ unsigned char smem_raw[];
+  T2* tile = reinterpret_cast<T2*>(smem_raw);
+  __shared__ T2 prow[mg::kLeaf], orow[mg::kLeaf];
+  __shared__ double sv[kThreads / 64];
+  __shared__ int sr[kThreads / 64];
+
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int rb = g * R, re = min(m, rb + R);
+  const int nr = re > rb ? re - rb : 0;
+  T2* cdata = reinterpret_cast<T2*>(wk.cand_data);
+  T2* rowj = reinterpret_cast<T2*>(wk.rowj);
+
+  for (int e = tid; e < nr * w; e += kThreads) tile[e] = A[(int64_t)(rb + e / w) * lda + e % w];
+  __syncthreads();
+
+  for (int j = 0; j < w; ++j) {
+    const int par = j & 1;
+    double best = -1.0;
+    int brow = INT_MAX;
+    for (int lr = tid; lr < nr; lr += kThreads) {
+      const int grow = rb + lr;
+      if (grow >= j) {
+        const T2 v = tile[lr * w + j];
+        const double a = fabs((double)v.x) + fabs((double)v.y);
+        if (a > best) { best = a; brow = grow; }
+      }
+    }
+    block_argmax(best, brow, sv, sr);
+    if (tid == 0) {
+      wk.cand_val[par * G + g] = best;
+      wk.cand_row[par * G + g] = brow;
+    }
+    if (brow != INT_MAX && tid < w) cdata[(par * G + g) * mg::kLeaf + tid] = tile[(brow - rb) * w + tid];
+    if (j >= rb && j < re && tid < w) rowj[par * mg::kLeaf + tid] = tile[(j - rb) * w + tid];
+    if (G > 1) grid_barrier(wk.bar, (unsigned)(j + 1) * (unsigned)G, wk.err);
+    else { __threadfence(); __syncthreads(); }
+
+    double pv = -1.0;
+    int p = INT_MAX;
+    if (tid < G) { pv = wk.cand_val[par * G + tid]; p = wk.cand_row[par * G + tid]; }
+    block_argmax(pv, p, sv, sr);
+    if (p == INT_MAX) p = j;   // whole column NaN: keep row j, NaN propagates
+    const int pwg = p / R;
+    if (tid < w) {
+      orow[tid] = rowj[par * mg::kLeaf + tid];
+      prow[tid] = (p == j) ? orow[tid] : cdata[(par * G + pwg) * mg::kLeaf + tid];
+    }
+    __syncthreads();
+    if (tid < w) {
+      if (j >= rb && j < re) tile[(j - rb) * w + tid] = prow[tid];
+      if (p != j && p >= rb && p < re) tile[(p - rb) * w + tid] = orow[tid];
+    }
+    if (g == 0 && tid == 0) {
+      ipiv[row0 + j] = (int)(row0 + p);
+      if (pv == 0.0) atomicCAS(info, 0, (int)(row0 + j + 1));
+    }
+    __syncthreads();
+    const T2 piv = prow[j];
+    if (piv.x != 0 || piv.y != 0) {
+      const T2 inv = crecip(piv);
+      for (int lr = tid; lr < nr; lr += kThreads)
+        if (rb + lr > j) tile[lr * w + j] = cmul(tile[lr * w + j], inv);
+    }
+    __syncthreads();
+    const int wr = w - j - 1;
+    if (wr > 0) {
+      for (int e = tid; e < nr * wr; e += kThreads) {
+        const int lr = e / wr, c = j + 1 + e % wr;
+        if (rb + lr > j) {
+          const T2 l = tile[lr * w + j];
+          T2 x = tile[lr * w + c];
+          const T2 u = prow[c];
+          x.x -= l.x * u.x - l.y * u.y;
+          x.y -= l.x * u.y + l.y * u.x;
+          tile[lr * w + c] = x;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nr * w; e += kThreads) A[(int64_t)(rb + e / w) * lda + e % w] = tile[e];
+}
+
+__global__ void k_zero_ints(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Apply the row swaps k1..k2-1 (in order) to columns [c0, c1).
+template <typename T2>
+__global__ __launch_bounds__(256) void k_laswp(T2* __restrict__ A, int64_t lda, int64_t c0, int64_t c1, int64_t k1,
+                                                int64_t k2, const int* __restrict__ ipiv) {
+  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  for (int64_t i = k1; i < k2; ++i) {
+    const int64_t p = ipiv[i];
+    if (p != i) {
+      const T2 a = A[i * lda + c];
+      A[i * lda + c] = A[p * lda + c];
+      A[p * lda + c] = a;
+    }
+  }
+}
+
+// Triangular solve of an nb x nb (nb <= 32) block against ncols right-hand
+// sides, one column per thread: unit-lower forward or non-unit upper backward.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void k_trsm_leaf(const typename C2<T>::t* __restrict__ L, int64_t ldl, int nb,
+                                                   typename C2<T>::t* __restrict__ B, int64_t ldb, int64_t ncols) {
+  typedef typename C2<T>::t T2;
+  __shared__ T2 Ls[mg::kLeaf][mg::kLeaf + 1];
+  __shared__ T2 dinv[mg::kLeaf];
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) Ls[e / nb][e % nb] = L[(e / nb) * ldl + e % nb];
+  __syncthreads();
+  if (UPPER && threadIdx.x < nb) dinv[threadIdx.x] = crecip(Ls[threadIdx.x][threadIdx.x]);
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols) return;
+  T2 x[mg::kLeaf];
+#pragma unroll
+  for (int i = 0; i < mg::kLeaf; ++i)
+    if (i < nb) x[i] = B[i * ldb + c];
+  if (!UPPER) {
+#pragma unroll
+    for (int i = 1; i < mg::kLeaf; ++i) {
+      if (i < nb) {
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+          const T2 l = Ls[i][k];
+          x[i].x -= l.x * x[k].x - l.y * x[k].y;
+          x[i].y -= l.x * x[k].y + l.y * x[k].x;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = mg::kLeaf - 1; i >= 0; --i) {
+      if (i < nb) {
+#pragma unroll
+        for (int k = i + 1; k < mg::kLeaf; ++k) {
+          if (k < nb) {
+            const T2 u = Ls[i][k];
+            x[i].x -= u.x * x[k].x - u.y * x[k].y;
+            x[i].y -= u.x * x[k].y + u.y * x[k].x;
+          }
+        }
+        x[i] = cmul(x[i], dinv[i]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < mg::kLeaf; ++i)
+    if (i < nb) B[i * ldb + c] = x[i];
+}
+
+inline int64_t split_half(int64_t w) {
+  // left part: half, rounded up to a multiple of 16 (keeps GEMM tiles whole)
+  int64_t h = ((w / 2) + 15) / 16 * 16;
+  if (h >= w) h = w / 2;
+  return h > 0 ? h : 1;
+}
+
+}  // namespace
+
+namespace mg {
+
+template <typename T>
+int laswp(void* A, int64_t lda, int64_t c0, int64_t c1, int64_t k1, int64_t k2, const int* ipiv, hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  if (c1 <= c0 || k2 <= k1) return MG_OK;
+  const int64_t nb = (c1 - c0 + 255) / 256;
+  hipLaunchKernelGGL(k_laswp<T2>, dim3((unsigned)nb), dim3(256), 0, s, reinterpret_cast<T2*>(A), lda, c0, c1, k1, k2,
+                     ipiv);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+template <typename T>
+int trsm_lower_unit(const void* L, int64_t ldl, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  if (nb <= 0 || ncols <= 0) return MG_OK;
+  if (nb <= kLeaf) {
+    hipLaunchKernelGGL((k_trsm_leaf<T, false>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const T2*>(L), ldl, (int)nb, reinterpret_cast<T2*>(B), ldb, ncols);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+  }
+  const int64_t n1 = split_half(nb);
+  const T2* Lp = reinterpret_cast<const T2*>(L);
+  T2* Bp = reinterpret_cast<T2*>(B);
+  MG_TRY(trsm_lower_unit<T>(Lp, ldl, n1, Bp, ldb, ncols, s));
+  MG_TRY(cgemm<T>(nb - n1, ncols, n1, cd{-1, 0}, Lp + n1 * ldl, ldl, Bp, ldb, cd{1, 0}, Bp + n1 * ldb, ldb, s));
+  return trsm_lower_unit<T>(Lp + n1 * ldl + n1, ldl, nb - n1, Bp + n1 * ldb, ldb, ncols, s);
+}
+
+template <typename T>
+int trsm_upper(const void* U, int64_t ldu, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  if (nb <= 0 || ncols <= 0) return MG_OK;
+  if (nb <= kLeaf) {
+    hipLaunchKernelGGL((k_trsm_leaf<T, true>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const T2*>(U), ldu, (int)nb, reinterpret_cast<T2*>(B), ldb, ncols);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+  }
+  const int64_t n1 = split_half(nb);
+  const T2* Up = reinterpret_cast<const T2*>(U);
+  T2* Bp = reinterpret_cast<T2*>(B);
+  MG_TRY(trsm_upper<T>(Up + n1 * ldu + n1, ldu, nb - n1, Bp + n1 * ldb, ldb, ncols, s));
+  MG_TRY(cgemm<T>(n1, ncols, nb - n1, cd{-1, 0}, Up + n1, ldu, Bp + n1 * ldb, ldb, cd{1, 0}, Bp, ldb, s));
+  return trsm_upper<T>(Up, ldu, n1, Bp, ldb, ncols, s);
+}
+
+template <typename T>
+static int getf2(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int64_t w, int* ipiv, int* info,
+                 hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  const int64_t bytes_per_row = w * (int64_t)sizeof(T2);
+  const int64_t max_rows = kGetf2Lds / bytes_per_row;
+  int G = (int)((m + kGetf2MinRows - 1) / kGetf2MinRows);
+  const int gmax = ws.coop_blocks;
+  if (G > gmax) G = gmax;
+  if (G < 1) G = 1;
+  int R = (int)((m + G - 1) / G);
+  if (R > max_rows) {
+    G = (int)((m + max_rows - 1) / max_rows);
+    if (G > gmax) {
+      set_error("getf2: panel of %lld rows does not fit %d workgroups", (long long)m, gmax);
+      return MG_ERR_UNSUPPORTED;
+    }
+    R = (int)((m + G - 1) / G);
+  }
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(ws.getf2.bar), 1);
+  T2* Ap = reinterpret_cast<T2*>(A) + k * lda + k;
+  int mi = (int)m, wi = (int)w;
+  Getf2Work wk = ws.getf2;
+  const size_t shmem = (size_t)R * bytes_per_row;
+  void* args[] = {&Ap, (void*)&lda, &mi, &wi, &R, (void*)&k, &ipiv, &info, &wk};
+  MG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_getf2<T>), dim3(G), dim3(kThreads), args,
+                                    shmem, s));
+  return MG_OK;
+}
+
+template <typename T>
+static int panel(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int64_t w, int* ipiv, int* info,
+                 hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  // leaf width: kLeaf while kMaxGetf2Blocks workgroups can hold the rows in LDS, else half
+  const int64_t full_rows = (int64_t)kMaxGetf2Blocks * (kGetf2Lds / (kLeaf * (int64_t)sizeof(T2)));
+  const int64_t lw = m <= full_rows ? kLeaf : kLeaf / 2;
+  if (w <= lw) return getf2<T>(ws, A, lda, k, m, w, ipiv, info, s);
+  T2* Ap = reinterpret_cast<T2*>(A);
+  const int64_t w1 = split_half(w);
+  MG_TRY(panel<T>(ws, A, lda, k, m, w1, ipiv, info, s));
+  MG_TRY(laswp<T>(A, lda, k + w1, k + w, k, k + w1, ipiv, s));
+  MG_TRY(trsm_lower_unit<T>(Ap + k * lda + k, lda, w1, Ap + k * lda + k + w1, lda, w - w1, s));
+  MG_TRY(cgemm<T>(m - w1, w - w1, w1, cd{-1, 0}, Ap + (k + w1) * lda + k, lda, Ap + k * lda + k + w1, lda, cd{1, 0},
+                  Ap + (k + w1) * lda + k + w1, lda, s));
+  MG_TRY(panel<T>(ws, A, lda, k + w1, m - w1, w - w1, ipiv, info, s));
+  return laswp<T>(A, lda, k, k + w1, k + w1, k + w, ipiv, s);
+}
+
+template <typename T>
+int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, hipStream_t s, int nb,
+          GemmFn update) {
+  typedef typename C2<T>::t T2;
+  if (n < 0 || lda < n) return MG_ERR_ARG;
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
+  if (n == 0) return MG_OK;
+  T2* Ap = reinterpret_cast<T2*>(A);
+  for (int64_t k = 0; k < n; k += nb) {
+    const int64_t jb = (n - k < nb) ? n - k : nb;
+    MG_TRY(panel<T>(ws, A, lda, k, n - k, jb, ipiv, info, s));
+    MG_TRY(laswp<T>(A, lda, 0, k, k, k + jb, ipiv, s));
+    MG_TRY(laswp<T>(A, lda, k + jb, n, k, k + jb, ipiv, s));
+    if (k + jb < n) {
+      MG_TRY(trsm_lower_unit<T>(Ap + k * lda + k, lda, jb, Ap + k * lda + k + jb, lda, n - k - jb, s));
+      const int64_t r = n - k - jb;
+      UpdateProfile& pr = ws.prof;
+      if (pr.enabled) {
+        if (pr.used * 2 + 2 > pr.ev.size()) {
+          hipEvent_t a, b;
+          MG_HIP(hipEventCreate(&a));
+          MG_HIP(hipEventCreate(&b));
+          pr.ev.push_back(a);
+          pr.ev.push_back(b);
+        }
+        MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
+      }
+      GemmFn fn = update ? update : &cgemm_trail<T>;
+      MG_TRY(fn(r, r, jb, cd{-1, 0}, Ap + (k + jb) * lda + k, lda, Ap + k * lda + k + jb, lda, cd{1, 0},
+                Ap + (k + jb) * lda + k + jb, lda, s));
+      if (pr.enabled) {
+        MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
+        if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
+        pr.flops[pr.used] = 8.0 * (double)r * (double)r * (double)jb;
+        ++pr.used;
+      }
+    }
+  }
+  return MG_OK;
+}
+
+template <typename T>
+int getrs(const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb, int64_t nrhs,
+          hipStream_t s) {
+  if (n < 0 || nrhs < 0 || lda < n || ldb < nrhs) return MG_ERR_ARG;
+  if (n == 0 || nrhs == 0) return MG_OK;
+  MG_TRY(laswp<T>(B, ldb, 0, nrhs, 0, n, ipiv, s));
+  MG_TRY(trsm_lower_unit<T>(LU, lda, n, B, ldb, nrhs, s));
+  return trsm_upper<T>(LU, lda, n, B, ldb, nrhs, s);
+}
+
+int workspace_init(Workspace& ws) {
+  if (ws.ready) return MG_OK;
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  MG_HIP(hipGetDeviceProperties(&prop, dev));
+  int per_cu_d = 0, per_cu_f = 0;
+  MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, reinterpret_cast<const void*>(&k_getf2<double>),
+                                                      kThreads, kGetf2Lds));
+  MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_f, reinterpret_cast<const void*>(&k_getf2<float>),
+                                                      kThreads, kGetf2Lds));
+  int per_cu = per_cu_d < per_cu_f ? per_cu_d : per_cu_f;
+  if (per_cu < 1) per_cu = 1;
+  // one workgroup per CU at most: the barrier cost grows with the count
+  ws.coop_blocks = prop.multiProcessorCount;
+  ws.cus = prop.multiProcessorCount;
+  (void)per_cu;
+  const int G = kMaxGetf2Blocks;
+  MG_HIP(hipMalloc(&ws.getf2.cand_val, sizeof(double) * 2 * G));
+  MG_HIP(hipMalloc(&ws.getf2.cand_row, sizeof(int) * 2 * G));
+  MG_HIP(hipMalloc(&ws.getf2.cand_data, sizeof(double2) * 2 * G * kLeaf));
+  MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kLeaf));
+  MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * 64));
+  MG_HIP(hipMalloc(&ws.getf2.err, sizeof(int) * 64));
+  MG_HIP(hipMemset(ws.getf2.err, 0, sizeof(int) * 64));
+  if (ws.coop_blocks > G) ws.coop_blocks = G;
+  ws.ready = true;
+  return MG_OK;
+}
+
+int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms) {
+  MG_HIP(hipStreamSynchronize(s));
+  double f = 0.0, t = 0.0;
+  for (size_t i = 0; i < ws.prof.used; ++i) {
+    float e = 0.f;
+    MG_HIP(hipEventElapsedTime(&e, ws.prof.ev[2 * i], ws.prof.ev[2 * i + 1]));
+    t += e;
+    f += ws.prof.flops[i];
+  }
+  if (launches) *launches = (long long)ws.prof.used;
+  if (flops) *flops = f;
+  if (ms) *ms = t;
+  ws.prof.used = 0;
+  return MG_OK;
+}
+
+void workspace_free(Workspace& ws) {
+  for (hipEvent_t e : ws.prof.ev) (void)hipEventDestroy(e);
+  ws.prof.ev.clear();
+  if (!ws.ready) return;
+  (void)hipFree(ws.getf2.cand_val);
+  (void)hipFree(ws.getf2.cand_row);
+  (void)hipFree(ws.getf2.cand_data);
+  (void)hipFree(ws.getf2.rowj);
+  (void)hipFree(ws.getf2.bar);
+  (void)hipFree(ws.getf2.err);
+  ws = Workspace();
+}
+
+int workspace_check(Workspace& ws, hipStream_t s) {
+  int err = 0;
+  MG_HIP(hipMemcpyAsync(&err, ws.getf2.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));
+  if (err) {
+    set_error("a grid barrier of the LU leaf did not complete");
+    return MG_ERR_SYNC;
+  }
+  return MG_OK;
+}
+
+template int laswp<double>(void*, int64_t, int64_t, int64_t, int64_t, int64_t, const int*, hipStream_t);
+template int laswp<float>(void*, int64_t, int64_t, int64_t, int64_t, int64_t, const int*, hipStream_t);
+template int trsm_lower_unit<double>(const void*, int64_t, int64_t, void*, int64_t, int64_t, hipStream_t);
+template int trsm_lower_unit<float>(const void*, int64_t, int64_t, void*, int64_t, int64_t, hipStream_t);
+template int trsm_upper<double>(const void*, int64_t, int64_t, void*, int64_t, int64_t, hipStream_t);
+template int trsm_upper<float>(const void*, int64_t, int64_t, void*, int64_t, int64_t, hipStream_t);
+template int getrf<double>(Workspace&, void*, int64_t, int64_t, int*, int*, hipStream_t, int, GemmFn);
+template int getrf<float>(Workspace&, void*, int64_t, int64_t, int*, int*, hipStream_t, int, GemmFn);
+template int getrs<double>(const void*, int64_t, int64_t, const int*, void*, int64_t, int64_t, hipStream_t);
+template int getrs<float>(const void*, int64_t, int64_t, const int*, void*, int64_t, int64_t, hipStream_t);
+
+}  // namespace mg

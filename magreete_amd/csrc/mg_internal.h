@@ -1,0 +1,52 @@
+// Internal C++ API shared by the translation units of libmagreete_hip.so.
+#pragma once
+#include "mg_common.h"
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <complex>
+
+#define MG_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      mg::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                \
+                    hipGetErrorString(e_));                                   \
+      return MG_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+
+#define MG_TRY(expr)                     \
+  do {                                   \
+    int s_ = (expr);                     \
+    if (s_ != MG_OK) return s_;          \
+  } while (0)
+
+namespace mg {
+
+void set_error(const char* fmt, ...);
+
+struct cd { double re, im; };
+
+// C = alpha A B + beta C for row-major complex matrices of element T2.
+template <typename T>
+int cgemm(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+          cd beta, void* C, int64_t ldc, hipStream_t s);
+
+// The same GEMM launched under a separate kernel symbol (LU trailing update).
+template <typename T>
+int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                cd beta, void* C, int64_t ldc, hipStream_t s);
+
+// C = alpha A B + beta C with A and B complex float, products as three
+// bf16 MFMA products each (hi*hi + hi*lo + lo*hi).
+int cgemm_bf16x3(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
+                 int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
+
+// Device properties of the current device (cached).
+struct DevInfo {
+  int cus = 0;
+  int coop_blocks_getf2 = 0;
+};
+DevInfo& dev_info();
+
+}  // namespace mg

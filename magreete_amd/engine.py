@@ -1,0 +1,216 @@
+"""Device-side operations of the coupled-dipole path, over libmagreete_hip.
+
+Every function takes and returns torch tensors on a ROCm device and launches
+HIP kernels on torch's current stream of that device.  Matrices are row-major
+complex128 (or complex64 for the factor precisions), with the reference's
+interleaved layout: row d*m + a, column d*j + b.
+"""
+import ctypes
+
+import torch
+
+from . import _lib as L
+
+KINDS = {"2d_tm": L.MG_2D_TM, "2d_te": L.MG_2D_TE, "3d_vector": L.MG_3D_VECTOR, "3d_scalar": L.MG_3D_SCALAR}
+DIM = {"2d_tm": 1, "2d_te": 2, "3d_vector": 3, "3d_scalar": 1}
+SPACE = {"2d_tm": 2, "2d_te": 2, "3d_vector": 3, "3d_scalar": 3}
+PRECS = {"c128": L.MG_C128, "c64": L.MG_C64, "bf16x3": L.MG_BF16X3}
+
+_handles = {}
+
+
+def device_of(t=None):
+    """Target device: the tensor's if it is on a GPU, else cuda:0."""
+    if t is not None and t.is_cuda:
+        return t.device
+    if not torch.cuda.is_available():
+        raise RuntimeError("magreete_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def handle(device):
+    """The library handle (LU workspace) of a device, created on first use."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    h = _handles.get(idx)
+    if h is None:
+        lib = L.load()
+        with torch.cuda.device(idx):
+            hp = ctypes.c_void_p()
+            L.check(lib.mg_create(ctypes.byref(hp)), "mg_create")
+        h = hp
+        _handles[idx] = h
+    return h
+
+
+def _pts(x, device):
+    return x.to(device=device, dtype=torch.float64).contiguous()
+
+
+def _ld(t):
+    if t.dim() != 2 or t.stride(1) != 1:
+        raise ValueError("expected a row-major 2-D tensor")
+    return t.stride(0)
+
+
+# ---------------------------------------------------------------------------
+def assemble(kind, rows, cols, k0, scale=1.0, flags=0, diag=0.0, fill=0.0, radius=0.0, out=None, out64=None,
+             want64=False):
+    """out[d i + a, d j + b] = scale * G_ab(rows[i] - cols[j]) with the MG_F_* fix-ups.
+
+    Returns (out_c128 or None, out_c64 or None)."""
+    lib = L.load()
+    dev = device_of(rows)
+    d = DIM[kind]
+    rows = _pts(rows, dev)
+    cols = _pts(cols, dev)
+    nr, nc = rows.shape[0], cols.shape[0]
+    if out is None and out64 is None:
+        out = torch.empty((d * nr, d * nc), dtype=torch.complex128, device=dev)
+    if want64 and out64 is None:
+        out64 = torch.empty((d * nr, d * nc), dtype=torch.complex64, device=dev)
+    scale, diag, fill = complex(scale), complex(diag), complex(fill)
+    with torch.cuda.device(dev):
+        st = lib.mg_assemble(KINDS[kind], _p(rows), nr, _p(cols), nc, float(k0), scale.real, scale.imag,
+                             diag.real, diag.imag, fill.real, fill.imag, int(flags), float(radius),
+                             _p(out), _ld(out) if out is not None else 0,
+                             _p(out64), _ld(out64) if out64 is not None else 0, _stream(dev))
+    L.check(st, "mg_assemble")
+    return out, out64
+
+
+def getrf(A, prec="c128"):
+    """In-place LU with partial pivoting; returns (ipiv int32 (n,), info int32 (1,)) on the device."""
+    lib = L.load()
+    dev = A.device
+    n = A.shape[0]
+    ipiv = torch.empty(n, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_getrf(handle(dev), PRECS[prec], _p(A), n, _ld(A), _p(ipiv), _p(info), _stream(dev))
+    L.check(st, "mg_getrf")
+    return ipiv, info
+
+
+def getrs(LU, ipiv, B, prec="c128"):
+    """In place: B <- A^{-1} B from getrf's factors."""
+    lib = L.load()
+    dev = LU.device
+    with torch.cuda.device(dev):
+        st = lib.mg_getrs(handle(dev), PRECS[prec], _p(LU), LU.shape[0], _ld(LU), _p(ipiv), _p(B), _ld(B),
+                          B.shape[1], _stream(dev))
+    L.check(st, "mg_getrs")
+    return B
+
+
+def profile_updates(device, on):
+    """Enable/disable HIP-event timing of the LU trailing-update launches."""
+    L.check(L.load().mg_profile_enable(handle(device), 1 if on else 0), "mg_profile_enable")
+
+
+def read_update_profile(device):
+    """(launches, algorithmic flops, milliseconds) of the trailing updates since the last read."""
+    n = ctypes.c_longlong()
+    f = ctypes.c_double()
+    ms = ctypes.c_double()
+    with torch.cuda.device(device):
+        L.check(L.load().mg_profile_read(handle(device), _stream(device), ctypes.byref(n), ctypes.byref(f),
+                                         ctypes.byref(ms)), "mg_profile_read")
+    return n.value, f.value, ms.value
+
+
+def check_barriers(device):
+    lib = L.load()
+    with torch.cuda.device(device):
+        L.check(lib.mg_check(handle(device), _stream(device)), "mg_check")
+
+
+def cgemm(A, B, C, alpha=1.0, beta=0.0, prec="c128"):
+    """C <- alpha A B + beta C (row-major)."""
+    lib = L.load()
+    dev = C.device
+    alpha, beta = complex(alpha), complex(beta)
+    with torch.cuda.device(dev):
+        st = lib.mg_cgemm(PRECS[prec], C.shape[0], C.shape[1], A.shape[1], alpha.real, alpha.imag, _p(A), _ld(A),
+                          _p(B), _ld(B), beta.real, beta.imag, _p(C), _ld(C), _stream(dev))
+    L.check(st, "mg_cgemm")
+    return C
+
+
+def convert(src, dst):
+    lib = L.load()
+    dev = src.device
+    direction = 0 if src.dtype == torch.complex128 else 1
+    with torch.cuda.device(dev):
+        st = lib.mg_convert(direction, _p(src), _ld(src), _p(dst), _ld(dst), src.shape[0], src.shape[1],
+                            _stream(dev))
+    L.check(st, "mg_convert")
+    return dst
+
+
+def propagate(kind, meas, scat, k0, alpha, Ek, out, flags=0, radius=0.0, beta=1.0):
+    """out <- alpha * G(meas, scat) @ Ek + beta * out, G evaluated in-kernel."""
+    lib = L.load()
+    dev = out.device
+    meas = _pts(meas, dev)
+    scat = _pts(scat, dev)
+    alpha, beta = complex(alpha), complex(beta)
+    with torch.cuda.device(dev):
+        st = lib.mg_propagate(KINDS[kind], _p(meas), meas.shape[0], _p(scat), scat.shape[0], float(k0),
+                              alpha.real, alpha.imag, int(flags), float(radius), _p(Ek), _ld(Ek), Ek.shape[1],
+                              beta.real, beta.imag, _p(out), _ld(out), _stream(dev))
+    L.check(st, "mg_propagate")
+    return out
+
+
+def coldot(Y, X):
+    """out[c] = sum_r Y[r, c] X[r, c] (no conjugation), complex128."""
+    lib = L.load()
+    dev = Y.device
+    out = torch.empty(Y.shape[1], dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_coldot(_p(Y), _ld(Y), _p(X), _ld(X), Y.shape[0], Y.shape[1], _p(out), _stream(dev))
+    L.check(st, "mg_coldot")
+    return out
+
+
+def hankel1(nu, x, exact=False):
+    """H_nu^(1)(x); exact=False gives the reference's torch values (see bessel_ref.h)."""
+    lib = L.load()
+    dev = device_of(x)
+    xd = x.to(device=dev, dtype=torch.float64).contiguous()
+    out = torch.empty(xd.shape, dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_hankel1(int(nu), 1 if exact else 0, _p(xd), xd.numel(), _p(out), _stream(dev))
+    L.check(st, "mg_hankel1")
+    return out
+
+
+def sources(space, beam, points, dirs, pol, k0, w):
+    """Incident plane waves (beam=False) or Gaussian beams at points.
+
+    dirs: (T, 2) of (cos(-theta), sin(-theta)) in 2D, (T, 3) directions in 3D;
+    pol: None for scalar fields, else (T, d) polarisation components.
+    Returns (M, T) or (M, d, T) complex128."""
+    lib = L.load()
+    dev = device_of(points)
+    pts = _pts(points, dev)
+    dirs = dirs.to(device=dev, dtype=torch.float64).contiguous()
+    T = dirs.shape[0]
+    d = 1 if pol is None else pol.shape[1]
+    if pol is not None:
+        pol = pol.to(device=dev, dtype=torch.float64).contiguous()
+    shape = (pts.shape[0], T) if pol is None else (pts.shape[0], d, T)
+    out = torch.empty(shape, dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_sources(space, 1 if beam else 0, _p(pts), pts.shape[0], _p(dirs), T, _p(pol), d, float(k0),
+                            float(w), _p(out), _stream(dev))
+    L.check(st, "mg_sources")
+    return out

@@ -1,0 +1,196 @@
+"""Drop-in classes (GPU) against the oracle (CPU restatement of the reference)
+on identical seeded inputs.  Fields and intensities: max over columns of
+||x - x_ref||_2 / ||x_ref||_2 <= 1e-10."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from magreete_amd import Transmission2D as T2
+from magreete_amd import Transmission3D as T3
+from magreete_amd import workloads as W
+from oracle import transmission as O
+
+DUT = {"2d_tm": T2.Transmission2D_scalar, "2d_te": T2.Transmission2D_vector,
+       "3d_vector": T3.Transmission3D_vector, "3d_scalar": T3.Transmission3D_scalar}
+TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def colerr(x, ref):
+    x = x.reshape(x.shape[0], -1, x.shape[-1]) if x.dim() == 3 else x
+    ref = ref.reshape(x.shape)
+    if x.dim() == 3:
+        x = x.reshape(-1, x.shape[-1])
+        ref = ref.reshape(-1, ref.shape[-1])
+    if x.dim() == 1:
+        x, ref = x[:, None], ref[:, None]
+    num = torch.linalg.norm(x - ref, dim=0)
+    den = torch.linalg.norm(ref, dim=0)
+    return (num / den).max().item()
+
+
+def case(kind, n, seed=0, T=8, nmeas=64):
+    rng = np.random.default_rng(seed)
+    if kind.startswith("2d"):
+        pts = W.unit_disk(n, rng)
+        th = 2 * np.pi * np.arange(T) / T
+        tm = 2 * np.pi * np.arange(nmeas) / nmeas
+        meas = 2.0 * np.stack([np.cos(tm), np.sin(tm)], axis=1)
+        dirs = dict(thetas=th)
+    else:
+        pts = W.unit_ball(n, rng)
+        u = W.fibonacci_sphere(T)
+        p1, _ = W.polarisation_pairs(u)
+        meas = 2.0 * W.fibonacci_sphere(nmeas)
+        dirs = dict(u=torch.from_numpy(u), p=torch.from_numpy(p1))
+    k0 = 1.0 / W.mean_nn_spacing(pts)
+    alpha = {"2d_tm": W.alpha_2d_tm, "2d_te": W.alpha_2d_te}.get(kind, W.alpha_3d)(k0)
+    return torch.from_numpy(pts), torch.from_numpy(meas), k0, alpha, dirs
+
+
+def sources(cls, kind, pts, k0, dirs, w=0.2):
+    if kind.startswith("2d"):
+        return cls.generate_source(pts, k0, dirs["thetas"], w)
+    if kind == "3d_scalar":
+        return cls.generate_source(pts, k0, dirs["u"], w)
+    return cls.generate_source(pts, k0, dirs["u"], dirs["p"], w)
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+@pytest.mark.parametrize("src", ["plane", "beam"])
+def test_sources(kind, src):
+    say("sources", kind, src)
+    pts, meas, k0, alpha, dirs = case(kind, 50)
+    a = sources(DUT[kind](pts, source=src), kind, meas, k0, dirs)
+    b = sources(O.CLASSES[kind](pts, source=src), kind, meas, k0, dirs)
+    assert a.shape == b.shape and a.device.type == "cpu"
+    assert (a - b).abs().max() < 1e-12
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+@pytest.mark.parametrize("si", [False, True])
+def test_solve_propagate(kind, si):
+    n = {"2d_tm": 700, "2d_te": 400, "3d_vector": 300, "3d_scalar": 600}[kind]
+    say("solve+propagate", kind, si, n)
+    pts, meas, k0, alpha, dirs = case(kind, n, T=12, nmeas=96)
+    dut, ora = DUT[kind](pts, source='plane'), O.CLASSES[kind](pts, source='plane')
+    E0s, E0m = sources(ora, kind, pts, k0, dirs), sources(ora, kind, meas, k0, dirs)
+    radius = 0.2 / k0
+    Ek = dut.solve(k0, alpha, radius, E0s, self_interaction=si)
+    Ek_ref = ora.solve(k0, alpha, radius, E0s, self_interaction=si)
+    assert Ek.shape == Ek_ref.shape
+    assert colerr(Ek, Ek_ref) < TOL
+    E = dut.propagate(meas, Ek, k0, alpha, E0m)
+    E_ref = ora.propagate(meas, Ek_ref, k0, alpha, E0m)
+    assert E.shape == E_ref.shape
+    assert colerr(E, E_ref) < TOL
+    I, I_ref = E.abs() ** 2, E_ref.abs() ** 2
+    if I.dim() == 3:
+        I, I_ref = I.sum(1), I_ref.sum(1)
+    assert colerr(I, I_ref) < TOL
+    Ess = dut.propagate_ss(meas, k0, alpha, E0m, E0s)
+    Ess_ref = ora.propagate_ss(meas, k0, alpha, E0m, E0s)
+    assert colerr(Ess, Ess_ref) < TOL
+
+
+def test_cfg_2d_tm_full_size():
+    """The CPU-runnable configuration at full size: N=1000, 64 plane waves, 1024 points."""
+    say("2d tm N=1000")
+    w = W.cfg_2d_tm()
+    dut, ora = T2.Transmission2D_scalar(w.points, 'plane'), O.Transmission2D_scalar(w.points, 'plane')
+    E0s = ora.generate_source(w.points, w.k0, w.thetas, 0.2)
+    E0m = ora.generate_source(w.measure, w.k0, w.thetas, 0.2)
+    Ek = dut.solve(w.k0, w.alpha, 0.0, E0s, self_interaction=False)
+    Ek_ref = ora.solve(w.k0, w.alpha, 0.0, E0s, self_interaction=False)
+    E = dut.propagate(w.measure, Ek, w.k0, w.alpha, E0m)
+    E_ref = ora.propagate(w.measure, Ek_ref, w.k0, w.alpha, E0m)
+    say("errors", colerr(Ek, Ek_ref), colerr(E, E_ref), colerr(E.abs() ** 2, E_ref.abs() ** 2))
+    assert colerr(Ek, Ek_ref) < TOL and colerr(E, E_ref) < TOL
+    assert colerr(E.abs() ** 2, E_ref.abs() ** 2) < TOL
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+def test_ldos_and_dos(kind):
+    n = {"2d_tm": 150, "2d_te": 90, "3d_vector": 60, "3d_scalar": 150}[kind]
+    say("ldos", kind, n)
+    pts, meas, k0, alpha, dirs = case(kind, n)
+    rng = np.random.default_rng(5)
+    probes = torch.from_numpy(0.8 * (W.unit_disk(20, rng) if kind.startswith("2d") else W.unit_ball(20, rng)))
+    probes = torch.cat([probes, pts[:6]])   # measurement points on scatterers: NaN fix-ups
+    dut, ora = DUT[kind](pts, source='plane'), O.CLASSES[kind](pts, source='plane')
+    radius = 0.2 / k0
+    for si in (False, True):
+        for da in (False, True):
+            a = dut.LDOS_measurements(probes, k0, alpha, radius, self_interaction=si, discard_absorption=da)
+            b = ora.LDOS_measurements(probes, k0, alpha, radius, self_interaction=si, discard_absorption=da)
+            assert a.shape == b.shape
+            assert (a - b).abs().max() <= 1e-9 * b.abs().max(), (si, da)
+    md = dut.mean_DOS_measurements(probes, k0, alpha, radius)
+    mr = ora.mean_DOS_measurements(probes, k0, alpha, radius)
+    assert abs(md.item() - mr.item()) <= 1e-8 * abs(mr.item())
+    bl = dut.batch_LDOS_measurements(torch.split(probes, 7), k0, alpha, radius)
+    br = ora.batch_LDOS_measurements(torch.split(probes, 7), k0, alpha, radius)
+    assert bl.shape == br.shape and (bl - br).abs().max() <= 1e-9 * br.abs().max()
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+def test_G0_and_greens(kind):
+    say("G0", kind)
+    pts, meas, k0, alpha, dirs = case(kind, 40)
+    dut, ora = DUT[kind](pts, source='plane'), O.CLASSES[kind](pts, source='plane')
+    g0 = "G0_vector" if kind == "2d_te" else "G0"
+    pairs = [(meas, {})]
+    if kind != "2d_tm":
+        pairs.append((None, {}))
+    else:
+        pairs.append((pts, {}))
+    for p, _ in pairs:
+        a = getattr(dut, g0)(p, k0)
+        b = getattr(ora, g0)(p, k0)
+        assert torch.equal(torch.isnan(a), torch.isnan(b))
+        m = ~torch.isnan(b)
+        assert ((a[m] - b[m]).abs() <= 1e-12 * b[m].abs() + 1e-15).all()
+    gname = "torch_greens" if kind.startswith("2d") else "greens"
+    r = meas.reshape(-1, 1, meas.shape[1]) - pts.reshape(1, -1, pts.shape[1])
+    a = getattr(dut, gname)(r.clone(), k0)
+    b = getattr(ora, gname)(r.clone(), k0)
+    assert a.shape == b.shape
+    assert ((a - b).abs() <= 1e-12 * b.abs() + 1e-15).all()
+    if kind.startswith("2d"):
+        x = torch.linspace(0.01, 40, 1000, dtype=torch.float64)
+        for nu in (0, 1):
+            assert ((dut.torch_hankel1(nu, x) - O.hankel1(nu, x)).abs() <= 1e-14 * O.hankel1(nu, x).abs()).all()
+
+
+@pytest.mark.parametrize("kind", ["2d_tm", "3d_vector"])
+def test_nan_fixup_rows(kind):
+    """Measurement points exactly on scatterers, with and without regularize."""
+    say("nan fixup", kind)
+    pts, meas, k0, alpha, dirs = case(kind, 80)
+    meas = torch.cat([meas[:10], pts[3:5], meas[10:20]])
+    dut, ora = DUT[kind](pts, source='plane'), O.CLASSES[kind](pts, source='plane')
+    E0s, E0m = sources(ora, kind, pts, k0, dirs), sources(ora, kind, meas, k0, dirs)
+    Ek = ora.solve(k0, alpha, 0.0, E0s, self_interaction=False)
+    for reg in (False, True):
+        a = dut.propagate(meas, Ek, k0, alpha, E0m, regularize=reg, radius=1e-3)
+        b = ora.propagate(meas, Ek, k0, alpha, E0m, regularize=reg, radius=1e-3)
+        assert not torch.isnan(b).any()
+        assert colerr(a, b) < TOL
+
+
+def test_exact_bessel_mode_is_closer_to_mpmath_green():
+    """bessel='exact' departs from the reference's torch values by the torch Bessel error only."""
+    say("exact mode")
+    pts, meas, k0, alpha, dirs = case("2d_tm", 200)
+    ref_mode = T2.Transmission2D_scalar(pts, 'plane')
+    ex = T2.Transmission2D_scalar(pts, 'plane')
+    ex.bessel = "exact"
+    a = ref_mode.G0(meas, k0)
+    b = ex.G0(meas, k0)
+    rel = ((a - b).abs() / b.abs()).max().item()
+    assert 1e-12 < rel < 3e-6

@@ -1,0 +1,188 @@
+"""HIP kernels through the C ABI against the oracle / CPU references."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from magreete_amd import _lib as L
+from magreete_amd import engine as E
+from magreete_amd import workloads as W
+from oracle import transmission as O
+
+DEV = "cuda:0"
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def crand(shape, rng, dtype=np.complex128):
+    return (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(dtype)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 33, 9), (64, 64, 8), (130, 70, 257), (300, 200, 100), (5, 700, 3)])
+def test_cgemm(prec, mnk):
+    m, n, k = mnk
+    say("cgemm", prec, mnk)
+    rng = np.random.default_rng(sum(mnk))
+    dt = np.complex128 if prec == "c128" else np.complex64
+    A, B, C = crand((m, k), rng, dt), crand((k, n), rng, dt), crand((m, n), rng, dt)
+    alpha, beta = 0.7 - 0.3j, -1.1 + 0.4j
+    ref = alpha * (A.astype(np.complex128) @ B.astype(np.complex128)) + beta * C.astype(np.complex128)
+    Cd = torch.from_numpy(C).to(DEV)
+    E.cgemm(torch.from_numpy(A).to(DEV), torch.from_numpy(B).to(DEV), Cd, alpha, beta, prec)
+    got = Cd.cpu().numpy().astype(np.complex128)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < (1e-14 if prec == "c128" else 2e-6), err
+
+
+def test_cgemm_strided_views():
+    say("cgemm strided")
+    rng = np.random.default_rng(7)
+    big = torch.from_numpy(crand((200, 300), rng)).to(DEV)
+    A = big[10:90, 20:60]
+    B = big[100:140, 5:105]
+    C = torch.zeros((80, 100), dtype=torch.complex128, device=DEV)
+    E.cgemm(A, B, C)
+    ref = A.cpu().numpy() @ B.cpu().numpy()
+    assert np.abs(C.cpu().numpy() - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("kind", ["2d_tm", "2d_te", "3d_vector", "3d_scalar"])
+@pytest.mark.parametrize("exact", [False, True])
+def test_assemble_vs_oracle(kind, exact):
+    say("assemble", kind, exact)
+    rng = np.random.default_rng(1)
+    sp = 2 if kind.startswith("2d") else 3
+    rows = rng.uniform(-1, 1, size=(37, sp))
+    cols = rng.uniform(-1, 1, size=(53, sp))
+    cols[5] = rows[3]  # one coincident pair -> NaN block
+    k0 = 11.3
+    rt, ct = torch.from_numpy(rows), torch.from_numpy(cols)
+    flags = L.MG_F_BESSEL_EXACT if exact else 0
+    G, G64 = E.assemble(kind, rt.to(DEV), ct.to(DEV), k0, flags=flags, want64=True)
+    cls = O.CLASSES[kind](ct, source='plane')
+    if kind == "2d_tm":
+        ref = cls.G0(rt, k0)
+    elif kind == "2d_te":
+        ref = cls.G0_vector(rt, k0)
+    else:
+        ref = cls.G0(rt, k0)
+    got = G.cpu()
+    assert torch.equal(torch.isnan(got), torch.isnan(ref))
+    m = ~torch.isnan(ref)
+    diff = (got[m] - ref[m]).abs()
+    scale = ref[m].abs()
+    tol = 3e-6 if (exact and kind.startswith("2d")) else 1e-12   # exact mode departs from torch's Bessel values
+    assert (diff <= tol * scale + 1e-15).all(), diff.max()
+    g64 = G64.cpu().to(torch.complex128)
+    assert ((g64[m] - got[m]).abs() <= 1e-6 * scale + 1e-30).all()
+
+
+@pytest.mark.parametrize("kind", ["2d_tm", "2d_te", "3d_vector", "3d_scalar"])
+def test_system_matrix_vs_oracle(kind):
+    """M = -alpha k0^2 G0(None), unit diagonal, self-interaction shift (drop-in _system)."""
+    say("system", kind)
+    from magreete_amd import Transmission2D as T2, Transmission3D as T3
+    rng = np.random.default_rng(2)
+    sp = 2 if kind.startswith("2d") else 3
+    pts = torch.from_numpy(rng.uniform(-1, 1, size=(40, sp)))
+    k0, radius = 9.0, 0.02
+    alpha = W.alpha_3d(k0) if kind.startswith("3d") else W.alpha_2d_tm(k0)
+    mod = {"2d_tm": T2.Transmission2D_scalar, "2d_te": T2.Transmission2D_vector,
+           "3d_vector": T3.Transmission3D_vector, "3d_scalar": T3.Transmission3D_scalar}[kind]
+    dut = mod(pts.to(DEV), source='plane')
+    ora = O.CLASSES[kind](pts, source='plane')
+    for si in (False, True):
+        M, _ = dut._system(k0, alpha, radius, si, "Rayleigh")
+        if kind == "2d_tm":
+            G = ora.G0(ora.r, k0)
+        elif kind == "2d_te":
+            G = ora.G0_vector(None, k0)
+        else:
+            G = ora.G0(None, k0)
+        S = ora.__class__  # noqa
+        ref = O._system_matrix(G.clone(), alpha, k0,
+                               {"2d_tm": O.self_interaction_integral_scalar_2d,
+                                "2d_te": O.self_interaction_integral_vector_2d,
+                                "3d_vector": O.self_interaction_integral_vector_3d,
+                                "3d_scalar": O.self_interaction_integral_scalar_3d}[kind](k0, radius) if si else 0,
+                               (np.pi * radius ** 2) if kind.startswith("2d") else 4 * np.pi * radius ** 3 / 3,
+                               si, order2d=kind.startswith("2d"))
+        got = M.cpu()
+        assert ((got - ref).abs() <= 1e-12 * ref.abs() + 1e-15).all()
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("n", [1, 7, 33, 100, 257, 600, 1100])
+def test_getrf_getrs(prec, n):
+    say("lu", prec, n)
+    rng = np.random.default_rng(n)
+    dt = np.complex128 if prec == "c128" else np.complex64
+    A = crand((n, n), rng) + 0.5 * np.sqrt(n) * np.eye(n)
+    B = crand((n, 5), rng)
+    Ad = torch.from_numpy(A.astype(dt)).to(DEV)
+    Bd = torch.from_numpy(B.astype(dt)).to(DEV)
+    ipiv, info = E.getrf(Ad, prec)
+    E.getrs(Ad, ipiv, Bd, prec)
+    E.check_barriers(Ad.device)
+    assert info.item() == 0
+    x = Bd.cpu().numpy().astype(np.complex128)
+    ref = np.linalg.solve(A, B)
+    err = np.linalg.norm(x - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    assert err.max() < (1e-12 if prec == "c128" else 1e-4), err.max()
+    # pivots are a valid LAPACK sequence
+    p = ipiv.cpu().numpy()
+    assert np.all(p >= np.arange(n)) and np.all(p < n)
+
+
+def test_getrf_matches_lapack_pivots():
+    say("lu pivots")
+    rng = np.random.default_rng(11)
+    n = 300
+    A = crand((n, n), rng)
+    Ad = torch.from_numpy(A).to(DEV)
+    ipiv, info = E.getrf(Ad)
+    import scipy.linalg
+    lu, piv = scipy.linalg.lu_factor(A)
+    assert np.array_equal(ipiv.cpu().numpy(), piv)
+    assert np.abs(Ad.cpu().numpy() - lu).max() < 1e-11 * np.abs(lu).max()
+
+
+def test_getrf_singular_info():
+    say("lu singular")
+    rng = np.random.default_rng(3)
+    n = 70
+    A = crand((n, n), rng)
+    A[:, 40] = 0.0
+    Ad = torch.from_numpy(A).to(DEV)
+    ipiv, info = E.getrf(Ad)
+    E.check_barriers(Ad.device)
+    assert info.item() == 41
+
+
+@pytest.mark.parametrize("kind", ["2d_tm", "2d_te", "3d_vector", "3d_scalar"])
+def test_propagate_kernel_vs_oracle(kind):
+    say("propagate", kind)
+    rng = np.random.default_rng(4)
+    sp = 2 if kind.startswith("2d") else 3
+    d = {"2d_tm": 1, "2d_te": 2, "3d_vector": 3, "3d_scalar": 1}[kind]
+    scat = torch.from_numpy(rng.uniform(-1, 1, size=(45, sp)))
+    meas = torch.from_numpy(rng.uniform(-2, 2, size=(29, sp)))
+    k0 = 7.5
+    T = 70
+    Ek = torch.from_numpy(crand((d * 45, T), rng))
+    E0 = torch.from_numpy(crand((d * 29, T), rng))
+    alpha = 0.3 - 0.1j
+    out = E0.clone().to(DEV)
+    E.propagate(kind, meas.to(DEV), scat.to(DEV), k0, alpha, Ek.to(DEV), out)
+    cls = O.CLASSES[kind](scat, source='plane')
+    if kind == "2d_te":
+        G = cls.G0_vector(meas, k0)
+    else:
+        G = cls.G0(meas, k0)
+    ref = alpha * (G @ Ek) + E0
+    err = (out.cpu() - ref).abs().max() / ref.abs().max()
+    assert err < 1e-13, err
