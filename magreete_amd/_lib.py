@@ -42,7 +42,7 @@ SIGNATURES = {
                      _vp, _i64, _vp],
     "mg_hankel1": [_int, _int, _vp, _i64, _vp, _vp],
     "mg_coldot": [_vp, _i64, _vp, _i64, _i64, _i64, _vp, _vp],
-    "mg_sources": [_int, _int, _vp, _i64, _vp, _vp, _i64, _dbl, _dbl, _vp, _i64, _vp],
+    "mg_sources": [_int, _int, _vp, _i64, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _vp],
 }
 _RESTYPES = {"mg_last_error": _c.c_char_p}
 

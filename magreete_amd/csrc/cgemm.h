@@ -256,9 +256,12 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
     __syncthreads();
   }
 
-  // epilogue: C = alpha*acc + beta*C
+  // epilogue: C = alpha*acc + beta*C.  Accumulator item r of lane l holds
+  // column l%16 and row l/16 + 4r (f64 MFMA) or 4(l/16) + r (f32 MFMA),
+  // as measured on gfx950.
   T2* C = reinterpret_cast<T2*>(Cp);
   const bool beta0 = (beta.x == 0 && beta.y == 0);
+  constexpr bool kRowsStrided = std::is_same<T, double>::value;
 #pragma unroll
   for (int a = 0; a < TM; ++a) {
 #pragma unroll
@@ -266,7 +269,7 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
       const int64_t col = j0 + wn * (BN / WN) + b * 16 + li;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = i0 + wm * (BM / WM) + a * 16 + lk * 4 + r;
+        const int64_t row = i0 + wm * (BM / WM) + a * 16 + (kRowsStrided ? lk + 4 * r : lk * 4 + r);
         if (row < M && col < N) {
           const T xr = cr[a][b][r], xi = ci[a][b][r];
           T2 out;

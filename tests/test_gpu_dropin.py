@@ -93,9 +93,16 @@ def test_solve_propagate(kind, si):
     if I.dim() == 3:
         I, I_ref = I.sum(1), I_ref.sum(1)
     assert colerr(I, I_ref) < TOL
-    Ess = dut.propagate_ss(meas, k0, alpha, E0m, E0s)
-    Ess_ref = ora.propagate_ss(meas, k0, alpha, E0m, E0s)
+    # the reference's 3D-vector propagate_ss multiplies G by E0_scat as given, so
+    # it only accepts the flat (3N, T) layout; the drop-in also takes (N, 3, T)
+    src = E0s.reshape(dut.d * dut.N, -1) if kind == "3d_vector" else E0s
+    Ess = dut.propagate_ss(meas, k0, alpha, E0m, src)
+    Ess_ref = ora.propagate_ss(meas, k0, alpha, E0m, src)
     assert colerr(Ess, Ess_ref) < TOL
+    if kind == "3d_vector":
+        with pytest.raises(RuntimeError):
+            ora.propagate_ss(meas, k0, alpha, E0m, E0s)
+        assert colerr(dut.propagate_ss(meas, k0, alpha, E0m, E0s), Ess_ref) < TOL
 
 
 def test_cfg_2d_tm_full_size():
