@@ -71,6 +71,14 @@ int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv,
  * solve step of torch.linalg.solve at the same call sites. */
 int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb,
              int64_t nrhs, void* stream);
+/* Stored inverses of the 256 x 256 diagonal blocks of L and U, so that the
+ * solve runs as GEMMs only (repeated solves against a resident factor; same
+ * call sites as mg_getrs).  dinv holds mg_dinv_elems(n) elements of the
+ * factor's type; work holds n x nrhs elements. */
+int64_t mg_dinv_elems(int64_t n);
+int mg_getrf_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, void* dinv, void* stream);
+int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv,
+                  void* B, int64_t ldb, int64_t nrhs, void* work, void* stream);
 /* Waits for `stream`; MG_ERR_SYNC if a grid barrier of an LU leaf failed. */
 int mg_check(mg_handle* h, void* stream);
 /* Measurement hooks with no reference counterpart: HIP-event timing of the

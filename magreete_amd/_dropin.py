@@ -99,8 +99,7 @@ class _Base:
 
     def _factor(self, k0, alpha, radius, self_interaction, si_type):
         M, _ = self._system(k0, alpha, radius, self_interaction, si_type)
-        ipiv, info = E.getrf(M)
-        return M, ipiv, info
+        return E.Factor(M)
 
     # -- G0 -------------------------------------------------------------------
     def _G0(self, points, k0, regularize=False, radius=0.0):
@@ -130,13 +129,13 @@ class _Base:
 
     # -- solve / propagate -----------------------------------------------------
     def _solve(self, k0, alpha, radius, E0, self_interaction, si_type, flat):
-        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
-        dev = M.device
+        F = self._factor(k0, alpha, radius, self_interaction, si_type)
+        dev = F.A.device
         vec = E0.dim() == 1 and flat
         B = E0.reshape(self.d * self.N, -1).to(device=dev, dtype=torch.complex128).contiguous().clone()
-        E.getrs(M, ipiv, B)
+        F.solve(B)
         E.check_barriers(dev)
-        self._raise_singular(info)
+        self._raise_singular(F.info)
         if vec:
             B = B.reshape(-1)
         return _like(B, E0)
@@ -173,10 +172,10 @@ class _Base:
         return Ek_
 
     # -- LDOS ------------------------------------------------------------------
-    def _ldos_dots(self, M, ipiv, pts, k0, radius, self_interaction, si_type, regularize, chunk=None):
+    def _ldos_dots(self, F, pts, k0, radius, self_interaction, si_type, regularize, chunk=None):
         """Row sums diag(G_p W G_p^T) for the measurement points, by solving
-        M X = G_p^T and contracting X with G_p^T column by column."""
-        dev = M.device
+        M X = G_p^T with the factor F and contracting X with G_p^T column by column."""
+        dev = F.A.device
         pts = pts.to(device=dev, dtype=torch.float64).contiguous()
         fill = self._self_int(k0, radius, si_type) / self._volume(radius) if self_interaction else 0.0
         flags = L.MG_F_NANFILL | (L.MG_F_REGULARIZE if regularize else 0) | self._bf()
@@ -189,7 +188,7 @@ class _Base:
             p = pts[s:s + chunk]
             GT, _ = E.assemble(self.kind, self._r(), p, k0, flags=flags, fill=fill, radius=radius)
             X = GT.clone()
-            E.getrs(M, ipiv, X)
+            F.solve(X)
             out.append(E.coldot(GT, X))
             del GT, X
         return torch.cat(out) if out else torch.zeros(0, dtype=torch.complex128, device=dev)
@@ -205,31 +204,31 @@ class _Base:
 
     def _ldos(self, measure_points, k0, alpha, radius, self_interaction, si_type, regularize, discard_absorption,
               pref):
-        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
-        dots = self._ldos_dots(M, ipiv, measure_points, k0, radius, self_interaction, si_type, regularize)
-        E.check_barriers(M.device)
-        self._raise_singular(info)
+        F = self._factor(k0, alpha, radius, self_interaction, si_type)
+        dots = self._ldos_dots(F, measure_points, k0, radius, self_interaction, si_type, regularize)
+        E.check_barriers(F.A.device)
+        self._raise_singular(F.info)
         return _like(self._ldos_finish(dots, measure_points.shape[0], pref, alpha, discard_absorption),
                      measure_points)
 
     def _batch_ldos(self, batches, k0, alpha, radius, self_interaction, si_type, regularize, discard_absorption,
                     pref):
-        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
+        F = self._factor(k0, alpha, radius, self_interaction, si_type)
         outs = []
         for b in range(len(batches)):
             pts = batches[b]
-            dots = self._ldos_dots(M, ipiv, pts, k0, radius, self_interaction, si_type, regularize)
+            dots = self._ldos_dots(F, pts, k0, radius, self_interaction, si_type, regularize)
             outs.append(self._ldos_finish(dots, pts.shape[0], pref, alpha, discard_absorption))
-        E.check_barriers(M.device)
-        self._raise_singular(info)
+        E.check_barriers(F.A.device)
+        self._raise_singular(F.info)
         return _like(torch.cat(outs), batches[0] if len(batches) else None)
 
     def _mean_dos(self, measure_points, k0, alpha, radius, self_interaction, si_type, regularize,
                   discard_absorption, pref):
-        M, ipiv, info = self._factor(k0, alpha, radius, self_interaction, si_type)
-        dots = self._ldos_dots(M, ipiv, measure_points, k0, radius, self_interaction, si_type, regularize)
-        E.check_barriers(M.device)
-        self._raise_singular(info)
+        F = self._factor(k0, alpha, radius, self_interaction, si_type)
+        dots = self._ldos_dots(F, measure_points, k0, radius, self_interaction, si_type, regularize)
+        E.check_barriers(F.A.device)
+        self._raise_singular(F.info)
         alpha_ = onp.real(alpha) if discard_absorption else alpha
         dos = dots.sum() / measure_points.shape[0]
         dos = dos * (pref * alpha_)

@@ -35,6 +35,9 @@ SIGNATURES = {
     "mg_getrf": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp],
     "mg_getrs": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp],
     "mg_check": [_vp, _vp],
+    "mg_dinv_elems": [_i64],
+    "mg_getrf_dinv": [_vp, _int, _vp, _i64, _i64, _vp, _vp],
+    "mg_getrs_dinv": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp],
     "mg_profile_enable": [_vp, _int],
     "mg_profile_read": [_vp, _vp, _c.POINTER(_c.c_longlong), _c.POINTER(_dbl), _c.POINTER(_dbl)],
     "mg_convert": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _vp],
@@ -44,7 +47,7 @@ SIGNATURES = {
     "mg_coldot": [_vp, _i64, _vp, _i64, _i64, _i64, _vp, _vp],
     "mg_sources": [_int, _int, _vp, _i64, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _vp],
 }
-_RESTYPES = {"mg_last_error": _c.c_char_p}
+_RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64}
 
 _lib = None
 

@@ -109,6 +109,35 @@ int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, con
   }
 }
 
+// Elements (of the factor's type) of the diagonal-block inverse store for order n.
+int64_t mg_dinv_elems(int64_t n) { return n < 0 ? 0 : mg::dinv_elems(n); }
+
+// Inverses of the 256 x 256 diagonal blocks of L and U from getrf's factors.
+int mg_getrf_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, void* dinv, void* stream) {
+  if (!h || !LU || !dinv || lda < n) return MG_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  switch (prec) {
+    case MG_C128: return mg::diag_inverses<double>(LU, n, lda, dinv, s);
+    case MG_C64:
+    case MG_BF16X3: return mg::diag_inverses<float>(LU, n, lda, dinv, s);
+    default: return MG_ERR_ARG;
+  }
+}
+
+// Solve with getrf's factors and mg_getrf_dinv's inverses, GEMMs only; work
+// holds n x nrhs elements.
+int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv,
+                  void* B, int64_t ldb, int64_t nrhs, void* work, void* stream) {
+  if (!h || !LU || !ipiv || !dinv || !B || !work) return MG_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  switch (prec) {
+    case MG_C128: return mg::getrs_dinv<double>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
+    case MG_C64:
+    case MG_BF16X3: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
+    default: return MG_ERR_ARG;
+  }
+}
+
 // Waits for the stream and reports a failed grid barrier of earlier factorisations.
 int mg_check(mg_handle* h, void* stream) {
   if (!h) return MG_ERR_ARG;

@@ -4,14 +4,21 @@
 
 namespace {
 
+// blockIdx.x walks the tile grid, blockIdx.y the batch (element strides
+// sA, sB, sC between consecutive problems).
 template <typename T, int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void k_cgemm(int64_t M, int64_t N, int64_t K,
-                                                        typename Cplx<T>::t alpha, MemA a, const void* B,
-                                                        int64_t ldb, typename Cplx<T>::t beta, void* C,
-                                                        int64_t ldc, int nbm, int nbn) {
+                                                        typename Cplx<T>::t alpha, MemA a, int64_t sA,
+                                                        const void* B, int64_t ldb, int64_t sB,
+                                                        typename Cplx<T>::t beta, void* C, int64_t ldc, int64_t sC,
+                                                        int nbm, int nbn) {
+  typedef typename Cplx<T>::t T2;
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
-  cgemm_body<T, BM, BN, BK, WM, WN, MemA>(M, N, K, alpha, a, B, ldb, beta, C, ldc, tm, tn);
+  const int64_t b = blockIdx.y;
+  MemA ab{reinterpret_cast<const T2*>(a.A) + b * sA, a.lda};
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA>(M, N, K, alpha, ab, reinterpret_cast<const T2*>(B) + b * sB, ldb, beta,
+                                          reinterpret_cast<T2*>(C) + b * sC, ldc, tm, tn);
 }
 
 // The same GEMM under its own symbol for the LU trailing update, so profiles
@@ -37,25 +44,38 @@ struct GemmCfg<float> {
   static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2;
 };
 
+template <typename T>
+typename Cplx<T>::t to_t2(mg::cd z) {
+  typename Cplx<T>::t r;
+  r.x = (T)z.re;
+  r.y = (T)z.im;
+  return r;
+}
+
 }  // namespace
 
 namespace mg {
 
 template <typename T>
-int cgemm(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
-          cd beta, void* C, int64_t ldc, hipStream_t s) {
-  if (M < 0 || N < 0 || K < 0) return MG_ERR_ARG;
-  if (M == 0 || N == 0) return MG_OK;
+int cgemm_batched(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+                  int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch,
+                  hipStream_t s) {
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || batch > 65535) return MG_ERR_ARG;
+  if (M == 0 || N == 0 || batch == 0) return MG_OK;
   typedef GemmCfg<T> G;
   const int nbm = (int)((M + G::BM - 1) / G::BM), nbn = (int)((N + G::BN - 1) / G::BN);
-  typename Cplx<T>::t al, be;
-  al.x = (T)alpha.re; al.y = (T)alpha.im;
-  be.x = (T)beta.re; be.y = (T)beta.im;
   MemA a{A, lda};
-  hipLaunchKernelGGL((k_cgemm<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn), dim3(64 * G::WM * G::WN), 0,
-                     s, M, N, K, al, a, B, ldb, be, C, ldc, nbm, nbn);
+  hipLaunchKernelGGL((k_cgemm<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn, (unsigned)batch),
+                     dim3(64 * G::WM * G::WN), 0, s, M, N, K, to_t2<T>(alpha), a, sA, B, ldb, sB, to_t2<T>(beta), C,
+                     ldc, sC, nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
+}
+
+template <typename T>
+int cgemm(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+          cd beta, void* C, int64_t ldc, hipStream_t s) {
+  return cgemm_batched<T>(M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, s);
 }
 
 template <typename T>
@@ -65,24 +85,24 @@ int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
   if (M == 0 || N == 0) return MG_OK;
   typedef GemmCfg<T> G;
   const int nbm = (int)((M + G::BM - 1) / G::BM), nbn = (int)((N + G::BN - 1) / G::BN);
-  typename Cplx<T>::t al, be;
-  al.x = (T)alpha.re; al.y = (T)alpha.im;
-  be.x = (T)beta.re; be.y = (T)beta.im;
   MemA a{A, lda};
   hipLaunchKernelGGL((k_cgemm_trail<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn),
-                     dim3(64 * G::WM * G::WN), 0, s, M, N, K, al, a, B, ldb, be, C, ldc, nbm, nbn);
+                     dim3(64 * G::WM * G::WN), 0, s, M, N, K, to_t2<T>(alpha), a, B, ldb, to_t2<T>(beta), C, ldc,
+                     nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
 
-template int cgemm_trail<double>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd,
-                                 void*, int64_t, hipStream_t);
-template int cgemm_trail<float>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd,
-                                void*, int64_t, hipStream_t);
-template int cgemm<double>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,
-                           int64_t, hipStream_t);
-template int cgemm<float>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,
-                          int64_t, hipStream_t);
+#define MG_INST(T)                                                                                                 \
+  template int cgemm_batched<T>(int64_t, int64_t, int64_t, cd, const void*, int64_t, int64_t, const void*,         \
+                                int64_t, int64_t, cd, void*, int64_t, int64_t, int64_t, hipStream_t);              \
+  template int cgemm<T>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,      \
+                        int64_t, hipStream_t);                                                                     \
+  template int cgemm_trail<T>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd,      \
+                              void*, int64_t, hipStream_t);
+MG_INST(double)
+MG_INST(float)
+#undef MG_INST
 
 }  // namespace mg
 

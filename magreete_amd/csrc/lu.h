@@ -37,6 +37,8 @@ struct Workspace {
   int coop_blocks = 0;
   Getf2Work getf2;
   UpdateProfile prof;
+  hipStream_t s2 = nullptr;        // look-ahead panel stream (default priority)
+  hipEvent_t evA = nullptr, evB = nullptr;
 };
 
 typedef int (*GemmFn)(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*, int64_t,
@@ -63,5 +65,14 @@ int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, 
 template <typename T>
 int getrs(const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb, int64_t nrhs,
           hipStream_t s);
+
+// Block size of the stored diagonal-block inverses used by getrs_dinv.
+constexpr int64_t kInvBlock = 256;
+inline int64_t dinv_elems(int64_t n) { return 2 * ((n + kInvBlock - 1) / kInvBlock) * kInvBlock * kInvBlock; }
+template <typename T>
+int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_t s);
+template <typename T>
+int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv, void* B, int64_t ldb,
+               int64_t nrhs, void* work, hipStream_t s);
 
 }  // namespace mg

@@ -18,7 +18,8 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr long long kSpinLimit = 1ll << 25;   // ~ seconds of s_sleep before a barrier gives up
+constexpr int kBarWords = 1024;   // top counter + 8 group counters, 64 words apart
+constexpr long long kSpinLimit = 1ll << 22;   // polls (~1 s) before a barrier gives up
 
 template <typename T> struct C2;
 template <> struct C2<double> { typedef double2 t; };
@@ -66,19 +67,46 @@ __device__ inline void block_argmax(double& v, int& r, double* sv, int* sr) {
   __syncthreads();
 }
 
-// Grid barrier over all workgroups of a cooperative launch.  The counter
-// starts at 0; barrier number j waits for (j+1)*G arrivals.  A wait that
-// exceeds the spin limit, or sees another workgroup's failure flag, sets the
-// flag and returns, so every workgroup always reaches the end of the kernel.
-__device__ inline void grid_barrier(unsigned* bar, unsigned target, int* err) {
-  __threadfence();
+// Data exchanged between the workgroups of the leaf goes through agent-scope
+// atomic stores and loads, which are coherent across XCDs without the L2
+// write-back / invalidate that agent-scope fences cost (measured on MI355X:
+// ~3 us vs ~31 us per barrier at 192 workgroups).
+template <typename V>
+__device__ inline void pub_store(V* p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename V>
+__device__ inline V pub_load(const V* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T2>
+__device__ inline void pub_store2(T2* p, T2 v) { pub_store(&p->x, v.x); pub_store(&p->y, v.y); }
+template <typename T2>
+__device__ inline T2 pub_load2(const T2* p) { T2 v; v.x = pub_load(&p->x); v.y = pub_load(&p->y); return v; }
+
+// Grid barrier number `it` over all G workgroups of a cooperative launch.
+// The workgroup fences wait for this wave's exchange stores to complete
+// before the arrival is counted.  With more than 64 workgroups, arrivals go
+// to 8 group counters (g % 8) and the last arriver of a group bumps the top
+// counter, which keeps contention on one address bounded.  All counters
+// start at 0.  A wait that exceeds the spin limit, or sees another
+// workgroup's failure flag, sets the flag and returns, so every workgroup
+// always reaches the end of the kernel.
+__device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned target;
+    if (G > 64) {
+      const int grp = blockIdx.x & 7;
+      const unsigned gsize = (unsigned)((G - grp + 7) / 8);
+      const unsigned old = __hip_atomic_fetch_add(bar + 64 * (1 + grp), 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+      if (old + 1 == (it + 1) * gsize) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      target = (it + 1) * 8u;
+    } else {
+      __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      target = (it + 1) * (unsigned)G;
+    }
     long long spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      __builtin_amdgcn_s_sleep(2);
       if (++spins > kSpinLimit) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
@@ -86,7 +114,7 @@ __device__ inline void grid_barrier(unsigned* bar, unsigned target, int* err) {
     }
   }
   __syncthreads();
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // Leaf LU of an m x w panel (w <= kLeaf) at A (row-major, lda).  Workgroup g
@@ -126,23 +154,23 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
     }
     block_argmax(best, brow, sv, sr);
     if (tid == 0) {
-      wk.cand_val[par * G + g] = best;
-      wk.cand_row[par * G + g] = brow;
+      pub_store(&wk.cand_val[par * G + g], best);
+      pub_store(&wk.cand_row[par * G + g], brow);
     }
-    if (brow != INT_MAX && tid < w) cdata[(par * G + g) * mg::kLeaf + tid] = tile[(brow - rb) * w + tid];
-    if (j >= rb && j < re && tid < w) rowj[par * mg::kLeaf + tid] = tile[(j - rb) * w + tid];
-    if (G > 1) grid_barrier(wk.bar, (unsigned)(j + 1) * (unsigned)G, wk.err);
-    else { __threadfence(); __syncthreads(); }
+    if (brow != INT_MAX && tid < w) pub_store2(&cdata[(par * G + g) * mg::kLeaf + tid], tile[(brow - rb) * w + tid]);
+    if (j >= rb && j < re && tid < w) pub_store2(&rowj[par * mg::kLeaf + tid], tile[(j - rb) * w + tid]);
+    if (G > 1) grid_barrier(wk.bar, (unsigned)j, G, wk.err);
+    else __syncthreads();
 
     double pv = -1.0;
     int p = INT_MAX;
-    if (tid < G) { pv = wk.cand_val[par * G + tid]; p = wk.cand_row[par * G + tid]; }
+    if (tid < G) { pv = pub_load(&wk.cand_val[par * G + tid]); p = pub_load(&wk.cand_row[par * G + tid]); }
     block_argmax(pv, p, sv, sr);
     if (p == INT_MAX) p = j;   // whole column NaN: keep row j, NaN propagates
     const int pwg = p / R;
     if (tid < w) {
-      orow[tid] = rowj[par * mg::kLeaf + tid];
-      prow[tid] = (p == j) ? orow[tid] : cdata[(par * G + pwg) * mg::kLeaf + tid];
+      orow[tid] = pub_load2(&rowj[par * mg::kLeaf + tid]);
+      prow[tid] = (p == j) ? orow[tid] : pub_load2(&cdata[(par * G + pwg) * mg::kLeaf + tid]);
     }
     __syncthreads();
     if (tid < w) {
@@ -202,54 +230,56 @@ __global__ __launch_bounds__(256) void k_laswp(T2* __restrict__ A, int64_t lda, 
 }
 
 // Triangular solve of an nb x nb (nb <= 32) block against ncols right-hand
-// sides, one column per thread: unit-lower forward or non-unit upper backward.
+// sides: unit-lower forward or non-unit upper backward substitution.  Each
+// half-wave holds one column with one row per lane; the solved x_i is
+// broadcast by a lane shuffle and the other rows update in parallel, so a
+// column takes nb dependent steps of one complex FMA each.  A workgroup of
+// 256 threads covers 8 columns.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(256) void k_trsm_leaf(const typename C2<T>::t* __restrict__ L, int64_t ldl, int nb,
-                                                   typename C2<T>::t* __restrict__ B, int64_t ldb, int64_t ncols) {
+__global__ __launch_bounds__(256) void k_trsm_leaf(const typename C2<T>::t* __restrict__ L, int64_t ldl, int64_t sL,
+                                                   int nb, typename C2<T>::t* __restrict__ B, int64_t ldb,
+                                                   int64_t sB, int64_t ncols) {
   typedef typename C2<T>::t T2;
+  L += blockIdx.y * sL;   // batch
+  B += blockIdx.y * sB;
   __shared__ T2 Ls[mg::kLeaf][mg::kLeaf + 1];
   __shared__ T2 dinv[mg::kLeaf];
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) Ls[e / nb][e % nb] = L[(e / nb) * ldl + e % nb];
   __syncthreads();
   if (UPPER && threadIdx.x < nb) dinv[threadIdx.x] = crecip(Ls[threadIdx.x][threadIdx.x]);
   __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  T2 x[mg::kLeaf];
-#pragma unroll
-  for (int i = 0; i < mg::kLeaf; ++i)
-    if (i < nb) x[i] = B[i * ldb + c];
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 31, half = lane & 32;
+  const int64_t c = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+  const bool live = c < ncols && r < nb;
+  T2 x;
+  x.x = 0; x.y = 0;
+  if (live) x = B[r * ldb + c];
   if (!UPPER) {
-#pragma unroll
-    for (int i = 1; i < mg::kLeaf; ++i) {
-      if (i < nb) {
-#pragma unroll
-        for (int k = 0; k < i; ++k) {
-          const T2 l = Ls[i][k];
-          x[i].x -= l.x * x[k].x - l.y * x[k].y;
-          x[i].y -= l.x * x[k].y + l.y * x[k].x;
-        }
+    for (int i = 0; i < nb - 1; ++i) {
+      T2 xi;
+      xi.x = __shfl(x.x, half + i);
+      xi.y = __shfl(x.y, half + i);
+      if (r > i && r < nb) {
+        const T2 l = Ls[r][i];
+        x.x -= l.x * xi.x - l.y * xi.y;
+        x.y -= l.x * xi.y + l.y * xi.x;
       }
     }
   } else {
-#pragma unroll
-    for (int i = mg::kLeaf - 1; i >= 0; --i) {
-      if (i < nb) {
-#pragma unroll
-        for (int k = i + 1; k < mg::kLeaf; ++k) {
-          if (k < nb) {
-            const T2 u = Ls[i][k];
-            x[i].x -= u.x * x[k].x - u.y * x[k].y;
-            x[i].y -= u.x * x[k].y + u.y * x[k].x;
-          }
-        }
-        x[i] = cmul(x[i], dinv[i]);
+    for (int i = nb - 1; i >= 0; --i) {
+      if (r == i) x = cmul(x, dinv[i]);
+      T2 xi;
+      xi.x = __shfl(x.x, half + i);
+      xi.y = __shfl(x.y, half + i);
+      if (r < i) {
+        const T2 u = Ls[r][i];
+        x.x -= u.x * xi.x - u.y * xi.y;
+        x.y -= u.x * xi.y + u.y * xi.x;
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < mg::kLeaf; ++i)
-    if (i < nb) B[i * ldb + c] = x[i];
+  if (live) B[r * ldb + c] = x;
 }
 
 inline int64_t split_half(int64_t w) {
@@ -274,40 +304,126 @@ int laswp(void* A, int64_t lda, int64_t c0, int64_t c1, int64_t k1, int64_t k2, 
   return MG_OK;
 }
 
+// Recursive triangular solves, batched: problem b uses L + b sL and B + b sB.
 template <typename T>
-int trsm_lower_unit(const void* L, int64_t ldl, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s) {
+static int trsm_lower_unit_b(const void* L, int64_t ldl, int64_t sL, int64_t nb, void* B, int64_t ldb, int64_t sB,
+                             int64_t ncols, int64_t batch, hipStream_t s) {
   typedef typename C2<T>::t T2;
-  if (nb <= 0 || ncols <= 0) return MG_OK;
+  if (nb <= 0 || ncols <= 0 || batch <= 0) return MG_OK;
   if (nb <= kLeaf) {
-    hipLaunchKernelGGL((k_trsm_leaf<T, false>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<const T2*>(L), ldl, (int)nb, reinterpret_cast<T2*>(B), ldb, ncols);
+    hipLaunchKernelGGL((k_trsm_leaf<T, false>), dim3((unsigned)((ncols + 7) / 8), (unsigned)batch), dim3(256), 0, s,
+                       reinterpret_cast<const T2*>(L), ldl, sL, (int)nb, reinterpret_cast<T2*>(B), ldb, sB, ncols);
     MG_HIP(hipGetLastError());
     return MG_OK;
   }
   const int64_t n1 = split_half(nb);
   const T2* Lp = reinterpret_cast<const T2*>(L);
   T2* Bp = reinterpret_cast<T2*>(B);
-  MG_TRY(trsm_lower_unit<T>(Lp, ldl, n1, Bp, ldb, ncols, s));
-  MG_TRY(cgemm<T>(nb - n1, ncols, n1, cd{-1, 0}, Lp + n1 * ldl, ldl, Bp, ldb, cd{1, 0}, Bp + n1 * ldb, ldb, s));
-  return trsm_lower_unit<T>(Lp + n1 * ldl + n1, ldl, nb - n1, Bp + n1 * ldb, ldb, ncols, s);
+  MG_TRY(trsm_lower_unit_b<T>(Lp, ldl, sL, n1, Bp, ldb, sB, ncols, batch, s));
+  MG_TRY(cgemm_batched<T>(nb - n1, ncols, n1, cd{-1, 0}, Lp + n1 * ldl, ldl, sL, Bp, ldb, sB, cd{1, 0},
+                          Bp + n1 * ldb, ldb, sB, batch, s));
+  return trsm_lower_unit_b<T>(Lp + n1 * ldl + n1, ldl, sL, nb - n1, Bp + n1 * ldb, ldb, sB, ncols, batch, s);
 }
 
 template <typename T>
-int trsm_upper(const void* U, int64_t ldu, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s) {
+static int trsm_upper_b(const void* U, int64_t ldu, int64_t sU, int64_t nb, void* B, int64_t ldb, int64_t sB,
+                        int64_t ncols, int64_t batch, hipStream_t s) {
   typedef typename C2<T>::t T2;
-  if (nb <= 0 || ncols <= 0) return MG_OK;
+  if (nb <= 0 || ncols <= 0 || batch <= 0) return MG_OK;
   if (nb <= kLeaf) {
-    hipLaunchKernelGGL((k_trsm_leaf<T, true>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<const T2*>(U), ldu, (int)nb, reinterpret_cast<T2*>(B), ldb, ncols);
+    hipLaunchKernelGGL((k_trsm_leaf<T, true>), dim3((unsigned)((ncols + 7) / 8), (unsigned)batch), dim3(256), 0, s,
+                       reinterpret_cast<const T2*>(U), ldu, sU, (int)nb, reinterpret_cast<T2*>(B), ldb, sB, ncols);
     MG_HIP(hipGetLastError());
     return MG_OK;
   }
   const int64_t n1 = split_half(nb);
   const T2* Up = reinterpret_cast<const T2*>(U);
   T2* Bp = reinterpret_cast<T2*>(B);
-  MG_TRY(trsm_upper<T>(Up + n1 * ldu + n1, ldu, nb - n1, Bp + n1 * ldb, ldb, ncols, s));
-  MG_TRY(cgemm<T>(n1, ncols, nb - n1, cd{-1, 0}, Up + n1, ldu, Bp + n1 * ldb, ldb, cd{1, 0}, Bp, ldb, s));
-  return trsm_upper<T>(Up, ldu, n1, Bp, ldb, ncols, s);
+  MG_TRY(trsm_upper_b<T>(Up + n1 * ldu + n1, ldu, sU, nb - n1, Bp + n1 * ldb, ldb, sB, ncols, batch, s));
+  MG_TRY(cgemm_batched<T>(n1, ncols, nb - n1, cd{-1, 0}, Up + n1, ldu, sU, Bp + n1 * ldb, ldb, sB, cd{1, 0}, Bp,
+                          ldb, sB, batch, s));
+  return trsm_upper_b<T>(Up, ldu, sU, n1, Bp, ldb, sB, ncols, batch, s);
+}
+
+template <typename T>
+int trsm_lower_unit(const void* L, int64_t ldl, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s) {
+  return trsm_lower_unit_b<T>(L, ldl, 0, nb, B, ldb, 0, ncols, 1, s);
+}
+
+template <typename T>
+int trsm_upper(const void* U, int64_t ldu, int64_t nb, void* B, int64_t ldb, int64_t ncols, hipStream_t s) {
+  return trsm_upper_b<T>(U, ldu, 0, nb, B, ldb, 0, ncols, 1, s);
+}
+
+// Identity into `count` nb x nb blocks (ld nb, consecutive).
+template <typename T2>
+__global__ void k_eye_blocks(T2* D, int nb, int64_t count) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count * nb * nb) return;
+  const int64_t w = e % ((int64_t)nb * nb);
+  T2 v;
+  v.x = (w / nb == w % nb) ? 1 : 0;
+  v.y = 0;
+  D[e] = v;
+}
+
+// Inverses of the kInvBlock x kInvBlock diagonal blocks of L (unit lower) and
+// U: dinv[0 .. nblk) hold inv(L_bb), dinv[nblk .. 2 nblk) inv(U_bb), each a
+// row-major kInvBlock^2 block (the last, partial block uses its top-left).
+template <typename T>
+int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  if (n <= 0) return MG_OK;
+  const int64_t nb = kInvBlock, nblk = (n + nb - 1) / nb, full = n / nb, rem = n - full * nb;
+  const int64_t bs = nb * nb;
+  T2* D = reinterpret_cast<T2*>(dinv);
+  const T2* A = reinterpret_cast<const T2*>(LU);
+  const int64_t tot = 2 * nblk * bs;
+  hipLaunchKernelGGL(k_eye_blocks<T2>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, D, (int)nb, 2 * nblk);
+  MG_HIP(hipGetLastError());
+  const int64_t sdiag = nb * lda + nb;   // stride between diagonal blocks of LU
+  MG_TRY(trsm_lower_unit_b<T>(A, lda, sdiag, nb, D, nb, bs, nb, full, s));
+  MG_TRY(trsm_upper_b<T>(A, lda, sdiag, nb, D + nblk * bs, nb, bs, nb, full, s));
+  if (rem) {
+    const T2* Ab = A + full * sdiag;
+    MG_TRY(trsm_lower_unit_b<T>(Ab, lda, 0, rem, D + full * bs, nb, 0, rem, 1, s));
+    MG_TRY(trsm_upper_b<T>(Ab, lda, 0, rem, D + (nblk + full) * bs, nb, 0, rem, 1, s));
+  }
+  return MG_OK;
+}
+
+// Solve A X = B (B n x nrhs, overwritten) from getrf's factors and
+// diag_inverses' blocks, using only GEMMs: forward W_b = inv(L_bb) B_b,
+// B_{>b} -= L_{>b,b} W_b; backward B_b = inv(U_bb) W_b, W_{<b} -= U_{<b,b} B_b.
+// work: n x nrhs scratch (ld nrhs).
+template <typename T>
+int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv, void* B, int64_t ldb,
+               int64_t nrhs, void* work, hipStream_t s) {
+  typedef typename C2<T>::t T2;
+  if (n < 0 || nrhs < 0 || lda < n || ldb < nrhs) return MG_ERR_ARG;
+  if (n == 0 || nrhs == 0) return MG_OK;
+  const int64_t nb = kInvBlock, nblk = (n + nb - 1) / nb, bs = nb * nb;
+  const T2* A = reinterpret_cast<const T2*>(LU);
+  const T2* D = reinterpret_cast<const T2*>(dinv);
+  T2* Bp = reinterpret_cast<T2*>(B);
+  T2* W = reinterpret_cast<T2*>(work);
+  const int64_t ldw = nrhs;
+  MG_TRY(laswp<T>(B, ldb, 0, nrhs, 0, n, ipiv, s));
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t r0 = b * nb, r1 = (r0 + nb < n) ? r0 + nb : n, h = r1 - r0;
+    MG_TRY(cgemm<T>(h, nrhs, h, cd{1, 0}, D + b * bs, nb, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * ldw, ldw, s));
+    if (r1 < n)
+      MG_TRY(cgemm<T>(n - r1, nrhs, h, cd{-1, 0}, A + r1 * lda + r0, lda, W + r0 * ldw, ldw, cd{1, 0},
+                      Bp + r1 * ldb, ldb, s));
+  }
+  for (int64_t b = nblk - 1; b >= 0; --b) {
+    const int64_t r0 = b * nb, r1 = (r0 + nb < n) ? r0 + nb : n, h = r1 - r0;
+    MG_TRY(cgemm<T>(h, nrhs, h, cd{1, 0}, D + (nblk + b) * bs, nb, W + r0 * ldw, ldw, cd{0, 0}, Bp + r0 * ldb, ldb,
+                    s));
+    if (r0 > 0)
+      MG_TRY(cgemm<T>(r0, nrhs, h, cd{-1, 0}, A + r0, lda, Bp + r0 * ldb, ldb, cd{1, 0}, W, ldw, s));
+  }
+  return MG_OK;
 }
 
 template <typename T>
@@ -329,14 +445,16 @@ static int getf2(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int6
     }
     R = (int)((m + G - 1) / G);
   }
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(ws.getf2.bar), 1);
+  hipLaunchKernelGGL(k_zero_ints, dim3(2), dim3(512), 0, s, reinterpret_cast<int*>(ws.getf2.bar), kBarWords);
   T2* Ap = reinterpret_cast<T2*>(A) + k * lda + k;
   int mi = (int)m, wi = (int)w;
   Getf2Work wk = ws.getf2;
   const size_t shmem = (size_t)R * bytes_per_row;
-  void* args[] = {&Ap, (void*)&lda, &mi, &wi, &R, (void*)&k, &ipiv, &info, &wk};
-  MG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_getf2<T>), dim3(G), dim3(kThreads), args,
-                                    shmem, s));
+  // G <= the CU count, one workgroup per CU: every workgroup becomes resident
+  // once concurrently running kernels (which never wait on this one) drain,
+  // and each barrier wait is bounded by the spin limit.
+  hipLaunchKernelGGL(k_getf2<T>, dim3(G), dim3(kThreads), shmem, s, Ap, lda, mi, wi, R, k, ipiv, info, wk);
+  MG_HIP(hipGetLastError());
   return MG_OK;
 }
 
@@ -367,35 +485,58 @@ int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, 
   hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
   if (n == 0) return MG_OK;
   T2* Ap = reinterpret_cast<T2*>(A);
+  GemmFn fn = update ? update : &cgemm_trail<T>;
+  // C[r0:, c0:c0+nc] -= L[r0:, k:k+jb] U[k:k+jb, c0:c0+nc], timed when profiling
+  auto trail = [&](int64_t k, int64_t jb, int64_t c0, int64_t nc) -> int {
+    const int64_t r0 = k + jb, r = n - r0;
+    if (r <= 0 || nc <= 0) return MG_OK;
+    UpdateProfile& pr = ws.prof;
+    if (pr.enabled) {
+      if (pr.used * 2 + 2 > pr.ev.size()) {
+        hipEvent_t a, b;
+        MG_HIP(hipEventCreate(&a));
+        MG_HIP(hipEventCreate(&b));
+        pr.ev.push_back(a);
+        pr.ev.push_back(b);
+      }
+      MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
+    }
+    MG_TRY(fn(r, nc, jb, cd{-1, 0}, Ap + r0 * lda + k, lda, Ap + k * lda + c0, lda, cd{1, 0}, Ap + r0 * lda + c0, lda,
+              s));
+    if (pr.enabled) {
+      MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
+      if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
+      pr.flops[pr.used] = 8.0 * (double)r * (double)nc * (double)jb;
+      ++pr.used;
+    }
+    return MG_OK;
+  };
+  // Look-ahead of depth one: once the next panel's columns are updated, that
+  // panel is factored on the workspace's second stream while the rest of the
+  // trailing update runs here.  The two touch disjoint columns; the next
+  // step waits for the panel before applying its row swaps.
+  MG_TRY(panel<T>(ws, A, lda, 0, n, (n < nb) ? n : nb, ipiv, info, s));
   for (int64_t k = 0; k < n; k += nb) {
     const int64_t jb = (n - k < nb) ? n - k : nb;
-    MG_TRY(panel<T>(ws, A, lda, k, n - k, jb, ipiv, info, s));
-    MG_TRY(laswp<T>(A, lda, 0, k, k, k + jb, ipiv, s));
-    MG_TRY(laswp<T>(A, lda, k + jb, n, k, k + jb, ipiv, s));
-    if (k + jb < n) {
-      MG_TRY(trsm_lower_unit<T>(Ap + k * lda + k, lda, jb, Ap + k * lda + k + jb, lda, n - k - jb, s));
-      const int64_t r = n - k - jb;
-      UpdateProfile& pr = ws.prof;
-      if (pr.enabled) {
-        if (pr.used * 2 + 2 > pr.ev.size()) {
-          hipEvent_t a, b;
-          MG_HIP(hipEventCreate(&a));
-          MG_HIP(hipEventCreate(&b));
-          pr.ev.push_back(a);
-          pr.ev.push_back(b);
-        }
-        MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
-      }
-      GemmFn fn = update ? update : &cgemm_trail<T>;
-      MG_TRY(fn(r, r, jb, cd{-1, 0}, Ap + (k + jb) * lda + k, lda, Ap + k * lda + k + jb, lda, cd{1, 0},
-                Ap + (k + jb) * lda + k + jb, lda, s));
-      if (pr.enabled) {
-        MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
-        if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
-        pr.flops[pr.used] = 8.0 * (double)r * (double)r * (double)jb;
-        ++pr.used;
+    const int64_t kn = k + jb;
+    MG_TRY(laswp<T>(A, lda, kn, n, k, kn, ipiv, s));
+    if (kn < n) {
+      const int64_t jb2 = (n - kn < nb) ? n - kn : nb;
+      MG_TRY(trsm_lower_unit<T>(Ap + k * lda + k, lda, jb, Ap + k * lda + kn, lda, n - kn, s));
+      MG_TRY(trail(k, jb, kn, jb2));
+      if (ws.s2) {
+        MG_HIP(hipEventRecord(ws.evA, s));
+        MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
+        MG_TRY(panel<T>(ws, A, lda, kn, n - kn, jb2, ipiv, info, ws.s2));
+        MG_HIP(hipEventRecord(ws.evB, ws.s2));
+        MG_TRY(trail(k, jb, kn + jb2, n - kn - jb2));
+        MG_HIP(hipStreamWaitEvent(s, ws.evB, 0));
+      } else {
+        MG_TRY(trail(k, jb, kn + jb2, n - kn - jb2));
+        MG_TRY(panel<T>(ws, A, lda, kn, n - kn, jb2, ipiv, info, s));
       }
     }
+    MG_TRY(laswp<T>(A, lda, 0, k, k, kn, ipiv, s));
   }
   return MG_OK;
 }
@@ -432,10 +573,13 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipMalloc(&ws.getf2.cand_row, sizeof(int) * 2 * G));
   MG_HIP(hipMalloc(&ws.getf2.cand_data, sizeof(double2) * 2 * G * kLeaf));
   MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kLeaf));
-  MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * 64));
+  MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * kBarWords));
   MG_HIP(hipMalloc(&ws.getf2.err, sizeof(int) * 64));
   MG_HIP(hipMemset(ws.getf2.err, 0, sizeof(int) * 64));
   if (ws.coop_blocks > G) ws.coop_blocks = G;
+  MG_HIP(hipStreamCreateWithFlags(&ws.s2, hipStreamNonBlocking));
+  MG_HIP(hipEventCreateWithFlags(&ws.evA, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evB, hipEventDisableTiming));
   ws.ready = true;
   return MG_OK;
 }
@@ -466,6 +610,9 @@ void workspace_free(Workspace& ws) {
   (void)hipFree(ws.getf2.rowj);
   (void)hipFree(ws.getf2.bar);
   (void)hipFree(ws.getf2.err);
+  if (ws.s2) (void)hipStreamDestroy(ws.s2);
+  if (ws.evA) (void)hipEventDestroy(ws.evA);
+  if (ws.evB) (void)hipEventDestroy(ws.evB);
   ws = Workspace();
 }
 
@@ -490,5 +637,11 @@ template int getrf<double>(Workspace&, void*, int64_t, int64_t, int*, int*, hipS
 template int getrf<float>(Workspace&, void*, int64_t, int64_t, int*, int*, hipStream_t, int, GemmFn);
 template int getrs<double>(const void*, int64_t, int64_t, const int*, void*, int64_t, int64_t, hipStream_t);
 template int getrs<float>(const void*, int64_t, int64_t, const int*, void*, int64_t, int64_t, hipStream_t);
+template int diag_inverses<double>(const void*, int64_t, int64_t, void*, hipStream_t);
+template int diag_inverses<float>(const void*, int64_t, int64_t, void*, hipStream_t);
+template int getrs_dinv<double>(const void*, int64_t, int64_t, const int*, const void*, void*, int64_t, int64_t, void*,
+                                hipStream_t);
+template int getrs_dinv<float>(const void*, int64_t, int64_t, const int*, const void*, void*, int64_t, int64_t, void*,
+                               hipStream_t);
 
 }  // namespace mg

@@ -32,6 +32,12 @@ template <typename T>
 int cgemm(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
           cd beta, void* C, int64_t ldc, hipStream_t s);
 
+// Batched form: problem b uses A + b sA, B + b sB, C + b sC (element strides).
+template <typename T>
+int cgemm_batched(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+                  int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch,
+                  hipStream_t s);
+
 // The same GEMM launched under a separate kernel symbol (LU trailing update).
 template <typename T>
 int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,

@@ -110,6 +110,34 @@ def getrs(LU, ipiv, B, prec="c128"):
     return B
 
 
+class Factor:
+    """An LU factorisation resident on the device: factors, pivots, info and
+    the diagonal-block inverses that make every later solve GEMM-only."""
+
+    def __init__(self, A, prec="c128"):
+        lib = L.load()
+        self.A, self.prec, self.n = A, prec, A.shape[0]
+        self.ipiv, self.info = getrf(A, prec)
+        dev = A.device
+        self.dinv = torch.empty(lib.mg_dinv_elems(self.n), dtype=A.dtype, device=dev)
+        with torch.cuda.device(dev):
+            st = lib.mg_getrf_dinv(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.dinv), _stream(dev))
+        L.check(st, "mg_getrf_dinv")
+        self._work = None
+
+    def solve(self, B):
+        """In place: B <- A^{-1} B (B n x nrhs, the factor's dtype, row-major)."""
+        lib = L.load()
+        dev = self.A.device
+        if self._work is None or self._work.numel() < B.shape[0] * B.shape[1]:
+            self._work = torch.empty(B.shape[0] * B.shape[1], dtype=self.A.dtype, device=dev)
+        with torch.cuda.device(dev):
+            st = lib.mg_getrs_dinv(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A), _p(self.ipiv),
+                                   _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(self._work), _stream(dev))
+        L.check(st, "mg_getrs_dinv")
+        return B
+
+
 def profile_updates(device, on):
     """Enable/disable HIP-event timing of the LU trailing-update launches."""
     L.check(L.load().mg_profile_enable(handle(device), 1 if on else 0), "mg_profile_enable")

@@ -69,10 +69,11 @@ class Solver:
     def factor_solve(self):
         p = self.p
         if self.method == "fp64":
-            ipiv, info = E.getrf(self.M)
+            f = E.Factor(self.M)
             self.X.copy_(p.E0s)
-            E.getrs(self.M, ipiv, self.X)
-            self.info = info
+            f.solve(self.X)
+            self.info = f.info
+            self.factor = f
             self.stats = {"iters": 0, "fallback": False}
             return self.X
         st = refine.solve_mixed(self.M, self.M32, p.E0s, self.X, prec=self.method, tol=self.tol,

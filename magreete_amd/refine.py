@@ -28,14 +28,15 @@ def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, M
     iters (FP64 products), restarts, residual (max over columns), fallback, info."""
     dev = M.device
     n, T = B.shape
-    ipiv, info = E.getrf(M32, prec)
+    F = E.Factor(M32, prec)
+    info = F.info
     lowp = torch.empty((n, T), dtype=torch.complex64, device=dev)
     bnorm = torch.linalg.norm(B, dim=0)
     bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
 
     def precond(v, out):
         E.convert(v, lowp)
-        E.getrs(M32, ipiv, lowp, prec)
+        F.solve(lowp)
         E.convert(lowp, out)
         return out
 
@@ -57,15 +58,14 @@ def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, M
         if iters >= max_iters or (prev is not None and rel > 0.5 * prev):
             # stalled: complex128 LU
             stats["fallback"] = True
-            M64 = M.clone()
-            ip64, info64 = E.getrf(M64, "c128")
+            F64 = E.Factor(M.clone(), "c128")
             X.copy_(B)
-            E.getrs(M64, ip64, X, "c128")
-            info = info64
+            F64.solve(X)
+            info = F64.info
             R.copy_(B)
             E.cgemm(M, X, R, alpha=-1.0, beta=1.0)
             rel = (torch.linalg.norm(R, dim=0) / bnorm).max().item()
-            del M64
+            del F64
             break
         prev = rel
         restarts += 1
