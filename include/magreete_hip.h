@@ -71,6 +71,11 @@ int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv,
  * solve step of torch.linalg.solve at the same call sites. */
 int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb,
              int64_t nrhs, void* stream);
+/* LU without pivoting, in place: the low-precision factorisation that
+ * preconditions the FP64 GMRES refinement of the mixed-precision solve (no
+ * counterpart in the reference).  info: first exactly-zero pivot or 0.
+ * Solve with mg_getrs_dinv and ipiv = NULL. */
+int mg_getrf_nopiv(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* info, void* stream);
 /* Stored inverses of the 256 x 256 diagonal blocks of L and U, so that the
  * solve runs as GEMMs only (repeated solves against a resident factor; same
  * call sites as mg_getrs).  dinv holds mg_dinv_elems(n) elements of the

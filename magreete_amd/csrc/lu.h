@@ -65,6 +65,12 @@ int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, 
 template <typename T>
 int getrs(const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb, int64_t nrhs,
           hipStream_t s);
+// LU without pivoting (mixed-precision preconditioner); info: first zero pivot.
+template <typename T>
+int getrf_nopiv(Workspace& ws, void* A, int64_t n, int64_t lda, int* info, hipStream_t s, GemmFn update);
+template <typename T>
+int trsm_lower_unit_b(const void* L, int64_t ldl, int64_t sL, int64_t nb, void* B, int64_t ldb, int64_t sB,
+                      int64_t ncols, int64_t batch, hipStream_t s);
 
 // Block size of the stored diagonal-block inverses used by getrs_dinv.
 constexpr int64_t kInvBlock = 256;

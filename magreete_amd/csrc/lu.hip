@@ -306,7 +306,7 @@ int laswp(void* A, int64_t lda, int64_t c0, int64_t c1, int64_t k1, int64_t k2, 
 
 // Recursive triangular solves, batched: problem b uses L + b sL and B + b sB.
 template <typename T>
-static int trsm_lower_unit_b(const void* L, int64_t ldl, int64_t sL, int64_t nb, void* B, int64_t ldb, int64_t sB,
+int trsm_lower_unit_b(const void* L, int64_t ldl, int64_t sL, int64_t nb, void* B, int64_t ldb, int64_t sB,
                              int64_t ncols, int64_t batch, hipStream_t s) {
   typedef typename C2<T>::t T2;
   if (nb <= 0 || ncols <= 0 || batch <= 0) return MG_OK;
@@ -326,7 +326,7 @@ static int trsm_lower_unit_b(const void* L, int64_t ldl, int64_t sL, int64_t nb,
 }
 
 template <typename T>
-static int trsm_upper_b(const void* U, int64_t ldu, int64_t sU, int64_t nb, void* B, int64_t ldb, int64_t sB,
+int trsm_upper_b(const void* U, int64_t ldu, int64_t sU, int64_t nb, void* B, int64_t ldb, int64_t sB,
                         int64_t ncols, int64_t batch, hipStream_t s) {
   typedef typename C2<T>::t T2;
   if (nb <= 0 || ncols <= 0 || batch <= 0) return MG_OK;
@@ -408,7 +408,7 @@ int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const vo
   T2* Bp = reinterpret_cast<T2*>(B);
   T2* W = reinterpret_cast<T2*>(work);
   const int64_t ldw = nrhs;
-  MG_TRY(laswp<T>(B, ldb, 0, nrhs, 0, n, ipiv, s));
+  if (ipiv) MG_TRY(laswp<T>(B, ldb, 0, nrhs, 0, n, ipiv, s));   // no ipiv: unpivoted factors
   for (int64_t b = 0; b < nblk; ++b) {
     const int64_t r0 = b * nb, r1 = (r0 + nb < n) ? r0 + nb : n, h = r1 - r0;
     MG_TRY(cgemm<T>(h, nrhs, h, cd{1, 0}, D + b * bs, nb, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * ldw, ldw, s));
@@ -637,6 +637,10 @@ template int getrf<double>(Workspace&, void*, int64_t, int64_t, int*, int*, hipS
 template int getrf<float>(Workspace&, void*, int64_t, int64_t, int*, int*, hipStream_t, int, GemmFn);
 template int getrs<double>(const void*, int64_t, int64_t, const int*, void*, int64_t, int64_t, hipStream_t);
 template int getrs<float>(const void*, int64_t, int64_t, const int*, void*, int64_t, int64_t, hipStream_t);
+template int trsm_lower_unit_b<double>(const void*, int64_t, int64_t, int64_t, void*, int64_t, int64_t, int64_t, int64_t,
+                                       hipStream_t);
+template int trsm_lower_unit_b<float>(const void*, int64_t, int64_t, int64_t, void*, int64_t, int64_t, int64_t, int64_t,
+                                      hipStream_t);
 template int diag_inverses<double>(const void*, int64_t, int64_t, void*, hipStream_t);
 template int diag_inverses<float>(const void*, int64_t, int64_t, void*, hipStream_t);
 template int getrs_dinv<double>(const void*, int64_t, int64_t, const int*, const void*, void*, int64_t, int64_t, void*,

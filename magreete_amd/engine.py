@@ -114,11 +114,19 @@ class Factor:
     """An LU factorisation resident on the device: factors, pivots, info and
     the diagonal-block inverses that make every later solve GEMM-only."""
 
-    def __init__(self, A, prec="c128"):
+    def __init__(self, A, prec="c128", pivot=True):
         lib = L.load()
-        self.A, self.prec, self.n = A, prec, A.shape[0]
-        self.ipiv, self.info = getrf(A, prec)
+        self.A, self.prec, self.n, self.pivot = A, prec, A.shape[0], pivot
         dev = A.device
+        if pivot:
+            self.ipiv, self.info = getrf(A, prec)
+        else:
+            self.ipiv = None
+            self.info = torch.zeros(1, dtype=torch.int32, device=dev)
+            with torch.cuda.device(dev):
+                st = lib.mg_getrf_nopiv(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.info),
+                                        _stream(dev))
+            L.check(st, "mg_getrf_nopiv")
         self.dinv = torch.empty(lib.mg_dinv_elems(self.n), dtype=A.dtype, device=dev)
         with torch.cuda.device(dev):
             st = lib.mg_getrf_dinv(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.dinv), _stream(dev))

@@ -91,7 +91,22 @@ class Solver:
         E.propagate(p.kind, p.measure, p.points, p.k0, p.alpha * p.k0 * p.k0, self.X, self.out, flags=flags)
         return self.out
 
-    def run(self):
+    def run(self, phases=None):
+        """One solve.  phases: optional dict that receives per-phase milliseconds
+        (HIP events on the current stream; synchronises)."""
+        if phases is None:
+            self.assemble()
+            self.factor_solve()
+            return self.propagate()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
         self.assemble()
+        ev[1].record()
         self.factor_solve()
-        return self.propagate()
+        ev[2].record()
+        out = self.propagate()
+        ev[3].record()
+        ev[3].synchronize()
+        for name, a, b in (("assemble", 0, 1), ("factor_solve", 1, 2), ("propagate", 2, 3)):
+            phases[name] = phases.get(name, 0.0) + ev[a].elapsed_time(ev[b])
+        return out

@@ -1,0 +1,106 @@
+"""Mixed-precision solve (complex64 LU + FP64 GMRES-IR) against the FP64 path
+and the oracle, and its fallback tiers forced from the host."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from magreete_amd import engine as E
+from magreete_amd import refine
+from magreete_amd import workloads as W
+from magreete_amd.pipeline import Problem, Solver
+from oracle import transmission as O
+
+DEV = "cuda:0"
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def crand(shape, rng):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+@pytest.mark.parametrize("n", [7, 256, 300, 777])
+def test_nopiv_factor_solve(n):
+    say("nopiv", n)
+    rng = np.random.default_rng(n)
+    A = crand((n, n), rng) / np.sqrt(2 * n) + 2.0 * np.eye(n)
+    B = crand((n, 9), rng)
+    F = E.Factor(torch.from_numpy(A.astype(np.complex64)).to(DEV), "c64", pivot=False)
+    assert int(F.info.item()) == 0
+    Bd = torch.from_numpy(B.astype(np.complex64)).to(DEV)
+    F.solve(Bd)
+    ref = np.linalg.solve(A, B)
+    x = Bd.cpu().numpy().astype(np.complex128)
+    assert (np.linalg.norm(x - ref, axis=0) / np.linalg.norm(ref, axis=0)).max() < 1e-5
+
+
+def _problem(npts=400, ndirs=8, seed=0):
+    w = W.cfg_3d_vector(n_points=npts, n_dirs=ndirs, n_measure=64, seed=seed)
+    from magreete_amd.Transmission3D import _pvec
+    pts, meas = w.points.to(DEV), w.measure.to(DEV)
+    pol = _pvec(w.p)
+    E0s = E.sources(3, False, pts, w.u, pol, w.k0, 0.2).reshape(-1, w.T).contiguous()
+    E0m = E.sources(3, False, meas, w.u, pol, w.k0, 0.2).reshape(-1, w.T).contiguous()
+    return w, Problem("3d_vector", pts, w.k0, w.alpha, E0s, meas, E0m)
+
+
+def colerr(x, ref):
+    return (torch.linalg.norm(x - ref, dim=0) / torch.linalg.norm(ref, dim=0)).max().item()
+
+
+@pytest.mark.parametrize("method", ["c64"])
+def test_mixed_matches_fp64_and_oracle(method):
+    say("mixed", method)
+    w, prob = _problem()
+    s64 = Solver(prob, "fp64")
+    F64 = s64.run().clone()
+    X64 = s64.X.clone()
+    sm = Solver(prob, method, tol=1e-12)
+    Fm = sm.run()
+    assert sm.stats["tier"] == "nopiv" and not sm.stats["fallback"], sm.stats
+    assert colerr(sm.X, X64) < 1e-9 and colerr(Fm, F64) < 1e-9
+    # and against the oracle (CPU restatement of the reference)
+    o = O.Transmission3D_vector(w.points, "plane")
+    cE0s = o.generate_source(w.points, w.k0, w.u, w.p, 0.2)
+    cE0m = o.generate_source(w.measure, w.k0, w.u, w.p, 0.2)
+    cEk = o.solve(w.k0, w.alpha, 0.0, cE0s, self_interaction=False)
+    cE = o.propagate(w.measure, cEk, w.k0, w.alpha, cE0m)
+    assert colerr(sm.X.cpu(), cEk) < 1e-9
+    assert colerr(Fm.cpu(), cE.reshape(Fm.shape)) < 1e-9
+
+
+def test_zero_pivot_falls_back_to_pivoted():
+    """A(0,0) = 0 breaks the unpivoted factorisation: the pivoted tier answers."""
+    say("fallback pivoted")
+    rng = np.random.default_rng(3)
+    n = 200
+    A = crand((n, n), rng) / np.sqrt(2 * n) + 2.0 * np.eye(n)
+    A[0, 0] = 0.0
+    B = crand((n, 4), rng)
+    M = torch.from_numpy(A).to(DEV)
+    M32 = M.to(torch.complex64)
+    X = torch.empty((n, 4), dtype=torch.complex128, device=DEV)
+    st = refine.solve_mixed(M, M32, torch.from_numpy(B).to(DEV), X, "c64", tol=1e-13)
+    assert st["tier"] == "pivoted" and st["fallback"]
+    ref = np.linalg.solve(A, B)
+    assert colerr(X.cpu(), torch.from_numpy(ref)) < 1e-11
+
+
+def test_stall_falls_back_to_fp64():
+    """An iteration cap of 1 forces the complex128 tier."""
+    say("fallback fp64")
+    rng = np.random.default_rng(4)
+    n = 150
+    A = crand((n, n), rng) / np.sqrt(2 * n) + 2.0 * np.eye(n)
+    B = crand((n, 3), rng)
+    M = torch.from_numpy(A).to(DEV)
+    X = torch.empty((n, 3), dtype=torch.complex128, device=DEV)
+    st = refine.solve_mixed(M, M.to(torch.complex64), torch.from_numpy(B).to(DEV), X, "c64", tol=1e-14,
+                            max_iters=1)
+    assert st["tier"] == "fp64" and st["fallback"]
+    ref = np.linalg.solve(A, B)
+    assert colerr(X.cpu(), torch.from_numpy(ref)) < 1e-12
