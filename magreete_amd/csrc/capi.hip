@@ -145,8 +145,8 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (prec) {
     case MG_C128: return mg::getrs_dinv<double>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
-    case MG_C64:
-    case MG_BF16X3: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
+    case MG_C64: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
+    case MG_BF16X3: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3);
     default: return MG_ERR_ARG;
   }
 }

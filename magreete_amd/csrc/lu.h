@@ -77,8 +77,9 @@ constexpr int64_t kInvBlock = 256;
 inline int64_t dinv_elems(int64_t n) { return 2 * ((n + kInvBlock - 1) / kInvBlock) * kInvBlock * kInvBlock; }
 template <typename T>
 int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_t s);
+// gemm: GEMM of the block solves (nullptr: the FP64/FP32 MFMA GEMM of T).
 template <typename T>
 int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv, void* B, int64_t ldb,
-               int64_t nrhs, void* work, hipStream_t s);
+               int64_t nrhs, void* work, hipStream_t s, GemmFn gemm = nullptr);
 
 }  // namespace mg

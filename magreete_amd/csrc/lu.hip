@@ -398,10 +398,11 @@ int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_
 // work: n x nrhs scratch (ld nrhs).
 template <typename T>
 int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv, void* B, int64_t ldb,
-               int64_t nrhs, void* work, hipStream_t s) {
+               int64_t nrhs, void* work, hipStream_t s, GemmFn gemm) {
   typedef typename C2<T>::t T2;
   if (n < 0 || nrhs < 0 || lda < n || ldb < nrhs) return MG_ERR_ARG;
   if (n == 0 || nrhs == 0) return MG_OK;
+  const GemmFn g = gemm ? gemm : &cgemm<T>;
   const int64_t nb = kInvBlock, nblk = (n + nb - 1) / nb, bs = nb * nb;
   const T2* A = reinterpret_cast<const T2*>(LU);
   const T2* D = reinterpret_cast<const T2*>(dinv);
@@ -411,17 +412,15 @@ int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const vo
   if (ipiv) MG_TRY(laswp<T>(B, ldb, 0, nrhs, 0, n, ipiv, s));   // no ipiv: unpivoted factors
   for (int64_t b = 0; b < nblk; ++b) {
     const int64_t r0 = b * nb, r1 = (r0 + nb < n) ? r0 + nb : n, h = r1 - r0;
-    MG_TRY(cgemm<T>(h, nrhs, h, cd{1, 0}, D + b * bs, nb, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * ldw, ldw, s));
+    MG_TRY(g(h, nrhs, h, cd{1, 0}, D + b * bs, nb, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * ldw, ldw, s));
     if (r1 < n)
-      MG_TRY(cgemm<T>(n - r1, nrhs, h, cd{-1, 0}, A + r1 * lda + r0, lda, W + r0 * ldw, ldw, cd{1, 0},
-                      Bp + r1 * ldb, ldb, s));
+      MG_TRY(g(n - r1, nrhs, h, cd{-1, 0}, A + r1 * lda + r0, lda, W + r0 * ldw, ldw, cd{1, 0}, Bp + r1 * ldb, ldb,
+               s));
   }
   for (int64_t b = nblk - 1; b >= 0; --b) {
     const int64_t r0 = b * nb, r1 = (r0 + nb < n) ? r0 + nb : n, h = r1 - r0;
-    MG_TRY(cgemm<T>(h, nrhs, h, cd{1, 0}, D + (nblk + b) * bs, nb, W + r0 * ldw, ldw, cd{0, 0}, Bp + r0 * ldb, ldb,
-                    s));
-    if (r0 > 0)
-      MG_TRY(cgemm<T>(r0, nrhs, h, cd{-1, 0}, A + r0, lda, Bp + r0 * ldb, ldb, cd{1, 0}, W, ldw, s));
+    MG_TRY(g(h, nrhs, h, cd{1, 0}, D + (nblk + b) * bs, nb, W + r0 * ldw, ldw, cd{0, 0}, Bp + r0 * ldb, ldb, s));
+    if (r0 > 0) MG_TRY(g(r0, nrhs, h, cd{-1, 0}, A + r0, lda, Bp + r0 * ldb, ldb, cd{1, 0}, W, ldw, s));
   }
   return MG_OK;
 }
@@ -644,8 +643,8 @@ template int trsm_lower_unit_b<float>(const void*, int64_t, int64_t, int64_t, vo
 template int diag_inverses<double>(const void*, int64_t, int64_t, void*, hipStream_t);
 template int diag_inverses<float>(const void*, int64_t, int64_t, void*, hipStream_t);
 template int getrs_dinv<double>(const void*, int64_t, int64_t, const int*, const void*, void*, int64_t, int64_t, void*,
-                                hipStream_t);
+                                hipStream_t, GemmFn);
 template int getrs_dinv<float>(const void*, int64_t, int64_t, const int*, const void*, void*, int64_t, int64_t, void*,
-                               hipStream_t);
+                               hipStream_t, GemmFn);
 
 }  // namespace mg
