@@ -84,6 +84,19 @@ int64_t mg_dinv_elems(int64_t n);
 int mg_getrf_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, void* dinv, void* stream);
 int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv,
                   void* B, int64_t ldb, int64_t nrhs, void* work, void* stream);
+/* Block LDU factorisation without pivoting, in place (complex float; prec
+ * MG_C64 or MG_BF16X3): A = Lh D Uh with 256 x 256 diagonal blocks; dinv
+ * (mg_ldu_dinv_elems(n)) receives the inverses of the D blocks, A the scaled
+ * off-diagonal blocks; work holds mg_ldu_work_elems(n) elements; info: first
+ * exactly-zero pivot or 0.  The preconditioner of the mixed-precision
+ * refinement (no counterpart in the reference).  mg_getrs_ldu solves with it
+ * by GEMMs only (work: n x nrhs elements). */
+int64_t mg_ldu_dinv_elems(int64_t n);
+int64_t mg_ldu_work_elems(int64_t n);
+int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info,
+                 void* stream);
+int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B,
+                 int64_t ldb, int64_t nrhs, void* work, void* stream);
 /* Waits for `stream`; MG_ERR_SYNC if a grid barrier of an LU leaf failed. */
 int mg_check(mg_handle* h, void* stream);
 /* Measurement hooks with no reference counterpart: HIP-event timing of the
@@ -121,6 +134,18 @@ int mg_coldot(const void* Y, int64_t ldy, const void* X, int64_t ldx, int64_t n,
  * npts x T; else out[(m d + a) T + t] = f_t(x_m) pol[t d + a]. */
 int mg_sources(int space, int type, const double* points, int64_t npts, const double* dirs, int64_t T,
                const double* pol, int64_t d, double k0, double w, void* out, void* stream);
+
+/* Gram-Schmidt passes of the mixed-precision refinement (no counterpart in
+ * the reference, whose solve is a single torch.linalg.solve).  V_i = V + i sV
+ * are n x T complex128 matrices (row stride ldv), one Krylov process per
+ * column t.  mg_mdot: out[i ldo + t] = sum_r conj(V_i[r, t]) w[r, t] for
+ * i < k, summed in a fixed order (work: mg_mdot_work_elems(k, T) complex128).
+ * mg_maxpy: w[r, t] += alpha sum_i V_i[r, t] h[i ldh + t]. */
+int64_t mg_mdot_work_elems(int64_t k, int64_t T);
+int mg_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* w, int64_t ldw, int64_t n, int64_t T,
+            void* work, void* out, int64_t ldo, void* stream);
+int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
+             int64_t ldw, int64_t n, int64_t T, void* stream);
 
 #ifdef __cplusplus
 }

@@ -151,6 +151,33 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
   }
 }
 
+int64_t mg_ldu_dinv_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_dinv_elems(n); }
+int64_t mg_ldu_work_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_work_elems(n); }
+
+// Block LDU factorisation without pivoting (complex float; MG_C64: FP32 MFMA
+// GEMMs, MG_BF16X3: BF16x3 GEMMs).
+int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info,
+                 void* stream) {
+  if (!h || !A || !dinv || !work || !info) return MG_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  switch (prec) {
+    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr);
+    case MG_BF16X3: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3);
+    default: return MG_ERR_ARG;
+  }
+}
+
+int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B,
+                 int64_t ldb, int64_t nrhs, void* work, void* stream) {
+  if (!h || !LDU || !dinv || !B || !work) return MG_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  switch (prec) {
+    case MG_C64: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, nullptr);
+    case MG_BF16X3: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3);
+    default: return MG_ERR_ARG;
+  }
+}
+
 // Waits for the stream and reports a failed grid barrier of earlier factorisations.
 int mg_check(mg_handle* h, void* stream) {
   if (!h) return MG_ERR_ARG;

@@ -1,8 +1,5 @@
 // Complex-float GEMM C = alpha A B + beta C with every real product formed
-// as three bf16 MFMA products (BF16x3): x = x_hi + x_lo with x_hi = bf16(x)
-// and x_lo = bf16(x - x_hi), and x y ~ x_hi y_hi + x_hi y_lo + x_lo y_hi,
-// accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  The split keeps about 17
-// significant bits per product (relative error ~1e-5); it serves the
+// as three bf16 MFMA products (BF16x3, see bf16x3.h).  It serves the
 // mixed-precision factorisation, whose FP64 refinement restores accuracy.
 //
 // Tiles: 64 x 128 complex outputs per 256-thread workgroup (2 x 2 waves of
@@ -10,46 +7,18 @@
 // k-tile is fetched into registers while the current one is multiplied; LDS
 // holds it split into four bf16 planes (re hi, re lo, im hi, im lo) with k
 // contiguous, so each lane reads its 8 k-values as one 16-byte load.
-// MFMA lane layout (measured on gfx950): lane l feeds A[l%16][8(l/16)+e] and
-// B[8(l/16)+e][l%16], e = 0..7, and item r of its result is C[4(l/16)+r][l%16].
+#include "bf16x3.h"
 #include "cgemm.h"
 #include "mg_internal.h"
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
 constexpr int BM = 64, BN = 128, BK = 32, KP = BK + 8;   // KP: padded k stride (bf16 elements)
-
-__device__ inline unsigned short bf16_rne(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-
-// hi/lo split of one float into bf16 bit patterns
-__device__ inline void split(float x, unsigned short& hi, unsigned short& lo) {
-  hi = bf16_rne(x);
-  const float xh = __uint_as_float((unsigned)hi << 16);
-  lo = bf16_rne(x - xh);
-}
 
 struct alignas(16) Smem {
   unsigned short a[4][BM][KP];   // planes: re hi, re lo, im hi, im lo
   unsigned short b[4][BN][KP];
 };
-
-__device__ inline bf16x8 ld8(const unsigned short* p) {
-  u32x4 v = *reinterpret_cast<const u32x4*>(p);
-  return __builtin_bit_cast(bf16x8, v);
-}
-__device__ inline bf16x8 neg8(bf16x8 x) {
-  u32x4 v = __builtin_bit_cast(u32x4, x);
-  v ^= 0x80008000u;
-  return __builtin_bit_cast(bf16x8, v);
-}
-__device__ inline v4f mfma(bf16x8 a, bf16x8 b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int64_t M, int64_t N, int64_t K, float2 alpha,
                                                       const float2* __restrict__ A, int64_t lda,
@@ -92,34 +61,20 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int64_t M, int64_t N, int6
     unsigned short p[4][8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      split(ra[e].x, p[0][e], p[1][e]);
-      split(ra[e].y, p[2][e], p[3][e]);
+      mg_split(ra[e].x, p[0][e], p[1][e]);
+      mg_split(ra[e].y, p[2][e], p[3][e]);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      u32x4 v;
-      v.x = p[q][0] | ((unsigned)p[q][1] << 16);
-      v.y = p[q][2] | ((unsigned)p[q][3] << 16);
-      v.z = p[q][4] | ((unsigned)p[q][5] << 16);
-      v.w = p[q][6] | ((unsigned)p[q][7] << 16);
-      *reinterpret_cast<u32x4*>(&sm.a[q][ar][ak]) = v;
-    }
+    for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4*>(&sm.a[q][ar][ak]) = mg_pack8(p[q]);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        split(rb[h * 8 + e].x, p[0][e], p[1][e]);
-        split(rb[h * 8 + e].y, p[2][e], p[3][e]);
+        mg_split(rb[h * 8 + e].x, p[0][e], p[1][e]);
+        mg_split(rb[h * 8 + e].y, p[2][e], p[3][e]);
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        u32x4 v;
-        v.x = p[q][0] | ((unsigned)p[q][1] << 16);
-        v.y = p[q][2] | ((unsigned)p[q][3] << 16);
-        v.z = p[q][4] | ((unsigned)p[q][5] << 16);
-        v.w = p[q][6] | ((unsigned)p[q][7] << 16);
-        *reinterpret_cast<u32x4*>(&sm.b[q][bc][bk + 16 * h]) = v;
-      }
+      for (int q = 0; q < 4; ++q) *reinterpret_cast<u32x4*>(&sm.b[q][bc][bk + 16 * h]) = mg_pack8(p[q]);
     }
   };
 
@@ -142,37 +97,20 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int64_t M, int64_t N, int6
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const int row = wm * 32 + a * 16 + li;
-      arh[a] = ld8(&sm.a[0][row][lg * 8]);
-      arl[a] = ld8(&sm.a[1][row][lg * 8]);
-      aih[a] = ld8(&sm.a[2][row][lg * 8]);
-      ail[a] = ld8(&sm.a[3][row][lg * 8]);
+      arh[a] = mg_ld8(&sm.a[0][row][lg * 8]);
+      arl[a] = mg_ld8(&sm.a[1][row][lg * 8]);
+      aih[a] = mg_ld8(&sm.a[2][row][lg * 8]);
+      ail[a] = mg_ld8(&sm.a[3][row][lg * 8]);
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int col = wn * 64 + b * 16 + li;
-      const bf16x8 brh = ld8(&sm.b[0][col][lg * 8]);
-      const bf16x8 brl = ld8(&sm.b[1][col][lg * 8]);
-      const bf16x8 bih = ld8(&sm.b[2][col][lg * 8]);
-      const bf16x8 bil = ld8(&sm.b[3][col][lg * 8]);
+      const bf16x8 brh = mg_ld8(&sm.b[0][col][lg * 8]);
+      const bf16x8 brl = mg_ld8(&sm.b[1][col][lg * 8]);
+      const bf16x8 bih = mg_ld8(&sm.b[2][col][lg * 8]);
+      const bf16x8 bil = mg_ld8(&sm.b[3][col][lg * 8]);
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const bf16x8 naih = neg8(aih[a]), nail = neg8(ail[a]);
-        v4f r = cr[a][b], i = ci[a][b];
-        r = mfma(arh[a], brh, r);
-        r = mfma(arh[a], brl, r);
-        r = mfma(arl[a], brh, r);
-        r = mfma(naih, bih, r);
-        r = mfma(naih, bil, r);
-        r = mfma(nail, bih, r);
-        i = mfma(arh[a], bih, i);
-        i = mfma(arh[a], bil, i);
-        i = mfma(arl[a], bih, i);
-        i = mfma(aih[a], brh, i);
-        i = mfma(aih[a], brl, i);
-        i = mfma(ail[a], brh, i);
-        cr[a][b] = r;
-        ci[a][b] = i;
-      }
+      for (int a = 0; a < 2; ++a) mg_cmac12(arh[a], arl[a], aih[a], ail[a], brh, brl, bih, bil, cr[a][b], ci[a][b]);
     }
   }
 

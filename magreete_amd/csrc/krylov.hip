@@ -1,0 +1,128 @@
+// Multi-vector kernels of the FP64 GMRES refinement.  Every right-hand side
+// (column t) runs its own Krylov process, so the Gram-Schmidt products are
+// per-column dot products, not GEMMs:
+//   mdot:  h[i][t] = sum_r conj(V_i[r][t]) w[r][t],   i < k
+//   maxpy: w[r][t] += alpha sum_i V_i[r][t] h[i][t]
+// with V_i = V + i sV an n x T row-major complex128 matrix.  Both passes are
+// HBM-bound (16 k n T bytes read).  mdot sums fixed row chunks in a fixed
+// order (partials, then one reduction kernel), so its result does not depend
+// on scheduling: the same inputs give the same bits on every call.
+#include "mg_internal.h"
+
+namespace {
+
+constexpr int kChunks = 128;   // row chunks of the dot-product pass
+constexpr int kVec = 8;        // vectors accumulated per pass over w
+
+__global__ __launch_bounds__(256) void k_mdot_part(const double2* __restrict__ V, int64_t sV, int64_t ldv, int k,
+                                                   const double2* __restrict__ w, int64_t ldw, int64_t n, int64_t T,
+                                                   int64_t rows, double2* __restrict__ part) {
+  __shared__ double2 red[4][kVec][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t c = blockIdx.x, t = (int64_t)blockIdx.y * 64 + tx;
+  const int64_t r0 = c * rows, r1 = (r0 + rows < n) ? r0 + rows : n;
+  for (int i0 = 0; i0 < k; i0 += kVec) {
+    double2 acc[kVec];
+#pragma unroll
+    for (int ii = 0; ii < kVec; ++ii) acc[ii] = make_double2(0.0, 0.0);
+    if (t < T) {
+      for (int64_t r = r0 + ty; r < r1; r += 4) {
+        const double2 x = w[r * ldw + t];
+#pragma unroll
+        for (int ii = 0; ii < kVec; ++ii) {
+          if (i0 + ii < k) {
+            const double2 v = V[(int64_t)(i0 + ii) * sV + r * ldv + t];
+            acc[ii].x += v.x * x.x + v.y * x.y;   // conj(v) x
+            acc[ii].y += v.x * x.y - v.y * x.x;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int ii = 0; ii < kVec; ++ii) red[ty][ii][tx] = acc[ii];
+    __syncthreads();
+    if (ty == 0 && t < T) {
+#pragma unroll
+      for (int ii = 0; ii < kVec; ++ii) {
+        if (i0 + ii < k) {
+          double2 s = red[0][ii][tx];
+#pragma unroll
+          for (int q = 1; q < 4; ++q) {
+            s.x += red[q][ii][tx].x;
+            s.y += red[q][ii][tx].y;
+          }
+          part[(c * k + i0 + ii) * T + t] = s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_mdot_reduce(const double2* __restrict__ part, int k, int64_t T, int nchunks,
+                              double2* __restrict__ out, int64_t ldo) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)k * T) return;
+  const int64_t i = e / T, t = e - i * T;
+  double2 s = make_double2(0.0, 0.0);
+  for (int c = 0; c < nchunks; ++c) {
+    const double2 p = part[((int64_t)c * k + i) * T + t];
+    s.x += p.x;
+    s.y += p.y;
+  }
+  out[i * ldo + t] = s;
+}
+
+__global__ __launch_bounds__(256) void k_maxpy(const double2* __restrict__ V, int64_t sV, int64_t ldv, int k,
+                                               const double2* __restrict__ h, int64_t ldh, double alpha,
+                                               double2* __restrict__ w, int64_t ldw, int64_t n, int64_t T) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * T) return;
+  const int64_t r = e / T, t = e - r * T;
+  double2 s = make_double2(0.0, 0.0);
+  for (int i = 0; i < k; ++i) {
+    const double2 v = V[(int64_t)i * sV + r * ldv + t], c = h[(int64_t)i * ldh + t];
+    s.x += v.x * c.x - v.y * c.y;
+    s.y += v.x * c.y + v.y * c.x;
+  }
+  double2 o = w[r * ldw + t];
+  o.x += alpha * s.x;
+  o.y += alpha * s.y;
+  w[r * ldw + t] = o;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t mg_mdot_work_elems(int64_t k, int64_t T) { return (k < 0 || T < 0) ? 0 : (int64_t)kChunks * k * T; }
+
+int mg_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* w, int64_t ldw, int64_t n, int64_t T,
+            void* work, void* out, int64_t ldo, void* stream) {
+  if (!V || !w || !work || !out || k < 0 || n < 0 || T < 0 || ldv < T || ldw < T || ldo < T) return MG_ERR_ARG;
+  if (k == 0 || T == 0) return MG_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int64_t rows = (n + kChunks - 1) / kChunks;
+  hipLaunchKernelGGL(k_mdot_part, dim3(kChunks, (unsigned)((T + 63) / 64)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(w), ldw, n, T,
+                     rows, reinterpret_cast<double2*>(work));
+  MG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mdot_reduce, dim3((unsigned)((k * T + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(work), k, T, kChunks, reinterpret_cast<double2*>(out), ldo);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
+             int64_t ldw, int64_t n, int64_t T, void* stream) {
+  if (!V || !h || !w || k < 0 || n < 0 || T < 0 || ldv < T || ldw < T || ldh < T) return MG_ERR_ARG;
+  if (k == 0 || n == 0 || T == 0) return MG_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_maxpy, dim3((unsigned)((n * T + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(h), ldh, alpha,
+                     reinterpret_cast<double2*>(w), ldw, n, T);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+// Block LDU factorisation of a complex-float matrix without pivoting: the
+// preconditioner of the mixed-precision solve.
+//
+// With 256 x 256 diagonal blocks, A = Lh D Uh, Lh (Uh) block unit lower
+// (upper) triangular and D = blockdiag(S_k), S_k the k-th diagonal block of
+// the successive Schur complements.  Step k of the factorisation:
+//   S_k^{-1}            one workgroup, in-place Gauss-Jordan with 32-column
+//                       panels (BF16x3 MFMA rank-32 updates, FP32 pivots);
+//   Uh_k = S_k^{-1} A12, Lh_k = A21 S_k^{-1}     stored over A12 and A21;
+//   A22 -= A21 S_k^{-1} A12                      the trailing update.
+// A solve is then GEMMs only: one per block column with Lh (forward), one
+// batched product with the S_k^{-1}, one per block column with Uh
+// (backward); no triangular kernels and no per-column dependency chains.
+// Look-ahead: once step k has updated block row and column k+1, step k+1's
+// inverse and panels run on the workspace's second stream while the rest of
+// step k's trailing update runs on the caller's stream.
+// Nothing here pivots: a zero pivot inside a diagonal block is reported in
+// info (first one, 1-based row), and the refinement that uses these factors
+// falls back to a pivoted factorisation when they are not good enough.
+#include "bf16x3.h"
+#include "lu.h"
+#include "mg_internal.h"
+
+namespace {
+
+constexpr int kIB = 256;          // diagonal block order (= kInvBlock)
+constexpr int kPW = 32;           // Gauss-Jordan panel width
+constexpr int kPK = kPW + 8;      // padded k stride of the bf16 planes
+constexpr int kInvThreads = 512;  // 8 waves; wave w owns rows 32w .. 32w+31
+
+struct alignas(16) InvSmem {
+  unsigned short r[4][kIB][kPK];   // panel rows as MFMA B operand: [plane][column][k]
+  unsigned short pa[4][kPW][kPK];  // inverse pivot block as MFMA A operand: [plane][row][k]
+  float2 p[2][kPW][kPW + 1];       // pivot block, double-buffered Gauss-Jordan steps
+};
+
+__device__ inline float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float2 crecipf(float2 a) {
+  const float d = a.x * a.x + a.y * a.y;
+  return make_float2(a.x / d, -a.y / d);
+}
+
+__device__ inline void split4(float2 v, unsigned short& rh, unsigned short& rl, unsigned short& ih,
+                              unsigned short& il) {
+  mg_split(v.x, rh, rl);
+  mg_split(v.y, ih, il);
+}
+
+// X (256 x 256, row stride 256) <- inverse of the h x h block A padded with
+// the identity.  One workgroup of 512 threads.
+__global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
+                                                               float2* __restrict__ X, int* info, int64_t row0) {
+  __shared__ InvSmem sm;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  for (int e = tid; e < kIB * kIB; e += kInvThreads) {
+    const int i = e >> 8, j = e & (kIB - 1);
+    X[e] = (i < h && j < h) ? A[(int64_t)i * lda + j] : make_float2(i == j ? 1.f : 0.f, 0.f);
+  }
+  __syncthreads();
+  for (int p = 0; p < kIB / kPW; ++p) {
+    const int c0 = p * kPW;
+    // (1) pivot block and the panel rows as bf16 planes
+    for (int e = tid; e < kPW * kPW; e += kInvThreads) {
+      const int i = e >> 5, j = e & (kPW - 1);
+      sm.p[0][i][j] = X[(c0 + i) * kIB + c0 + j];
+    }
+    for (int e = tid; e < kPW * kIB; e += kInvThreads) {
+      const int k = e >> 8, col = e & (kIB - 1);
+      split4(X[(c0 + k) * kIB + col], sm.r[0][col][k], sm.r[1][col][k], sm.r[2][col][k], sm.r[3][col][k]);
+    }
+    __syncthreads();
+    // (2) Gauss-Jordan inversion of the pivot block (no pivoting), one barrier per step
+    for (int s = 0; s < kPW; ++s) {
+      const float2(*src)[kPW + 1] = sm.p[s & 1];
+      float2(*dst)[kPW + 1] = sm.p[(s + 1) & 1];
+      const float2 piv = src[s][s];
+      if (tid == 0 && piv.x == 0.f && piv.y == 0.f && c0 + s < h) atomicCAS(info, 0, (int)(row0 + c0 + s + 1));
+      const float2 inv = crecipf(piv);
+      for (int e = tid; e < kPW * kPW; e += kInvThreads) {
+        const int i = e >> 5, j = e & (kPW - 1);
+        float2 v;
+        if (i == s && j == s) {
+          v = inv;
+        } else if (i == s) {
+          v = cmulf(src[s][j], inv);
+        } else if (j == s) {
+          const float2 t = cmulf(src[i][s], inv);
+          v = make_float2(-t.x, -t.y);
+        } else {
+          const float2 t = cmulf(src[i][s], cmulf(src[s][j], inv));
+          v = make_float2(src[i][j].x - t.x, src[i][j].y - t.y);
+        }
+        dst[i][j] = v;
+      }
+      __syncthreads();
+    }
+    // kPW steps (even): the inverse pivot block is in sm.p[0]
+    for (int e = tid; e < kPW * kPW; e += kInvThreads) {
+      const int i = e >> 5, k = e & (kPW - 1);
+      split4(sm.p[0][i][k], sm.pa[0][i][k], sm.pa[1][i][k], sm.pa[2][i][k], sm.pa[3][i][k]);
+    }
+    __syncthreads();
+    // (3) panel rows <- P^{-1} rows; the pivot block's own columns get P^{-1}.
+    // Wave w computes column tiles 2w and 2w+1 and rewrites their planes.
+    {
+      bf16x8 pa[2][4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pa[a][q] = mg_ld8(&sm.pa[q][a * 16 + li][lg * 8]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = (2 * wave + t) * 16 + li;
+        const bool own = col >= c0 && col < c0 + kPW;
+        const bf16x8 brh = mg_ld8(&sm.r[0][col][lg * 8]), brl = mg_ld8(&sm.r[1][col][lg * 8]);
+        const bf16x8 bih = mg_ld8(&sm.r[2][col][lg * 8]), bil = mg_ld8(&sm.r[3][col][lg * 8]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          v4f r = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+          mg_cmac12(pa[a][0], pa[a][1], pa[a][2], pa[a][3], brh, brl, bih, bil, r, im);
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = a * 16 + 4 * lg + rr;
+            const float2 v = own ? sm.p[0][row][col - c0] : make_float2(r[rr], im[rr]);
+            X[(c0 + row) * kIB + col] = v;
+            split4(v, sm.r[0][col][row], sm.r[1][col][row], sm.r[2][col][row], sm.r[3][col][row]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // (4) rows outside the panel: X[I][:] = X[I][:] (0 on the pivot columns) - X[I][K] R
+    if (wave != p) {
+      bf16x8 na[2][4];   // -X[I][K] as A operand planes
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int row = wave * 32 + a * 16 + li;
+        unsigned short ph[4][8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float2 v = X[row * kIB + c0 + lg * 8 + e];
+          split4(make_float2(-v.x, -v.y), ph[0][e], ph[1][e], ph[2][e], ph[3][e]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) na[a][q] = mg_as_bf16x8(mg_pack8(ph[q]));
+      }
+#pragma unroll 1
+      for (int jt = 0; jt < kIB / 16; ++jt) {
+        const int col = jt * 16 + li;
+        const bool pivcols = (jt >> 1) == p;
+        const bf16x8 brh = mg_ld8(&sm.r[0][col][lg * 8]), brl = mg_ld8(&sm.r[1][col][lg * 8]);
+        const bf16x8 bih = mg_ld8(&sm.r[2][col][lg * 8]), bil = mg_ld8(&sm.r[3][col][lg * 8]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          v4f r, im;
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = wave * 32 + a * 16 + 4 * lg + rr;
+            const float2 c = pivcols ? make_float2(0.f, 0.f) : X[row * kIB + col];
+            r[rr] = c.x;
+            im[rr] = c.y;
+          }
+          mg_cmac12(na[a][0], na[a][1], na[a][2], na[a][3], brh, brl, bih, bil, r, im);
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = wave * 32 + a * 16 + 4 * lg + rr;
+            X[row * kIB + col] = make_float2(r[rr], im[rr]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_info(int* p) { *p = 0; }
+
+}  // namespace
+
+namespace mg {
+
+int64_t ldu_dinv_elems(int64_t n) { return ((n + kIB - 1) / kIB) * (int64_t)kIB * kIB; }
+int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n; }
+
+int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
+              GemmFn gemm) {
+  if (n < 0 || lda < n) return MG_ERR_ARG;
+  hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(1), 0, s, info);
+  if (n == 0) return MG_OK;
+  static_assert(kIB == kInvBlock, "diagonal block order");
+  float2* Ap = reinterpret_cast<float2*>(A);
+  float2* D = reinterpret_cast<float2*>(dinv);
+  float2* W = reinterpret_cast<float2*>(work);
+  const GemmFn g = gemm ? gemm : &cgemm<float>;
+  const GemmFn gt = gemm ? gemm : &cgemm_trail<float>;
+  const int64_t nb = kIB, nblk = (n + nb - 1) / nb;
+  // scratch copies of A12 (h x r) and A21 (r x h) per step, double-buffered by step parity
+  auto s12 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n; };
+  auto s21 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n + nb * n; };
+  // step k's inverse and panels, on stream st
+  auto panel = [&](int64_t k, hipStream_t st) -> int {
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
+    float2* Dk = D + k * nb * nb;
+    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(kInvThreads), 0, st, Ap + r0 * lda + r0, lda, (int)h, Dk, info,
+                       r0);
+    MG_HIP(hipGetLastError());
+    if (r == 0) return MG_OK;
+    float2 *S12 = s12(k), *S21 = s21(k);
+    MG_HIP(hipMemcpy2DAsync(S12, r * sizeof(float2), Ap + r0 * lda + r1, lda * sizeof(float2), r * sizeof(float2),
+                            h, hipMemcpyDeviceToDevice, st));
+    MG_HIP(hipMemcpy2DAsync(S21, h * sizeof(float2), Ap + r1 * lda + r0, lda * sizeof(float2), h * sizeof(float2),
+                            r, hipMemcpyDeviceToDevice, st));
+    MG_TRY(g(h, r, h, cd{1, 0}, Dk, nb, S12, r, cd{0, 0}, Ap + r0 * lda + r1, lda, st));   // Uh = S^-1 A12
+    MG_TRY(g(r, h, h, cd{1, 0}, S21, h, Dk, nb, cd{0, 0}, Ap + r1 * lda + r0, lda, st));   // Lh = A21 S^-1
+    return MG_OK;
+  };
+  // A22[rows i0.., cols j0..j0+nc] -= A21 S^-1 A12 for step k (A21 from the scratch copy, Uh in place)
+  auto trail = [&](int64_t k, int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) -> int {
+    if (mr <= 0 || nc <= 0) return MG_OK;
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h;
+    return gt(mr, nc, h, cd{-1, 0}, s21(k) + (i0 - r1) * h, h, Ap + r0 * lda + j0, lda, cd{1, 0},
+              Ap + i0 * lda + j0, lda, st);
+  };
+  MG_TRY(panel(0, s));
+  for (int64_t k = 0; k < nblk; ++k) {
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
+    if (r == 0) break;
+    const int64_t h2 = (r < nb) ? r : nb;   // next block's order
+    // block row and column k+1 first
+    MG_TRY(trail(k, r1, h2, r1, r, s));
+    MG_TRY(trail(k, r1 + h2, r - h2, r1, h2, s));
+    if (ws.s2) {
+      MG_HIP(hipEventRecord(ws.evA, s));
+      MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
+      MG_TRY(panel(k + 1, ws.s2));
+      MG_HIP(hipEventRecord(ws.evB, ws.s2));
+      MG_TRY(trail(k, r1 + h2, r - h2, r1 + h2, r - h2, s));
+      MG_HIP(hipStreamWaitEvent(s, ws.evB, 0));
+    } else {
+      MG_TRY(trail(k, r1 + h2, r - h2, r1 + h2, r - h2, s));
+      MG_TRY(panel(k + 1, s));
+    }
+  }
+  return MG_OK;
+}
+
+int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb, int64_t nrhs,
+              void* work, hipStream_t s, GemmFn gemm) {
+  if (n < 0 || nrhs < 0 || lda < n || ldb < nrhs) return MG_ERR_ARG;
+  if (n == 0 || nrhs == 0) return MG_OK;
+  const GemmFn g = gemm ? gemm : &cgemm<float>;
+  const float2* Ap = reinterpret_cast<const float2*>(LDU);
+  const float2* D = reinterpret_cast<const float2*>(dinv);
+  float2* Bp = reinterpret_cast<float2*>(B);
+  float2* W = reinterpret_cast<float2*>(work);
+  const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
+  // forward: B[r1:] -= Lh[r1:, block k] B[block k]
+  for (int64_t k = 0; k + 1 < nblk; ++k) {
+    const int64_t r0 = k * nb, r1 = r0 + nb;
+    MG_TRY(g(n - r1, nrhs, nb, cd{-1, 0}, Ap + r1 * lda + r0, lda, Bp + r0 * ldb, ldb, cd{1, 0}, Bp + r1 * ldb, ldb,
+             s));
+  }
+  // W = D^{-1} B, block by block (FP32 products)
+  if (full > 0)
+    MG_TRY(cgemm_batched<float>(nb, nrhs, nb, cd{1, 0}, D, nb, nb * nb, Bp, ldb, nb * ldb, cd{0, 0}, W, nrhs,
+                                nb * nrhs, full, s));
+  if (full < nblk) {
+    const int64_t r0 = full * nb, h = n - r0;
+    MG_TRY(cgemm<float>(h, nrhs, h, cd{1, 0}, D + full * nb * nb, nb, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * nrhs,
+                        nrhs, s));
+  }
+  // backward: W[:r0] -= Uh[:r0, block k] W[block k]
+  for (int64_t k = nblk - 1; k > 0; --k) {
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb;
+    MG_TRY(g(r0, nrhs, h, cd{-1, 0}, Ap + r0, lda, W + r0 * nrhs, nrhs, cd{1, 0}, W, nrhs, s));
+  }
+  MG_HIP(hipMemcpy2DAsync(Bp, ldb * sizeof(float2), W, nrhs * sizeof(float2), nrhs * sizeof(float2), n,
+                          hipMemcpyDeviceToDevice, s));
+  return MG_OK;
+}
+
+}  // namespace mg

@@ -77,6 +77,18 @@ constexpr int64_t kInvBlock = 256;
 inline int64_t dinv_elems(int64_t n) { return 2 * ((n + kInvBlock - 1) / kInvBlock) * kInvBlock * kInvBlock; }
 template <typename T>
 int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_t s);
+// Block LDU without pivoting (complex float): inverses of the 256 x 256
+// diagonal blocks of the Schur complements in dinv (ldu_dinv_elems), scaled
+// off-diagonal blocks in place, work: ldu_work_elems(n) elements; the solve
+// needs n x nrhs work elements.  gemm: GEMM of the panels, trailing updates
+// and block solves (nullptr: the FP32 MFMA GEMM).
+int64_t ldu_dinv_elems(int64_t n);
+int64_t ldu_work_elems(int64_t n);
+int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
+              GemmFn gemm);
+int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb, int64_t nrhs,
+              void* work, hipStream_t s, GemmFn gemm);
+
 // gemm: GEMM of the block solves (nullptr: the FP64/FP32 MFMA GEMM of T).
 template <typename T>
 int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv, void* B, int64_t ldb,

@@ -111,8 +111,13 @@ def getrs(LU, ipiv, B, prec="c128"):
 
 
 class Factor:
-    """An LU factorisation resident on the device: factors, pivots, info and
-    the diagonal-block inverses that make every later solve GEMM-only."""
+    """A factorisation resident on the device, every later solve GEMM-only.
+
+    pivot=True: LU with partial pivoting (getrf) plus the inverses of the
+    256 x 256 diagonal blocks of L and U.  pivot=False (complex-float
+    precisions only): block LDU without pivoting, A = Lh D Uh, with the
+    inverses of the D blocks; the preconditioner of the mixed-precision
+    refinement."""
 
     def __init__(self, A, prec="c128", pivot=True):
         lib = L.load()
@@ -120,17 +125,23 @@ class Factor:
         dev = A.device
         if pivot:
             self.ipiv, self.info = getrf(A, prec)
+            self.dinv = torch.empty(lib.mg_dinv_elems(self.n), dtype=A.dtype, device=dev)
+            with torch.cuda.device(dev):
+                st = lib.mg_getrf_dinv(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.dinv),
+                                       _stream(dev))
+            L.check(st, "mg_getrf_dinv")
         else:
+            if A.dtype != torch.complex64:
+                raise ValueError("the unpivoted block LDU factors complex64 matrices")
             self.ipiv = None
             self.info = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.dinv = torch.empty(lib.mg_ldu_dinv_elems(self.n), dtype=A.dtype, device=dev)
+            work = torch.empty(lib.mg_ldu_work_elems(self.n), dtype=A.dtype, device=dev)
             with torch.cuda.device(dev):
-                st = lib.mg_getrf_nopiv(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.info),
-                                        _stream(dev))
-            L.check(st, "mg_getrf_nopiv")
-        self.dinv = torch.empty(lib.mg_dinv_elems(self.n), dtype=A.dtype, device=dev)
-        with torch.cuda.device(dev):
-            st = lib.mg_getrf_dinv(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.dinv), _stream(dev))
-        L.check(st, "mg_getrf_dinv")
+                st = lib.mg_getrf_ldu(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.dinv), _p(work),
+                                      _p(self.info), _stream(dev))
+            L.check(st, "mg_getrf_ldu")
+            del work          # stream-ordered: the factorisation's last step waits on this stream
         self._work = None
 
     def solve(self, B):
@@ -140,9 +151,14 @@ class Factor:
         if self._work is None or self._work.numel() < B.shape[0] * B.shape[1]:
             self._work = torch.empty(B.shape[0] * B.shape[1], dtype=self.A.dtype, device=dev)
         with torch.cuda.device(dev):
-            st = lib.mg_getrs_dinv(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A), _p(self.ipiv),
-                                   _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(self._work), _stream(dev))
-        L.check(st, "mg_getrs_dinv")
+            if self.pivot:
+                st = lib.mg_getrs_dinv(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A),
+                                       _p(self.ipiv), _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(self._work),
+                                       _stream(dev))
+            else:
+                st = lib.mg_getrs_ldu(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A),
+                                      _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(self._work), _stream(dev))
+        L.check(st, "mg_getrs")
         return B
 
 
@@ -215,6 +231,41 @@ def coldot(Y, X):
         st = lib.mg_coldot(_p(Y), _ld(Y), _p(X), _ld(X), Y.shape[0], Y.shape[1], _p(out), _stream(dev))
     L.check(st, "mg_coldot")
     return out
+
+
+def _vecs(V):
+    """(pointer, vector stride, row stride) of a stack V (k, n, T) of row-major matrices."""
+    assert V.dim() == 3 and V.stride(2) == 1, "Krylov vectors: (k, n, T) with unit column stride"
+    return _p(V), V.stride(0), V.stride(1)
+
+
+def mdot(V, w, out=None, work=None):
+    """h[i, t] = sum_r conj(V[i, r, t]) w[r, t] (per-column dot products, fixed summation order)."""
+    lib = L.load()
+    dev = w.device
+    k, n, T = V.shape
+    if out is None:
+        out = torch.empty((k, T), dtype=torch.complex128, device=dev)
+    need = lib.mg_mdot_work_elems(k, T)
+    if work is None or work.numel() < need:
+        work = torch.empty(max(need, 1), dtype=torch.complex128, device=dev)
+    pv, sv, ldv = _vecs(V)
+    with torch.cuda.device(dev):
+        st = lib.mg_mdot(pv, sv, ldv, k, _p(w), _ld(w), n, T, _p(work), _p(out), _ld(out), _stream(dev))
+    L.check(st, "mg_mdot")
+    return out
+
+
+def maxpy(V, h, w, alpha=1.0):
+    """w[r, t] += alpha sum_i V[i, r, t] h[i, t] (in place)."""
+    lib = L.load()
+    dev = w.device
+    k, n, T = V.shape
+    pv, sv, ldv = _vecs(V)
+    with torch.cuda.device(dev):
+        st = lib.mg_maxpy(pv, sv, ldv, k, _p(h), _ld(h), float(alpha), _p(w), _ld(w), n, T, _stream(dev))
+    L.check(st, "mg_maxpy")
+    return w
 
 
 def hankel1(nu, x, exact=False):

@@ -1,14 +1,17 @@
 """Mixed-precision solve: complex64 LU (FP32 or BF16x3 MFMA products) plus
 FP64 GMRES iterative refinement, with fallbacks.
 
-GMRES is right-preconditioned by the low-precision factors P = L U:
-x = x0 + Z y with z_j = P^{-1} v_j, the Krylov basis built from A z_j in
-FP64 (A = M, complex128, kept resident).  All T right-hand sides run
-together: the matrix products are one (n x n) @ (n x T) FP64 GEMM and one
-complex64 multi-RHS solve (GEMM-only, through stored diagonal-block
-inverses) per iteration; the per-column Gram-Schmidt and Givens updates are
-small (T x j) operations.  Convergence is declared on the true FP64 residual
-||b - A x|| / ||b|| <= tol per column.
+GMRES is right-preconditioned by the low-precision factors P = L U and
+flexible (the preconditioned directions are kept): x = Z y with
+z_j = P^{-1} v_j, the Krylov basis built from A z_j in FP64 (A = M,
+complex128, kept resident), starting from x = 0 so that the first direction
+is P^{-1} b at no FP64 cost.  All T right-hand sides run together: the
+matrix products are one (n x n) @ (n x T) FP64 GEMM and one complex64
+multi-RHS solve (GEMM-only, through stored diagonal-block inverses) per
+iteration; the per-column Gram-Schmidt runs as HIP multi-vector dot/axpy
+passes and the Givens updates are small (T x j) torch operations.
+Convergence is declared on the true FP64 residual ||b - A x|| / ||b|| <= tol
+per column, computed at the end of every cycle.
 
 The first factorisation is unpivoted (no pivot search, no grid barrier, no
 row swap); the refinement restores the accuracy.  If it breaks down (an exact
@@ -18,15 +21,20 @@ The returned stats say which tier answered.
 """
 import torch
 
+from . import _lib as L
 from . import engine as E
 
 
 def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
-    """Refine X (initially unset) for M X = B with preconditioner F.
-    Returns (converged, iters, restarts, residual)."""
+    """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F.
+    The first Krylov direction is P^{-1} b, so the first cycle starts with no
+    FP64 product; every cycle ends with the true residual b - M x in FP64.
+    Returns (converged, FP64 products, cycles, residual)."""
     dev = M.device
     n, T = B.shape
     lowp = torch.empty((n, T), dtype=torch.complex64, device=dev)
+    m_max = max(1, min(restart, max_iters - 1))
+    work = torch.empty(L.load().mg_mdot_work_elems(m_max + 1, T), dtype=torch.complex128, device=dev)
 
     def precond(v, out):
         E.convert(v, lowp)
@@ -34,23 +42,20 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
         E.convert(lowp, out)
         return out
 
-    precond(B, X)                                          # x0 = P^{-1} b
-    R = torch.empty_like(B)
-    iters = restarts = 0
+    X.zero_()
+    R = B.clone()
+    beta = torch.linalg.norm(R, dim=0)
+    rel = (beta / bnorm).max().item()
+    iters = cycles = 0
     prev = None
     while True:
-        R.copy_(B)
-        E.cgemm(M, X, R, alpha=-1.0, beta=1.0)              # r = b - A x  (FP64)
-        iters += 1
-        beta = torch.linalg.norm(R, dim=0)
-        rel = (beta / bnorm).max().item()
         if rel <= tol:
-            return True, iters, restarts, rel
+            return True, iters, cycles, rel
         if not (rel == rel) or iters >= max_iters or (prev is not None and rel > 0.5 * prev):
-            return False, iters, restarts, rel
+            return False, iters, cycles, rel
         prev = rel
-        restarts += 1
-        m = min(restart, max_iters - iters)
+        cycles += 1
+        m = max(1, min(restart, max_iters - iters - 1))    # keep one product for the closing residual
         V = torch.empty((m + 1, n, T), dtype=torch.complex128, device=dev)
         Z = torch.empty((m, n, T), dtype=torch.complex128, device=dev)
         H = torch.zeros((m + 1, m, T), dtype=torch.complex128, device=dev)
@@ -58,7 +63,7 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
         sn = torch.zeros((m, T), dtype=torch.complex128, device=dev)
         g = torch.zeros((m + 1, T), dtype=torch.complex128, device=dev)
         safe = torch.where(beta == 0, torch.ones_like(beta), beta)
-        V[0] = R / safe
+        torch.div(R, safe, out=V[0])
         g[0] = beta.to(torch.complex128)
         k = 0
         for j in range(m):
@@ -67,8 +72,8 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
             E.cgemm(M, Z[j], w, alpha=1.0, beta=0.0)          # w = A z_j (FP64)
             iters += 1
             for _ in range(2):                                # classical Gram-Schmidt, twice
-                h = torch.einsum('int,nt->it', V[: j + 1].conj(), w)
-                w -= torch.einsum('int,it->nt', V[: j + 1], h)
+                h = E.mdot(V[: j + 1], w, work=work)
+                E.maxpy(V[: j + 1], h, w, alpha=-1.0)
                 H[: j + 1, j] += h
             hn = torch.linalg.norm(w, dim=0)
             H[j + 1, j] = hn.to(torch.complex128)
@@ -88,20 +93,25 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
             g[j] = cs[j].conj() * g[j]
             k = j + 1
             est = (g[j + 1].abs() / bnorm).max().item()
-            if est <= 0.1 * tol or iters >= max_iters:
+            if est <= 0.1 * tol:
                 break
         Hk = H[:k, :k].permute(2, 0, 1).contiguous()
         gk = g[:k].T.contiguous().unsqueeze(-1)
-        y = torch.linalg.solve_triangular(Hk, gk, upper=True).squeeze(-1).T   # (k, T)
-        X += torch.einsum('knt,kt->nt', Z[:k], y)
+        y = torch.linalg.solve_triangular(Hk, gk, upper=True).squeeze(-1).T.contiguous()   # (k, T)
+        E.maxpy(Z[:k], y, X, alpha=1.0)                        # x += Z y
         del V, Z, H
+        R.copy_(B)
+        E.cgemm(M, X, R, alpha=-1.0, beta=1.0)                # r = b - A x  (FP64)
+        iters += 1
+        beta = torch.linalg.norm(R, dim=0)
+        rel = (beta / bnorm).max().item()
 
 
 def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False):
     """Solve M X = B to a relative FP64 residual <= tol.  M (complex128) is
     left intact; M32 holds M in complex64 on entry (overwritten by factors).
     Returns stats: tier ('nopiv', 'pivoted' or 'fp64'), iters (FP64
-    products), restarts, residual, fallback, info."""
+    products), cycles (GMRES restarts + 1), residual, fallback, info."""
     bnorm = torch.linalg.norm(B, dim=0)
     bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
     tiers = [True] if pivot else [False, True]
@@ -112,11 +122,11 @@ def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, p
         F = E.Factor(M32, prec, pivot=piv)
         if not piv and int(F.info.item()) != 0:
             continue                                       # exact zero pivot: pivot this time
-        ok, iters, restarts, rel = _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart)
+        ok, iters, cycles, rel = _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart)
         total += iters
         if ok:
             return {"tier": "pivoted" if piv else "nopiv", "fallback": idx > 0, "iters": total,
-                    "restarts": restarts, "residual": rel, "info": F.info}
+                    "cycles": cycles, "residual": rel, "info": F.info}
     # refinement stalled in low precision: complex128 LU
     F64 = E.Factor(M.clone(), "c128")
     X.copy_(B)
@@ -124,4 +134,4 @@ def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, p
     R = B.clone()
     E.cgemm(M, X, R, alpha=-1.0, beta=1.0)
     rel = (torch.linalg.norm(R, dim=0) / bnorm).max().item()
-    return {"tier": "fp64", "fallback": True, "iters": total + 1, "restarts": 0, "residual": rel, "info": F64.info}
+    return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel, "info": F64.info}

@@ -208,3 +208,21 @@ def test_propagate_kernel_vs_oracle(kind):
     ref = alpha * (G @ Ek) + E0
     err = (out.cpu() - ref).abs().max() / ref.abs().max()
     assert err < 1e-13, err
+
+
+@pytest.mark.parametrize("k,n,T", [(1, 1, 1), (3, 100, 70), (9, 777, 256), (13, 2000, 65)])
+def test_krylov_mdot_maxpy(k, n, T):
+    """Per-column Gram-Schmidt passes against numpy; the dot products repeat bit for bit."""
+    say("krylov", k, n, T)
+    rng = np.random.default_rng(k * n + T)
+    V = crand((k + 1, n, T), rng)
+    w = crand((n, T), rng)
+    Vd = torch.from_numpy(V).to(DEV)[:k]              # a view with the stack's strides
+    wd = torch.from_numpy(w).to(DEV)
+    h = E.mdot(Vd, wd)
+    ref = np.einsum("int,nt->it", V[:k].conj(), w)
+    assert np.abs(h.cpu().numpy() - ref).max() < 1e-12 * np.abs(ref).max() + 1e-13
+    assert torch.equal(h, E.mdot(Vd, wd))
+    E.maxpy(Vd, h, wd, alpha=-0.5)
+    ref_w = w - 0.5 * np.einsum("int,it->nt", V[:k], ref)
+    assert np.abs(wd.cpu().numpy() - ref_w).max() < 1e-12 * np.abs(ref_w).max()

@@ -24,9 +24,10 @@ def crand(shape, rng):
 
 
 @pytest.mark.parametrize("prec", ["c64", "bf16x3"])
-@pytest.mark.parametrize("n", [7, 256, 300, 777, 1100])
-def test_nopiv_factor_solve(n, prec):
-    say("nopiv", n, prec)
+@pytest.mark.parametrize("n", [7, 256, 300, 512, 777, 1100])
+def test_ldu_factor_solve(n, prec):
+    """Unpivoted block LDU (the preconditioner): one solve against LAPACK."""
+    say("ldu", n, prec)
     rng = np.random.default_rng(n)
     A = crand((n, n), rng) / np.sqrt(2 * n) + 2.0 * np.eye(n)
     B = crand((n, 9), rng)
@@ -36,8 +37,9 @@ def test_nopiv_factor_solve(n, prec):
     F.solve(Bd)
     ref = np.linalg.solve(A, B)
     x = Bd.cpu().numpy().astype(np.complex128)
-    # BF16x3 products carry ~16 significant bits: about 1e-5 per product
-    tol = 1e-5 if prec == "c64" else 3e-4
+    # the diagonal-block inverses (and, for bf16x3, every product) carry ~16
+    # significant bits: about 1e-5 per product
+    tol = 3e-4
     assert (np.linalg.norm(x - ref, axis=0) / np.linalg.norm(ref, axis=0)).max() < tol
 
 
@@ -74,6 +76,17 @@ def test_mixed_matches_fp64_and_oracle(method):
     cE = o.propagate(w.measure, cEk, w.k0, w.alpha, cE0m)
     assert colerr(sm.X.cpu(), cEk) < 1e-9
     assert colerr(Fm.cpu(), cE.reshape(Fm.shape)) < 1e-9
+
+
+def test_ldu_zero_pivot_info():
+    say("ldu info")
+    rng = np.random.default_rng(5)
+    n = 600
+    A = crand((n, n), rng) / np.sqrt(2 * n) + 2.0 * np.eye(n)
+    A[300, :] = 0.0
+    A[:, 300] = 0.0          # block 1 (rows 256..511) has an exactly zero pivot at row 301 (1-based)
+    F = E.Factor(torch.from_numpy(A.astype(np.complex64)).to(DEV), "c64", pivot=False)
+    assert int(F.info.item()) == 301
 
 
 def test_zero_pivot_falls_back_to_pivoted():
