@@ -27,6 +27,8 @@ constexpr int kIB = 256;          // diagonal block order (= kInvBlock)
 constexpr int kPW = 32;           // Gauss-Jordan panel width
 constexpr int kPK = kPW + 8;      // padded k stride of the bf16 planes
 constexpr int kInvThreads = 512;  // 8 waves; wave w owns rows 32w .. 32w+31
+constexpr int kGrp = 2;           // column tiles per load group of the rank update
+static_assert(2 * kInvThreads == kPW * kPW && kIB == 8 * 32, "thread layout of k_block_inverse");
 
 struct alignas(16) InvSmem {
   unsigned short r[4][kIB][kPK];   // panel rows as MFMA B operand: [plane][column][k]
@@ -70,30 +72,24 @@ __global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __r
       split4(X[(c0 + k) * kIB + col], sm.r[0][col][k], sm.r[1][col][k], sm.r[2][col][k], sm.r[3][col][k]);
     }
     __syncthreads();
-    // (2) Gauss-Jordan inversion of the pivot block (no pivoting), one barrier per step
-    for (int s = 0; s < kPW; ++s) {
-      const float2(*src)[kPW + 1] = sm.p[s & 1];
-      float2(*dst)[kPW + 1] = sm.p[(s + 1) & 1];
-      const float2 piv = src[s][s];
-      if (tid == 0 && piv.x == 0.f && piv.y == 0.f && c0 + s < h) atomicCAS(info, 0, (int)(row0 + c0 + s + 1));
-      const float2 inv = crecipf(piv);
-      for (int e = tid; e < kPW * kPW; e += kInvThreads) {
-        const int i = e >> 5, j = e & (kPW - 1);
-        float2 v;
-        if (i == s && j == s) {
-          v = inv;
-        } else if (i == s) {
-          v = cmulf(src[s][j], inv);
-        } else if (j == s) {
-          const float2 t = cmulf(src[i][s], inv);
-          v = make_float2(-t.x, -t.y);
-        } else {
-          const float2 t = cmulf(src[i][s], cmulf(src[s][j], inv));
-          v = make_float2(src[i][j].x - t.x, src[i][j].y - t.y);
-        }
-        dst[i][j] = v;
+    // (2) Gauss-Jordan inversion of the pivot block (no pivoting), one barrier
+    // per step; thread t updates elements (t/32, t%32) and (t/32 + 16, t%32)
+    {
+      const int i0 = tid >> 5, i1 = i0 + 16, j = tid & (kPW - 1);
+      for (int s = 0; s < kPW; ++s) {
+        const float2(*src)[kPW + 1] = sm.p[s & 1];
+        float2(*dst)[kPW + 1] = sm.p[(s + 1) & 1];
+        const float2 piv = src[s][s], rs = src[s][j];
+        const float2 q0 = src[i0][s], q1 = src[i1][s], a0 = src[i0][j], a1 = src[i1][j];
+        if (tid == 0 && piv.x == 0.f && piv.y == 0.f && c0 + s < h) atomicCAS(info, 0, (int)(row0 + c0 + s + 1));
+        const float rd = __builtin_amdgcn_rcpf(piv.x * piv.x + piv.y * piv.y);
+        const float2 inv = make_float2(piv.x * rd, -piv.y * rd);
+        const float2 u = (j == s) ? inv : cmulf(rs, inv);   // row s of the result
+        const float2 t0 = cmulf(q0, (j == s) ? inv : u), t1 = cmulf(q1, (j == s) ? inv : u);
+        dst[i0][j] = (i0 == s) ? u : (j == s) ? make_float2(-t0.x, -t0.y) : make_float2(a0.x - t0.x, a0.y - t0.y);
+        dst[i1][j] = (i1 == s) ? u : (j == s) ? make_float2(-t1.x, -t1.y) : make_float2(a1.x - t1.x, a1.y - t1.y);
+        __syncthreads();
       }
-      __syncthreads();
     }
     // kPW steps (even): the inverse pivot block is in sm.p[0]
     for (int e = tid; e < kPW * kPW; e += kInvThreads) {
@@ -122,7 +118,8 @@ __global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __r
 #pragma unroll
           for (int rr = 0; rr < 4; ++rr) {
             const int row = a * 16 + 4 * lg + rr;
-            const float2 v = own ? sm.p[0][row][col - c0] : make_float2(r[rr], im[rr]);
+            const float2 pv = sm.p[0][row][(col - c0) & (kPW - 1)];
+            const float2 v = own ? pv : make_float2(r[rr], im[rr]);
             X[(c0 + row) * kIB + col] = v;
             split4(v, sm.r[0][col][row], sm.r[1][col][row], sm.r[2][col][row], sm.r[3][col][row]);
           }
@@ -145,32 +142,48 @@ __global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __r
 #pragma unroll
         for (int q = 0; q < 4; ++q) na[a][q] = mg_as_bf16x8(mg_pack8(ph[q]));
       }
+      // kGrp column tiles per group: their loads are issued together
 #pragma unroll 1
-      for (int jt = 0; jt < kIB / 16; ++jt) {
-        const int col = jt * 16 + li;
-        const bool pivcols = (jt >> 1) == p;
-        const bf16x8 brh = mg_ld8(&sm.r[0][col][lg * 8]), brl = mg_ld8(&sm.r[1][col][lg * 8]);
-        const bf16x8 bih = mg_ld8(&sm.r[2][col][lg * 8]), bil = mg_ld8(&sm.r[3][col][lg * 8]);
+      for (int jg = 0; jg < kIB / (16 * kGrp); ++jg) {
+        float2 c[kGrp][2][4];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          v4f r, im;
+        for (int t = 0; t < kGrp; ++t) {
+          const int jt = jg * kGrp + t, col = jt * 16 + li;
+          const bool pivcols = (jt >> 1) == p;
 #pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const int row = wave * 32 + a * 16 + 4 * lg + rr;
-            const float2 c = pivcols ? make_float2(0.f, 0.f) : X[row * kIB + col];
-            r[rr] = c.x;
-            im[rr] = c.y;
-          }
-          mg_cmac12(na[a][0], na[a][1], na[a][2], na[a][3], brh, brl, bih, bil, r, im);
+          for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const int row = wave * 32 + a * 16 + 4 * lg + rr;
-            X[row * kIB + col] = make_float2(r[rr], im[rr]);
+            for (int rr = 0; rr < 4; ++rr) {
+              const int row = wave * 32 + a * 16 + 4 * lg + rr;
+              const float2 v = X[row * kIB + col];   // unconditional: keeps the loads in flight together
+              c[t][a][rr] = pivcols ? make_float2(0.f, 0.f) : v;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kGrp; ++t) {
+          const int col = (jg * kGrp + t) * 16 + li;
+          const bf16x8 brh = mg_ld8(&sm.r[0][col][lg * 8]), brl = mg_ld8(&sm.r[1][col][lg * 8]);
+          const bf16x8 bih = mg_ld8(&sm.r[2][col][lg * 8]), bil = mg_ld8(&sm.r[3][col][lg * 8]);
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            v4f r, im;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+              r[rr] = c[t][a][rr].x;
+              im[rr] = c[t][a][rr].y;
+            }
+            mg_cmac12(na[a][0], na[a][1], na[a][2], na[a][3], brh, brl, bih, bil, r, im);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+              const int row = wave * 32 + a * 16 + 4 * lg + rr;
+              X[row * kIB + col] = make_float2(r[rr], im[rr]);
+            }
           }
         }
       }
     }
     __syncthreads();
+    // end of panel p
   }
 }
 
