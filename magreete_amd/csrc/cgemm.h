@@ -25,6 +25,15 @@ __device__ inline v4d mg_mfma(double a, double b, v4d c) {
 __device__ inline v4f mg_mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+// c - a b: on gfx950 the F64 MFMA reads its blgp field as per-operand negate
+// bits (blgp = 1 assembles as neg:[1,0,0]), so the sign flip costs nothing;
+// the F32 MFMA has no such modifier.
+__device__ inline v4d mg_mfma_nega(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
+__device__ inline v4f mg_mfma_nega(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(-a, b, c, 0, 0, 0);
+}
 
 // Map a linear workgroup id to (tile_m, tile_n) so that the tiles one XCD
 // works on are contiguous groups of rows: ids are dealt round-robin over the
@@ -237,11 +246,10 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
       }
 #pragma unroll
       for (int a = 0; a < TM; ++a) {
-        const T naim = -aim[a];
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
           cr[a][b] = mg_mfma(are[a], bre[b], cr[a][b]);
-          cr[a][b] = mg_mfma(naim, bim[b], cr[a][b]);
+          cr[a][b] = mg_mfma_nega(aim[a], bim[b], cr[a][b]);
           ci[a][b] = mg_mfma(are[a], bim[b], ci[a][b]);
           ci[a][b] = mg_mfma(aim[a], bre[b], ci[a][b]);
         }

@@ -34,17 +34,6 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_trail(int64_t M, int64_t
 }
 
 template <typename T>
-struct GemmCfg;
-template <>
-struct GemmCfg<double> {
-  static constexpr int BM = 64, BN = 64, BK = 8, WM = 2, WN = 2;
-};
-template <>
-struct GemmCfg<float> {
-  static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2;
-};
-
-template <typename T>
 typename Cplx<T>::t to_t2(mg::cd z) {
   typename Cplx<T>::t r;
   r.x = (T)z.re;
@@ -52,9 +41,57 @@ typename Cplx<T>::t to_t2(mg::cd z) {
   return r;
 }
 
+using mg::cd;
+
+// Tile configurations.  FP64: 128 x 128 tiles (8 waves of 64 x 32) when the
+// grid still gives every CU two tiles, which halves the operand re-reads
+// from L2 against 64 x 64; otherwise 64 x 64 with k-steps of 16.  FP32:
+// 64 x 64 (measured fastest on gfx950 for both the LU update shape and the
+// tall-skinny residual shape).
+template <typename T, bool TRAIL, int BM, int BN, int BK, int WM, int WN>
+int launch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+           int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
+  const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
+  MemA a{A, lda};
+  if (TRAIL)
+    hipLaunchKernelGGL((k_cgemm_trail<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N, K,
+                       to_t2<T>(alpha), a, B, ldb, to_t2<T>(beta), C, ldc, nbm, nbn);
+  else
+    hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn, (unsigned)batch), dim3(64 * WM * WN), 0, s,
+                       M, N, K, to_t2<T>(alpha), a, sA, B, ldb, sB, to_t2<T>(beta), C, ldc, sC, nbm, nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+template <typename T, bool TRAIL>
+int dispatch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+             int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
+  if constexpr (std::is_same<T, double>::value) {
+    const int64_t big = ((M + 127) / 128) * ((N + 127) / 128) * batch;
+    if (big >= 2 * (int64_t)mg::cu_count())
+      return launch<T, TRAIL, 128, 128, 8, 2, 4>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                                  s);
+    return launch<T, TRAIL, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
+  } else {
+    return launch<T, TRAIL, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
+  }
+}
+
 }  // namespace
 
 namespace mg {
+
+int cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cache[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cache[dev] = v;
+  }
+  return cache[dev];
+}
 
 template <typename T>
 int cgemm_batched(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
@@ -62,14 +99,7 @@ int cgemm_batched(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int6
                   hipStream_t s) {
   if (M < 0 || N < 0 || K < 0 || batch < 0 || batch > 65535) return MG_ERR_ARG;
   if (M == 0 || N == 0 || batch == 0) return MG_OK;
-  typedef GemmCfg<T> G;
-  const int nbm = (int)((M + G::BM - 1) / G::BM), nbn = (int)((N + G::BN - 1) / G::BN);
-  MemA a{A, lda};
-  hipLaunchKernelGGL((k_cgemm<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn, (unsigned)batch),
-                     dim3(64 * G::WM * G::WN), 0, s, M, N, K, to_t2<T>(alpha), a, sA, B, ldb, sB, to_t2<T>(beta), C,
-                     ldc, sC, nbm, nbn);
-  MG_HIP(hipGetLastError());
-  return MG_OK;
+  return dispatch<T, false>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
 }
 
 template <typename T>
@@ -83,14 +113,7 @@ int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
                 cd beta, void* C, int64_t ldc, hipStream_t s) {
   if (M < 0 || N < 0 || K < 0) return MG_ERR_ARG;
   if (M == 0 || N == 0) return MG_OK;
-  typedef GemmCfg<T> G;
-  const int nbm = (int)((M + G::BM - 1) / G::BM), nbn = (int)((N + G::BN - 1) / G::BN);
-  MemA a{A, lda};
-  hipLaunchKernelGGL((k_cgemm_trail<T, G::BM, G::BN, G::BK, G::WM, G::WN>), dim3(nbm * nbn),
-                     dim3(64 * G::WM * G::WN), 0, s, M, N, K, to_t2<T>(alpha), a, B, ldb, to_t2<T>(beta), C, ldc,
-                     nbm, nbn);
-  MG_HIP(hipGetLastError());
-  return MG_OK;
+  return dispatch<T, true>(M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, s);
 }
 
 #define MG_INST(T)                                                                                                 \

@@ -48,11 +48,7 @@ int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
 int cgemm_bf16x3(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
                  int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
 
-// Device properties of the current device (cached).
-struct DevInfo {
-  int cus = 0;
-  int coop_blocks_getf2 = 0;
-};
-DevInfo& dev_info();
+// Compute units of the current device (cached per device).
+int cu_count();
 
 }  // namespace mg

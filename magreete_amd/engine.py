@@ -55,8 +55,10 @@ def _pts(x, device):
 
 
 def _ld(t):
-    if t.dim() != 2 or t.stride(1) != 1:
+    if t.dim() != 2 or (t.stride(1) != 1 and t.shape[1] > 1):
         raise ValueError("expected a row-major 2-D tensor")
+    if t.shape[0] <= 1:
+        return max(t.shape[1], 1)      # torch leaves the stride of a size-1 dimension arbitrary
     return t.stride(0)
 
 
@@ -235,8 +237,12 @@ def coldot(Y, X):
 
 def _vecs(V):
     """(pointer, vector stride, row stride) of a stack V (k, n, T) of row-major matrices."""
-    assert V.dim() == 3 and V.stride(2) == 1, "Krylov vectors: (k, n, T) with unit column stride"
-    return _p(V), V.stride(0), V.stride(1)
+    k, n, T = V.shape
+    if V.dim() != 3 or (V.stride(2) != 1 and T > 1):
+        raise ValueError("Krylov vectors: (k, n, T) with unit column stride")
+    ld = V.stride(1) if n > 1 else max(T, 1)           # size-1 dimensions carry arbitrary strides
+    sv = V.stride(0) if k > 1 else n * ld
+    return _p(V), sv, ld
 
 
 def mdot(V, w, out=None, work=None):
