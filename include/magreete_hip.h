@@ -71,11 +71,6 @@ int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv,
  * solve step of torch.linalg.solve at the same call sites. */
 int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb,
              int64_t nrhs, void* stream);
-/* LU without pivoting, in place: the low-precision factorisation that
- * preconditions the FP64 GMRES refinement of the mixed-precision solve (no
- * counterpart in the reference).  info: first exactly-zero pivot or 0.
- * Solve with mg_getrs_dinv and ipiv = NULL. */
-int mg_getrf_nopiv(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* info, void* stream);
 /* Stored inverses of the 256 x 256 diagonal blocks of L and U, so that the
  * solve runs as GEMMs only (repeated solves against a resident factor; same
  * call sites as mg_getrs).  dinv holds mg_dinv_elems(n) elements of the
@@ -97,6 +92,11 @@ int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* 
                  void* stream);
 int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B,
                  int64_t ldb, int64_t nrhs, void* work, void* stream);
+/* Cap on the workgroups of one LU leaf launch of this handle (g <= 0: the CU
+ * count).  The leaf synchronises its workgroups through a grid barrier, so
+ * factorisations that run at the same time on one device (one handle each)
+ * should keep the sum of their caps within the CU count. */
+int mg_set_leaf_blocks(mg_handle* h, int g);
 /* Waits for `stream`; MG_ERR_SYNC if a grid barrier of an LU leaf failed. */
 int mg_check(mg_handle* h, void* stream);
 /* Measurement hooks with no reference counterpart: HIP-event timing of the

@@ -35,8 +35,8 @@ SIGNATURES = {
     "mg_getrf": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp],
     "mg_getrs": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp],
     "mg_check": [_vp, _vp],
+    "mg_set_leaf_blocks": [_vp, _int],
     "mg_dinv_elems": [_i64],
-    "mg_getrf_nopiv": [_vp, _int, _vp, _i64, _i64, _vp, _vp],
     "mg_getrf_dinv": [_vp, _int, _vp, _i64, _i64, _vp, _vp],
     "mg_getrs_dinv": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp],
     "mg_profile_enable": [_vp, _int],

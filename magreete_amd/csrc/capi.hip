@@ -96,19 +96,6 @@ int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv,
   }
 }
 
-// LU without pivoting in place (the mixed-precision preconditioner); info:
-// first exactly-zero pivot (1-based) or 0.  Solve with mg_getrs_dinv and ipiv NULL.
-int mg_getrf_nopiv(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* info, void* stream) {
-  if (!h || !A || !info) return MG_ERR_ARG;
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  switch (prec) {
-    case MG_C128: return mg::getrf_nopiv<double>(h->ws, A, n, lda, info, s, nullptr);
-    case MG_C64: return mg::getrf_nopiv<float>(h->ws, A, n, lda, info, s, nullptr);
-    case MG_BF16X3: return mg::getrf_nopiv<float>(h->ws, A, n, lda, info, s, &mg::cgemm_bf16x3);
-    default: return MG_ERR_ARG;
-  }
-}
-
 // Solve A X = B in place of B (n x nrhs, row-major) from getrf's factors.
 int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb,
              int64_t nrhs, void* stream) {
@@ -176,6 +163,12 @@ int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda
     case MG_BF16X3: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3);
     default: return MG_ERR_ARG;
   }
+}
+
+// Cap on the workgroups of one LU leaf launch for this handle (g <= 0: all CUs).
+int mg_set_leaf_blocks(mg_handle* h, int g) {
+  if (!h) return MG_ERR_ARG;
+  return mg::workspace_leaf_blocks(h->ws, g);
 }
 
 // Waits for the stream and reports a failed grid barrier of earlier factorisations.

@@ -49,6 +49,10 @@ int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flop
 
 int workspace_init(Workspace& ws);
 void workspace_free(Workspace& ws);
+// Cap on the workgroups of one LU leaf launch (g <= 0: the CU count).  Solves
+// that run concurrently on one device keep the sum of their caps within the
+// CU count, so that every leaf can become resident.
+int workspace_leaf_blocks(Workspace& ws, int g);
 // Synchronises the stream and reports a failed grid barrier.
 int workspace_check(Workspace& ws, hipStream_t s);
 
@@ -65,9 +69,6 @@ int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, 
 template <typename T>
 int getrs(const void* LU, int64_t n, int64_t lda, const int* ipiv, void* B, int64_t ldb, int64_t nrhs,
           hipStream_t s);
-// LU without pivoting (mixed-precision preconditioner); info: first zero pivot.
-template <typename T>
-int getrf_nopiv(Workspace& ws, void* A, int64_t n, int64_t lda, int* info, hipStream_t s, GemmFn update);
 template <typename T>
 int trsm_lower_unit_b(const void* L, int64_t ldl, int64_t sL, int64_t nb, void* B, int64_t ldb, int64_t sB,
                       int64_t ncols, int64_t batch, hipStream_t s);

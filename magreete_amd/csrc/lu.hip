@@ -461,9 +461,10 @@ template <typename T>
 static int panel(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int64_t w, int* ipiv, int* info,
                  hipStream_t s) {
   typedef typename C2<T>::t T2;
-  // leaf width: kLeaf while kMaxGetf2Blocks workgroups can hold the rows in LDS, else half
-  const int64_t full_rows = (int64_t)kMaxGetf2Blocks * (kGetf2Lds / (kLeaf * (int64_t)sizeof(T2)));
-  const int64_t lw = m <= full_rows ? kLeaf : kLeaf / 2;
+  // leaf width: the widest power of two up to kLeaf whose rows fit the
+  // LDS of the workgroups a leaf may use
+  int64_t lw = kLeaf;
+  while (lw > 1 && (int64_t)ws.coop_blocks * (kGetf2Lds / (lw * (int64_t)sizeof(T2))) < m) lw /= 2;
   if (w <= lw) return getf2<T>(ws, A, lda, k, m, w, ipiv, info, s);
   T2* Ap = reinterpret_cast<T2*>(A);
   const int64_t w1 = split_half(w);
@@ -613,6 +614,12 @@ void workspace_free(Workspace& ws) {
   if (ws.evA) (void)hipEventDestroy(ws.evA);
   if (ws.evB) (void)hipEventDestroy(ws.evB);
   ws = Workspace();
+}
+
+int workspace_leaf_blocks(Workspace& ws, int g) {
+  const int top = (ws.cus < kMaxGetf2Blocks) ? ws.cus : kMaxGetf2Blocks;
+  ws.coop_blocks = (g <= 0 || g > top) ? top : g;
+  return MG_OK;
 }
 
 int workspace_check(Workspace& ws, hipStream_t s) {

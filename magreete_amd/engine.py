@@ -6,6 +6,7 @@ complex128 (or complex64 for the factor precisions), with the reference's
 interleaved layout: row d*m + a, column d*j + b.
 """
 import ctypes
+import threading
 
 import torch
 
@@ -36,18 +37,73 @@ def _stream(device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+_tls = threading.local()
+
+
 def handle(device):
-    """The library handle (LU workspace) of a device, created on first use."""
+    """The library handle (LU workspace, look-ahead stream) in effect for this
+    thread: the one installed by a Workspace context, else the device's
+    shared handle, created on first use."""
     idx = device.index if device.index is not None else torch.cuda.current_device()
+    own = getattr(_tls, "handles", None)
+    if own and idx in own:
+        return own[idx]
     h = _handles.get(idx)
     if h is None:
-        lib = L.load()
-        with torch.cuda.device(idx):
-            hp = ctypes.c_void_p()
-            L.check(lib.mg_create(ctypes.byref(hp)), "mg_create")
-        h = hp
+        h = _create(idx)
         _handles[idx] = h
     return h
+
+
+def _create(idx):
+    lib = L.load()
+    with torch.cuda.device(idx):
+        hp = ctypes.c_void_p()
+        L.check(lib.mg_create(ctypes.byref(hp)), "mg_create")
+    return hp
+
+
+class Workspace:
+    """A private library handle for one device, for concurrent solves.
+
+    Factorisations sequence their look-ahead through the handle's second
+    stream and events, so two solves that run at the same time (different
+    torch streams, usually different host threads) need different handles.
+    Used as a context manager it becomes this thread's handle for `device`."""
+
+    def __init__(self, device, leaf_blocks=0):
+        self.idx = torch.device(device).index
+        if self.idx is None:
+            self.idx = torch.cuda.current_device()
+        self.h = _create(self.idx)
+        self._prev = None
+        if leaf_blocks:
+            # workgroups of one LU leaf launch: concurrent factorisations keep
+            # their sum within the CU count (the leaf waits on a grid barrier)
+            L.check(L.load().mg_set_leaf_blocks(self.h, int(leaf_blocks)), "mg_set_leaf_blocks")
+
+    def __enter__(self):
+        own = getattr(_tls, "handles", None)
+        if own is None:
+            own = _tls.handles = {}
+        self._prev = own.get(self.idx)
+        own[self.idx] = self.h
+        return self
+
+    def __exit__(self, *exc):
+        own = _tls.handles
+        if self._prev is None:
+            own.pop(self.idx, None)
+        else:
+            own[self.idx] = self._prev
+        return False
+
+    def close(self):
+        """Destroy the handle (waits for nothing: synchronise its streams first)."""
+        if self.h is not None:
+            with torch.cuda.device(self.idx):
+                L.check(L.load().mg_destroy(self.h), "mg_destroy")
+            self.h = None
 
 
 def _pts(x, device):

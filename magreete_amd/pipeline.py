@@ -110,3 +110,36 @@ class Solver:
         for name, a, b in (("assemble", 0, 1), ("factor_solve", 1, 2), ("propagate", 2, 3)):
             phases[name] = phases.get(name, 0.0) + ev[a].elapsed_time(ev[b])
         return out
+
+
+def incident(kind, points, w):
+    """Incident field of workload w at points (device tensor), as (d M, T)
+    complex128: plane waves or beams along w's directions, with the drop-in
+    classes' conventions (2D: dirs (cos(-theta), sin(-theta)), TE polarisation
+    (-sin theta, cos theta); 3D vector: p rebuilt through atan2/arccos)."""
+    beam = w.source == "beam"
+    if kind in ("2d_tm", "2d_te"):
+        th = torch.as_tensor(w.thetas, dtype=torch.float64).reshape(-1)
+        dirs = torch.stack([torch.cos(-th), torch.sin(-th)], dim=1)
+        pol = None if kind == "2d_tm" else torch.stack([dirs[:, 1], dirs[:, 0]], dim=1).float().double()
+        out = E.sources(2, beam, points, dirs, pol, w.k0, w.beam_waist)
+    elif kind == "3d_vector":
+        from .Transmission3D import _pvec
+        out = E.sources(3, beam, points, w.u, _pvec(w.p), w.k0, w.beam_waist)
+    elif kind == "3d_scalar":
+        out = E.sources(3, beam, points, w.u, None, w.k0, w.beam_waist)
+    else:
+        raise NotImplementedError(kind)
+    return out.reshape(-1, out.shape[-1]).contiguous()
+
+
+def problem(w, device, measure=True):
+    """The Problem of a workload on a device (positions and incident fields
+    moved there; measurement points when the workload has them and measure)."""
+    pts = w.points.to(device=device, dtype=torch.float64).contiguous()
+    E0s = incident(w.kind, pts, w)
+    meas = E0m = None
+    if measure and w.measure is not None:
+        meas = w.measure.to(device=device, dtype=torch.float64).contiguous()
+        E0m = incident(w.kind, meas, w)
+    return Problem(w.kind, pts, w.k0, w.alpha, E0s, meas, E0m)

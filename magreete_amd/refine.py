@@ -107,31 +107,73 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
         rel = (beta / bnorm).max().item()
 
 
+class MixedSolver:
+    """A resident mixed-precision factorisation of M with FP64 GMRES-IR solves.
+
+    M (complex128) stays intact and must outlive the solver; M32 holds M in
+    complex64 on entry and is overwritten by the factors.  The tier starts
+    at the unpivoted block LDU ('nopiv', or 'pivoted' with pivot=True); a
+    solve whose refinement breaks down or stalls moves the solver to the
+    next tier (complex64 LU with partial pivoting, then complex128 LU) for
+    this and every later solve."""
+
+    TIERS = ("nopiv", "pivoted", "fp64")
+
+    def __init__(self, M, M32, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False):
+        self.M, self.M32, self.prec = M, M32, prec
+        self.tol, self.max_iters, self.restart = tol, max_iters, restart
+        self.tier = "pivoted" if pivot else "nopiv"
+        self.fell = False
+        self.F = None
+        self._factor(fresh=True)
+
+    def _factor(self, fresh=False):
+        if self.tier == "fp64":
+            self.F = E.Factor(self.M.clone(), "c128")
+            return
+        if not fresh:
+            E.convert(self.M, self.M32)                    # the previous tier overwrote M32
+        self.F = E.Factor(self.M32, self.prec, pivot=(self.tier == "pivoted"))
+        if self.tier == "nopiv" and int(self.F.info.item()) != 0:
+            self._escalate()                               # exact zero pivot: pivot this time
+
+    def _escalate(self):
+        self.tier = self.TIERS[self.TIERS.index(self.tier) + 1]
+        self.fell = True
+        self._factor()
+
+    @property
+    def info(self):
+        return self.F.info
+
+    def solve(self, B, X):
+        """X <- M^{-1} B to a relative FP64 residual <= tol per column.
+        Returns stats: tier, fallback, iters (FP64 products), cycles,
+        residual."""
+        bnorm = torch.linalg.norm(B, dim=0)
+        bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
+        total = 0
+        while self.tier != "fp64":
+            ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart)
+            total += iters
+            if ok:
+                return {"tier": self.tier, "fallback": self.fell, "iters": total, "cycles": cycles,
+                        "residual": rel}
+            self._escalate()
+        # refinement stalled in low precision: complex128 LU
+        X.copy_(B)
+        self.F.solve(X)
+        R = B.clone()
+        E.cgemm(self.M, X, R, alpha=-1.0, beta=1.0)
+        rel = (torch.linalg.norm(R, dim=0) / bnorm).max().item()
+        return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel}
+
+
 def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False):
-    """Solve M X = B to a relative FP64 residual <= tol.  M (complex128) is
-    left intact; M32 holds M in complex64 on entry (overwritten by factors).
-    Returns stats: tier ('nopiv', 'pivoted' or 'fp64'), iters (FP64
-    products), cycles (GMRES restarts + 1), residual, fallback, info."""
-    bnorm = torch.linalg.norm(B, dim=0)
-    bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
-    tiers = [True] if pivot else [False, True]
-    total = 0
-    for idx, piv in enumerate(tiers):
-        if idx > 0:
-            E.convert(M, M32)
-        F = E.Factor(M32, prec, pivot=piv)
-        if not piv and int(F.info.item()) != 0:
-            continue                                       # exact zero pivot: pivot this time
-        ok, iters, cycles, rel = _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart)
-        total += iters
-        if ok:
-            return {"tier": "pivoted" if piv else "nopiv", "fallback": idx > 0, "iters": total,
-                    "cycles": cycles, "residual": rel, "info": F.info}
-    # refinement stalled in low precision: complex128 LU
-    F64 = E.Factor(M.clone(), "c128")
-    X.copy_(B)
-    F64.solve(X)
-    R = B.clone()
-    E.cgemm(M, X, R, alpha=-1.0, beta=1.0)
-    rel = (torch.linalg.norm(R, dim=0) / bnorm).max().item()
-    return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel, "info": F64.info}
+    """Solve M X = B to a relative FP64 residual <= tol (one MixedSolver,
+    one solve).  Returns stats: tier ('nopiv', 'pivoted' or 'fp64'), iters
+    (FP64 products), cycles (GMRES restarts + 1), residual, fallback, info."""
+    ms = MixedSolver(M, M32, prec, tol=tol, max_iters=max_iters, restart=restart, pivot=pivot)
+    st = ms.solve(B, X)
+    st["info"] = ms.info
+    return st
