@@ -41,6 +41,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="configurations of the batched-ensemble measurement (0: skip)")
+    ap.add_argument("--serving", type=int, default=40, help="requests per serving load level (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -149,6 +152,8 @@ def main():
         roof = {"bound": "mfma", "achieved": src["update_tflops"], "peak": PEAK["fp64"], "unit": "TFLOP/s",
                 "frac": src["update_tflops"] / PEAK["fp64"], "traffic": None,
                 "kernel": "k_cgemm_trail<double> (LU trailing update, fp64 path)"}
+    ens = run_ensemble(args, dev) if args.ensemble > 0 else None
+    srv = run_serving(args, dev) if args.serving > 0 else None
     line = {
         "metric": METRIC, "value": rb["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": rb["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -159,8 +164,62 @@ def main():
             args.n_points, n, w.T, args.n_dirs, args.n_measure), "method": best, "k0": w.k0,
             "seed": args.seed, "tol": args.tol},
         "roofline": roof, "cpu_baseline": cpu, "methods": results, "errors": errors,
+        "ensemble": ens, "serving": srv,
     }
     print(json.dumps(line))
+
+
+def run_ensemble(args, dev):
+    """Batched ensemble: args.ensemble 3D vector configurations (N log-uniform
+    in [500, 3000], 16 directions x 2 polarisations), solved one at a time by
+    the FP64 path and concurrently (4 streams) by the FP64 and the BF16x3
+    mixed paths; errors of the batched results against the loop."""
+    from magreete_amd import workloads as W
+    from magreete_amd.ensemble import Ensemble
+    ens = W.cfg_ensemble(n_configs=args.ensemble, seed=args.seed)
+    out = {"workload": "%d x 3d_vector, N log-uniform [500, 3000], T=32" % len(ens), "unit": "configs/s"}
+    ref = None
+    for name, method, streams in (("fp64_loop", "fp64", 1), ("fp64_batched", "fp64", 4),
+                                  ("bf16x3_batched", "bf16x3", 4)):
+        e = Ensemble(dev, streams=streams, method=method, tol=args.tol)
+        e.run(ens[:4])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = e.run(ens)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = len(ens) / dt
+        if ref is None:
+            ref = [r["X"] for r in res]
+        else:
+            out[name + "_max_err_vs_loop"] = max(colerr(r["X"], x) for r, x in zip(res, ref))
+        print("# ensemble %s: %.1f configs/s" % (name, out[name]), file=sys.stderr, flush=True)
+        del res
+        torch.cuda.empty_cache()
+    return out
+
+
+def run_serving(args, dev):
+    """Serving: capacity from a burst of args.serving requests, then Poisson
+    arrivals at 0.5x, 0.8x and 1.1x of it (mix 50% 2D TM / 25% 2D TE / 25% 3D
+    vector, N log-uniform in [256, 16384], T uniform in [1, 256], 30% reuse,
+    HBM budget 200 GB, 4 workers, BF16x3 mixed path)."""
+    import math
+    from magreete_amd import serving as S
+    srv = S.Server(dev, workers=4, method="bf16x3", hbm_budget=200e9, tol=args.tol)
+    srv.run(S.make_trace(4, math.inf, seed=args.seed + 99, n_range=(256, 2048)))
+    burst = S.make_trace(args.serving, math.inf, seed=args.seed + 7)
+    t0 = time.perf_counter()
+    srv.run(burst)
+    cap = args.serving / (time.perf_counter() - t0)
+    out = {"requests_per_level": args.serving, "capacity_req_per_s": cap, "workers": 4, "hbm_budget_gb": 200}
+    for f in (0.5, 0.8, 1.1):
+        res = srv.run(S.make_trace(args.serving, cap * f, seed=args.seed + 8))
+        lvl = S.percentiles(res)
+        lvl.update({"offered_req_per_s": cap * f, "hits": sum(r["hit"] for r in res)})
+        out["load_%.1fx" % f] = lvl
+        print("# serving %.1fx: %s" % (f, lvl), file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

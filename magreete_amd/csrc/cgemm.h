@@ -70,11 +70,12 @@ struct GreenA {
   double radius;
 };
 
-// LDS planes for one stage: [re/im][BK][BM + PAD]
-template <typename T, int BM, int BN, int BK>
+// LDS planes for one stage: [re, im (, re + im)][BK][BM + PAD].  The third
+// plane serves the three-product (Gauss) complex multiply.
+template <typename T, int BM, int BN, int BK, int NP = 2>
 struct GemmSmem {
-  T a[2][BK][BM + 1];
-  T b[2][BK][BN + 1];
+  T a[NP][BK][BM + 1];
+  T b[NP][BK][BN + 1];
 };
 
 template <typename T, int BM, int BK, int NT>
@@ -100,8 +101,8 @@ __device__ inline void a_fetch(const MemA& src, int64_t M, int64_t K, int64_t i0
   }
 }
 
-template <typename T, int BM, int BN, int BK, int NT>
-__device__ inline void a_store(GemmSmem<T, BM, BN, BK>& s, const ARegs<T, BM, BK, NT>& r) {
+template <typename T, int BM, int BN, int BK, int NT, int NP>
+__device__ inline void a_store(GemmSmem<T, BM, BN, BK, NP>& s, const ARegs<T, BM, BK, NT>& r) {
 #pragma unroll
   for (int p = 0; p < r.PER; ++p) {
     const int e = p * NT + threadIdx.x;
@@ -109,6 +110,7 @@ __device__ inline void a_store(GemmSmem<T, BM, BN, BK>& s, const ARegs<T, BM, BK
       const int row = e / BK, kk = e % BK;
       s.a[0][kk][row] = r.v[p].x;
       s.a[1][kk][row] = r.v[p].y;
+      if constexpr (NP == 3) s.a[2][kk][row] = r.v[p].x + r.v[p].y;
     }
   }
 }
@@ -116,8 +118,8 @@ __device__ inline void a_store(GemmSmem<T, BM, BN, BK>& s, const ARegs<T, BM, BK
 // ---- A-tile staging: Green's function evaluated in FP64 ------------------
 // Tile rows i0..i0+BM are d*point + a, tile columns k0..k0+BK are
 // d*scatterer + b; BM and BK are multiples of d.
-template <int KIND, typename T, int BM, int BN, int BK, int NT>
-__device__ inline void a_green(GemmSmem<T, BM, BN, BK>& s, const GreenA<KIND>& src, int64_t i0, int64_t k0) {
+template <int KIND, typename T, int BM, int BN, int BK, int NT, int NP>
+__device__ inline void a_green(GemmSmem<T, BM, BN, BK, NP>& s, const GreenA<KIND>& src, int64_t i0, int64_t k0) {
   constexpr int D = (KIND == MG_2D_TE) ? 2 : (KIND == MG_3D_VECTOR ? 3 : 1);
   constexpr int SP = (KIND == MG_2D_TM || KIND == MG_2D_TE) ? 2 : 3;
   constexpr int PM = BM / D, PK = BK / D;
@@ -139,6 +141,7 @@ __device__ inline void a_green(GemmSmem<T, BM, BN, BK>& s, const GreenA<KIND>& s
       for (int b = 0; b < D; ++b) {
         s.a[0][pk * D + b][pm * D + a] = (T)g[a * D + b].re;
         s.a[1][pk * D + b][pm * D + a] = (T)g[a * D + b].im;
+        if constexpr (NP == 3) s.a[2][pk * D + b][pm * D + a] = (T)(g[a * D + b].re + g[a * D + b].im);
       }
     }
   }
@@ -167,8 +170,8 @@ __device__ inline void b_fetch(const void* Bp, int64_t ldb, int64_t K, int64_t N
   }
 }
 
-template <typename T, int BM, int BN, int BK, int NT>
-__device__ inline void b_store(GemmSmem<T, BM, BN, BK>& s, const BRegs<T, BN, BK, NT>& r) {
+template <typename T, int BM, int BN, int BK, int NT, int NP>
+__device__ inline void b_store(GemmSmem<T, BM, BN, BK, NP>& s, const BRegs<T, BN, BK, NT>& r) {
 #pragma unroll
   for (int p = 0; p < r.PER; ++p) {
     const int e = p * NT + threadIdx.x;
@@ -176,14 +179,17 @@ __device__ inline void b_store(GemmSmem<T, BM, BN, BK>& s, const BRegs<T, BN, BK
       const int kk = e / BN, col = e % BN;
       s.b[0][kk][col] = r.v[p].x;
       s.b[1][kk][col] = r.v[p].y;
+      if constexpr (NP == 3) s.b[2][kk][col] = r.v[p].x + r.v[p].y;
     }
   }
 }
 
 // ---- the kernel body -------------------------------------------------------
 // WM x WN waves of 64 lanes; each wave owns (BM/WM) x (BN/WN) of C as
-// 16x16 MFMA tiles.
-template <typename T, int BM, int BN, int BK, int WM, int WN, typename ASrc>
+// 16x16 MFMA tiles.  G3: complex products as three real MFMA products
+// (Gauss: t1 = Ar Br, t2 = Ai Bi, t3 = (Ar + Ai)(Br + Bi); Cr = t1 - t2,
+// Ci = t3 - t1 - t2) instead of four, at the cost of a third accumulator.
+template <typename T, int BM, int BN, int BK, int WM, int WN, typename ASrc, bool G3 = false>
 __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx<T>::t alpha, const ASrc& asrc,
                                   const void* Bp, int64_t ldb, typename Cplx<T>::t beta, void* Cp, int64_t ldc,
                                   int tm, int tn) {
@@ -191,21 +197,26 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
   typedef typename Cplx<T>::acc ACC;
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+  constexpr int NP = G3 ? 3 : 2;
+  typedef GemmSmem<T, BM, BN, BK, NP> Smem;
   static_assert(BM % (WM * 16) == 0 && BN % (WN * 16) == 0 && BK % 4 == 0, "tile shape");
-  __shared__ GemmSmem<T, BM, BN, BK> sm[2];
+  __shared__ Smem sm[2];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
   const int li = lane & 15, lk = lane >> 4;
 
-  ACC cr[TM][TN], ci[TM][TN];
+  // 4M: acc[0] = Cr, acc[1] = Ci.  3M: acc[0..2] = t1, t2, t3.
+  ACC acc[NP == 3 ? 3 : 2][TM][TN];
 #pragma unroll
-  for (int a = 0; a < TM; ++a)
+  for (int q = 0; q < (NP == 3 ? 3 : 2); ++q)
 #pragma unroll
-    for (int b = 0; b < TN; ++b)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0; ci[a][b][r] = 0; }
+      for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[q][a][b][r] = 0;
 
   const int64_t KT = (K + BK - 1) / BK;
   constexpr bool kMem = std::is_same<ASrc, MemA>::value;
@@ -215,12 +226,12 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
   // prologue
   if constexpr (kMem) {
     a_fetch<T, BM, BK, NT>(asrc, M, K, i0, 0, ar);
-    a_store<T, BM, BN, BK, NT>(sm[0], ar);
+    a_store<T, BM, BN, BK, NT, NP>(sm[0], ar);
   } else {
-    a_green<ASrc::kind, T, BM, BN, BK, NT>(sm[0], asrc, i0, 0);
+    a_green<ASrc::kind, T, BM, BN, BK, NT, NP>(sm[0], asrc, i0, 0);
   }
   b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, 0, j0, br);
-  b_store<T, BM, BN, BK, NT>(sm[0], br);
+  b_store<T, BM, BN, BK, NT, NP>(sm[0], br);
   __syncthreads();
 
   for (int64_t kt = 0; kt < KT; ++kt) {
@@ -230,36 +241,39 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
       if constexpr (kMem) a_fetch<T, BM, BK, NT>(asrc, M, K, i0, (kt + 1) * BK, ar);
       b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, (kt + 1) * BK, j0, br);
     }
-    GemmSmem<T, BM, BN, BK>& s = sm[cur];
+    Smem& s = sm[cur];
 #pragma unroll
     for (int g = 0; g < BK / 4; ++g) {
-      T are[TM], aim[TM], bre[TN], bim[TN];
+      T av[NP][TM], bv[NP][TN];
 #pragma unroll
-      for (int a = 0; a < TM; ++a) {
-        are[a] = s.a[0][g * 4 + lk][wm * (BM / WM) + a * 16 + li];
-        aim[a] = s.a[1][g * 4 + lk][wm * (BM / WM) + a * 16 + li];
-      }
+      for (int q = 0; q < NP; ++q) {
 #pragma unroll
-      for (int b = 0; b < TN; ++b) {
-        bre[b] = s.b[0][g * 4 + lk][wn * (BN / WN) + b * 16 + li];
-        bim[b] = s.b[1][g * 4 + lk][wn * (BN / WN) + b * 16 + li];
+        for (int a = 0; a < TM; ++a) av[q][a] = s.a[q][g * 4 + lk][wm * (BM / WM) + a * 16 + li];
+#pragma unroll
+        for (int b = 0; b < TN; ++b) bv[q][b] = s.b[q][g * 4 + lk][wn * (BN / WN) + b * 16 + li];
       }
 #pragma unroll
       for (int a = 0; a < TM; ++a) {
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
-          cr[a][b] = mg_mfma(are[a], bre[b], cr[a][b]);
-          cr[a][b] = mg_mfma_nega(aim[a], bim[b], cr[a][b]);
-          ci[a][b] = mg_mfma(are[a], bim[b], ci[a][b]);
-          ci[a][b] = mg_mfma(aim[a], bre[b], ci[a][b]);
+          if constexpr (NP == 3) {
+            acc[0][a][b] = mg_mfma(av[0][a], bv[0][b], acc[0][a][b]);
+            acc[1][a][b] = mg_mfma(av[1][a], bv[1][b], acc[1][a][b]);
+            acc[2][a][b] = mg_mfma(av[2][a], bv[2][b], acc[2][a][b]);
+          } else {
+            acc[0][a][b] = mg_mfma(av[0][a], bv[0][b], acc[0][a][b]);
+            acc[0][a][b] = mg_mfma_nega(av[1][a], bv[1][b], acc[0][a][b]);
+            acc[1][a][b] = mg_mfma(av[0][a], bv[1][b], acc[1][a][b]);
+            acc[1][a][b] = mg_mfma(av[1][a], bv[0][b], acc[1][a][b]);
+          }
         }
       }
     }
     if (more) {
-      GemmSmem<T, BM, BN, BK>& nx = sm[cur ^ 1];
-      if constexpr (kMem) a_store<T, BM, BN, BK, NT>(nx, ar);
-      else a_green<ASrc::kind, T, BM, BN, BK, NT>(nx, asrc, i0, (kt + 1) * BK);
-      b_store<T, BM, BN, BK, NT>(nx, br);
+      Smem& nx = sm[cur ^ 1];
+      if constexpr (kMem) a_store<T, BM, BN, BK, NT, NP>(nx, ar);
+      else a_green<ASrc::kind, T, BM, BN, BK, NT, NP>(nx, asrc, i0, (kt + 1) * BK);
+      b_store<T, BM, BN, BK, NT, NP>(nx, br);
     }
     __syncthreads();
   }
@@ -279,7 +293,15 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
       for (int r = 0; r < 4; ++r) {
         const int64_t row = i0 + wm * (BM / WM) + a * 16 + (kRowsStrided ? lk + 4 * r : lk * 4 + r);
         if (row < M && col < N) {
-          const T xr = cr[a][b][r], xi = ci[a][b][r];
+          T xr, xi;
+          if constexpr (NP == 3) {
+            const T t1 = acc[0][a][b][r], t2 = acc[1][a][b][r], t3 = acc[2][a][b][r];
+            xr = t1 - t2;
+            xi = t3 - t1 - t2;
+          } else {
+            xr = acc[0][a][b][r];
+            xi = acc[1][a][b][r];
+          }
           T2 out;
           out.x = alpha.x * xr - alpha.y * xi;
           out.y = alpha.x * xi + alpha.y * xr;

@@ -6,7 +6,7 @@ namespace {
 
 // blockIdx.x walks the tile grid, blockIdx.y the batch (element strides
 // sA, sB, sC between consecutive problems).
-template <typename T, int BM, int BN, int BK, int WM, int WN>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool G3>
 __global__ __launch_bounds__(64 * WM * WN) void k_cgemm(int64_t M, int64_t N, int64_t K,
                                                         typename Cplx<T>::t alpha, MemA a, int64_t sA,
                                                         const void* B, int64_t ldb, int64_t sB,
@@ -17,20 +17,20 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm(int64_t M, int64_t N, in
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
   const int64_t b = blockIdx.y;
   MemA ab{reinterpret_cast<const T2*>(a.A) + b * sA, a.lda};
-  cgemm_body<T, BM, BN, BK, WM, WN, MemA>(M, N, K, alpha, ab, reinterpret_cast<const T2*>(B) + b * sB, ldb, beta,
-                                          reinterpret_cast<T2*>(C) + b * sC, ldc, tm, tn);
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA, G3>(M, N, K, alpha, ab, reinterpret_cast<const T2*>(B) + b * sB, ldb,
+                                              beta, reinterpret_cast<T2*>(C) + b * sC, ldc, tm, tn);
 }
 
 // The same GEMM under its own symbol for the LU trailing update, so profiles
 // separate it from the small GEMMs inside panels and triangular solves.
-template <typename T, int BM, int BN, int BK, int WM, int WN>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool G3>
 __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_trail(int64_t M, int64_t N, int64_t K,
                                                               typename Cplx<T>::t alpha, MemA a, const void* B,
                                                               int64_t ldb, typename Cplx<T>::t beta, void* C,
                                                               int64_t ldc, int nbm, int nbn) {
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
-  cgemm_body<T, BM, BN, BK, WM, WN, MemA>(M, N, K, alpha, a, B, ldb, beta, C, ldc, tm, tn);
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA, G3>(M, N, K, alpha, a, B, ldb, beta, C, ldc, tm, tn);
 }
 
 template <typename T>
@@ -43,22 +43,25 @@ typename Cplx<T>::t to_t2(mg::cd z) {
 
 using mg::cd;
 
-// Tile configurations.  FP64: 128 x 128 tiles (8 waves of 64 x 32) when the
-// grid still gives every CU two tiles, which halves the operand re-reads
-// from L2 against 64 x 64; otherwise 64 x 64 with k-steps of 16.  FP32:
-// 64 x 64 (measured fastest on gfx950 for both the LU update shape and the
-// tall-skinny residual shape).
-template <typename T, bool TRAIL, int BM, int BN, int BK, int WM, int WN>
+// Tile configurations, measured on gfx950 for the LU trailing-update shape
+// (r x r x 256) and the tall-skinny residual shape (n x T x n):
+//  * 128 x 64 tiles (8 waves of 32 x 32) with three-product (Gauss) complex
+//    multiplies when the grid fills the CUs: 25% fewer MFMAs, and half the
+//    operand re-reads of 64 x 64 along the rows;
+//  * otherwise 64 x 64 (4 waves of 32 x 32): FP64 with four products (the
+//    third accumulator costs more occupancy than the saved MFMAs return at
+//    this size), FP32 with three.
+template <typename T, bool TRAIL, int BM, int BN, int BK, int WM, int WN, bool G3>
 int launch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
            int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
   const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
   MemA a{A, lda};
   if (TRAIL)
-    hipLaunchKernelGGL((k_cgemm_trail<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N, K,
-                       to_t2<T>(alpha), a, B, ldb, to_t2<T>(beta), C, ldc, nbm, nbn);
+    hipLaunchKernelGGL((k_cgemm_trail<T, BM, BN, BK, WM, WN, G3>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N,
+                       K, to_t2<T>(alpha), a, B, ldb, to_t2<T>(beta), C, ldc, nbm, nbn);
   else
-    hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, WM, WN>), dim3(nbm * nbn, (unsigned)batch), dim3(64 * WM * WN), 0, s,
-                       M, N, K, to_t2<T>(alpha), a, sA, B, ldb, sB, to_t2<T>(beta), C, ldc, sC, nbm, nbn);
+    hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, WM, WN, G3>), dim3(nbm * nbn, (unsigned)batch), dim3(64 * WM * WN),
+                       0, s, M, N, K, to_t2<T>(alpha), a, sA, B, ldb, sB, to_t2<T>(beta), C, ldc, sC, nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
@@ -66,15 +69,13 @@ int launch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda
 template <typename T, bool TRAIL>
 int dispatch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
              int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
-  if constexpr (std::is_same<T, double>::value) {
-    const int64_t big = ((M + 127) / 128) * ((N + 127) / 128) * batch;
-    if (big >= 2 * (int64_t)mg::cu_count())
-      return launch<T, TRAIL, 128, 128, 8, 2, 4>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+  constexpr bool F64 = std::is_same<T, double>::value;
+  const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64) * batch;
+  if (tiles >= (F64 ? 1 : 2) * (int64_t)mg::cu_count())
+    return launch<T, TRAIL, 128, 64, 16, 4, 2, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
+                                                     s);
+  return launch<T, TRAIL, 64, 64, 16, 2, 2, !F64>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
                                                   s);
-    return launch<T, TRAIL, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
-  } else {
-    return launch<T, TRAIL, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
-  }
 }
 
 }  // namespace
