@@ -5,7 +5,9 @@
 // (upper) triangular and D = blockdiag(S_k), S_k the k-th diagonal block of
 // the successive Schur complements.  Step k of the factorisation:
 //   S_k^{-1}            one workgroup, in-place Gauss-Jordan with 32-column
-//                       panels (BF16x3 MFMA rank-32 updates, FP32 pivots);
+//                       panels (BF16x3 MFMA rank-32 updates, FP32 pivots),
+//                       plus one FP32 Newton-Schulz step when the other
+//                       products are FP32;
 //   Uh_k = S_k^{-1} A12, Lh_k = A21 S_k^{-1}     stored over A12 and A21;
 //   A22 -= A21 S_k^{-1} A12                      the trailing update.
 // A solve is then GEMMs only: one per block column with Lh (forward), one
@@ -189,12 +191,20 @@ __global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __r
 
 __global__ void k_zero_info(int* p) { *p = 0; }
 
+// R[i][j] = 2 delta_ij for i, j < h (row stride kIB)
+__global__ void k_two_eye(float2* R, int h) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= h * kIB) return;
+  const int i = e / kIB, j = e % kIB;
+  R[e] = make_float2(i == j ? 2.f : 0.f, 0.f);
+}
+
 }  // namespace
 
 namespace mg {
 
 int64_t ldu_dinv_elems(int64_t n) { return ((n + kIB - 1) / kIB) * (int64_t)kIB * kIB; }
-int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n; }
+int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)kIB * kIB; }
 
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
               GemmFn gemm) {
@@ -212,12 +222,26 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   auto s12 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n; };
   auto s21 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n + nb * n; };
   // step k's inverse and panels, on stream st
+  // With FP32 products the BF16x3 block inverse would limit the factor's
+  // accuracy: one Newton-Schulz step X <- X (2I - S X) in FP32 squares its
+  // error.  X0 and R live after the panel copies in the work buffer (panels
+  // run one at a time).
+  const bool newton = (gemm == nullptr);
+  float2* X0 = W + 4 * nb * n;
+  float2* R = X0 + nb * nb;
   auto panel = [&](int64_t k, hipStream_t st) -> int {
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
     float2* Dk = D + k * nb * nb;
-    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(kInvThreads), 0, st, Ap + r0 * lda + r0, lda, (int)h, Dk, info,
+    const float2* S = Ap + r0 * lda + r0;
+    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(kInvThreads), 0, st, S, lda, (int)h, newton ? X0 : Dk, info,
                        r0);
     MG_HIP(hipGetLastError());
+    if (newton) {
+      hipLaunchKernelGGL(k_two_eye, dim3((unsigned)((h * nb + 255) / 256)), dim3(256), 0, st, R, (int)h);
+      MG_HIP(hipGetLastError());
+      MG_TRY(cgemm<float>(h, h, h, cd{-1, 0}, S, lda, X0, nb, cd{1, 0}, R, nb, st));    // R = 2I - S X0
+      MG_TRY(cgemm<float>(h, h, h, cd{1, 0}, X0, nb, R, nb, cd{0, 0}, Dk, nb, st));     // D_k = X0 R
+    }
     if (r == 0) return MG_OK;
     float2 *S12 = s12(k), *S21 = s21(k);
     MG_HIP(hipMemcpy2DAsync(S12, r * sizeof(float2), Ap + r0 * lda + r1, lda * sizeof(float2), r * sizeof(float2),
