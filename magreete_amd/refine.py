@@ -10,8 +10,9 @@ matrix products are one (n x n) @ (n x T) FP64 GEMM and one complex64
 multi-RHS solve (GEMM-only, through stored diagonal-block inverses) per
 iteration; the per-column Gram-Schmidt runs as HIP multi-vector dot/axpy
 passes and the Givens updates are small (T x j) torch operations.
-Convergence is declared on the true FP64 residual ||b - A x|| / ||b|| <= tol
-per column, computed at the end of every cycle.
+Convergence is declared on the FP64 residual ||b - A x|| / ||b|| <= tol per
+column, formed at the end of every cycle from the Arnoldi relation (equal to
+the product b - A x up to the rounding of the FP64 products already taken).
 
 The first factorisation is unpivoted (no pivot search, no grid barrier, no
 row swap); the refinement restores the accuracy.  If it breaks down (an exact
@@ -28,12 +29,16 @@ from . import engine as E
 def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
     """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F.
     The first Krylov direction is P^{-1} b, so the first cycle starts with no
-    FP64 product; every cycle ends with the true residual b - M x in FP64.
+    FP64 product.  Every cycle ends with the residual b - M x formed from the
+    Arnoldi relation M Z_k = V_{k+1} H_k, i.e. r = V_{k+1} (beta e1 - H_k y)
+    with the unrotated Hessenberg matrix: it equals the FP64 product b - M x
+    up to the rounding of the FP64 products M z_j already taken, at the cost
+    of one pass over the basis instead of another product.
     Returns (converged, FP64 products, cycles, residual)."""
     dev = M.device
     n, T = B.shape
     lowp = torch.empty((n, T), dtype=torch.complex64, device=dev)
-    m_max = max(1, min(restart, max_iters - 1))
+    m_max = max(1, min(restart, max_iters))
     work = torch.empty(L.load().mg_mdot_work_elems(m_max + 1, T), dtype=torch.complex128, device=dev)
 
     def precond(v, out):
@@ -55,10 +60,11 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
             return False, iters, cycles, rel
         prev = rel
         cycles += 1
-        m = max(1, min(restart, max_iters - iters - 1))    # keep one product for the closing residual
+        m = max(1, min(restart, max_iters - iters))
         V = torch.empty((m + 1, n, T), dtype=torch.complex128, device=dev)
         Z = torch.empty((m, n, T), dtype=torch.complex128, device=dev)
         H = torch.zeros((m + 1, m, T), dtype=torch.complex128, device=dev)
+        H0 = torch.zeros((m + 1, m, T), dtype=torch.complex128, device=dev)   # unrotated, for the residual
         cs = torch.zeros((m, T), dtype=torch.complex128, device=dev)
         sn = torch.zeros((m, T), dtype=torch.complex128, device=dev)
         g = torch.zeros((m + 1, T), dtype=torch.complex128, device=dev)
@@ -77,6 +83,7 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
                 H[: j + 1, j] += h
             hn = torch.linalg.norm(w, dim=0)
             H[j + 1, j] = hn.to(torch.complex128)
+            H0[: j + 2, j] = H[: j + 2, j]
             w /= torch.where(hn == 0, torch.ones_like(hn), hn)
             for i in range(j):                                # apply earlier Givens rotations
                 a, b = H[i, j].clone(), H[i + 1, j].clone()
@@ -92,17 +99,19 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
             g[j + 1] = -sn[j] * g[j]
             g[j] = cs[j].conj() * g[j]
             k = j + 1
+            # |g_{j+1}| is the norm of the Arnoldi residual formed below
             est = (g[j + 1].abs() / bnorm).max().item()
-            if est <= 0.1 * tol:
+            if est <= tol:
                 break
         Hk = H[:k, :k].permute(2, 0, 1).contiguous()
         gk = g[:k].T.contiguous().unsqueeze(-1)
         y = torch.linalg.solve_triangular(Hk, gk, upper=True).squeeze(-1).T.contiguous()   # (k, T)
         E.maxpy(Z[:k], y, X, alpha=1.0)                        # x += Z y
-        del V, Z, H
-        R.copy_(B)
-        E.cgemm(M, X, R, alpha=-1.0, beta=1.0)                # r = b - A x  (FP64)
-        iters += 1
+        c = -(H0[: k + 1, :k] * y.unsqueeze(0)).sum(1)           # beta e1 - H_k y
+        c[0] += beta.to(torch.complex128)
+        R.zero_()
+        E.maxpy(V[: k + 1], c.contiguous(), R, alpha=1.0)       # r = V_{k+1} c
+        del V, Z, H, H0
         beta = torch.linalg.norm(R, dim=0)
         rel = (beta / bnorm).max().item()
 
