@@ -21,6 +21,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "3D N=4096 solves/s: mixed-prec+IR vs FP64 @≤1e-9 err; serving p50/p99 latency"
 PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s per product form
+# the factorisation's trailing-update launches (timed with HIP events on their stream)
+KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
+          "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
+          "bf16x3": "k_cgemm_bf16x3 (block-LDU trailing update launches)"}
+# HBM bytes per trailing-update launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
+# counters' reading for a copy kernel of known size (profiles/); None where not measured
+TRAFFIC = {}
 
 
 def colerr(x, ref):
@@ -44,6 +51,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256,
                     help="configurations of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--serving", type=int, default=40, help="requests per serving load level (0: skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="GiB copied device to device once before the workload (untimed): a transfer of known "
+                         "size that calibrates profiler memory counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -55,6 +65,13 @@ def main():
     from magreete_amd.pipeline import Problem, Solver
 
     dev = torch.device("cuda", 0)
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     w = W.cfg_3d_vector(n_points=args.n_points, n_dirs=args.n_dirs, n_measure=args.n_measure, seed=args.seed)
     pts = w.points.to(dev)
     meas = w.measure.to(dev)
@@ -145,13 +162,17 @@ def main():
         print(json.dumps({"metric": METRIC, "value": None, "error": results}))
         sys.exit(1)
     rb = results[best]
-    fp = results.get("fp64", {})
-    roof = None
-    src = fp if "update_tflops" in fp else rb
-    if src.get("update_tflops"):
-        roof = {"bound": "mfma", "achieved": src["update_tflops"], "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                "frac": src["update_tflops"] / PEAK["fp64"], "traffic": None,
-                "kernel": "k_cgemm_trail<double> (LU trailing update, fp64 path)"}
+
+    def roofline(method):
+        r = results.get(method, {})
+        if not r.get("update_tflops"):
+            return None
+        return {"bound": "mfma", "achieved": r["update_tflops"], "peak": PEAK[method], "unit": "TFLOP/s",
+                "frac": r["update_tflops"] / PEAK[method], "traffic": TRAFFIC.get(method),
+                "kernel": KERNEL[method], "launches_per_step": r["update_launches"] / args.steps,
+                "ms_per_step": r["update_ms"]}
+
+    roof = roofline(best)
     ens = run_ensemble(args, dev) if args.ensemble > 0 else None
     srv = run_serving(args, dev) if args.serving > 0 else None
     line = {
@@ -163,7 +184,8 @@ def main():
         "config": {"workload": "3d_vector N=%d (n=%d) T=%d (%d dirs x 2 pol) measure=%d" % (
             args.n_points, n, w.T, args.n_dirs, args.n_measure), "method": best, "k0": w.k0,
             "seed": args.seed, "tol": args.tol},
-        "roofline": roof, "cpu_baseline": cpu, "methods": results, "errors": errors,
+        "roofline": roof, "roofline_fp64": roofline("fp64") if best != "fp64" else None,
+        "cpu_baseline": cpu, "methods": results, "errors": errors,
         "ensemble": ens, "serving": srv,
     }
     print(json.dumps(line))

@@ -256,8 +256,8 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   auto trail = [&](int64_t k, int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) -> int {
     if (mr <= 0 || nc <= 0) return MG_OK;
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h;
-    return gt(mr, nc, h, cd{-1, 0}, s21(k) + (i0 - r1) * h, h, Ap + r0 * lda + j0, lda, cd{1, 0},
-              Ap + i0 * lda + j0, lda, st);
+    return timed_update(ws, gt, mr, nc, h, cd{-1, 0}, s21(k) + (i0 - r1) * h, h, Ap + r0 * lda + j0, lda, cd{1, 0},
+                        Ap + i0 * lda + j0, lda, st);
   };
   MG_TRY(panel(0, s));
   for (int64_t k = 0; k < nblk; ++k) {

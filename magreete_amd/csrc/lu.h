@@ -44,6 +44,10 @@ struct Workspace {
 typedef int (*GemmFn)(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*, int64_t,
                       hipStream_t);
 
+// fn(...) on s, timed with HIP events on s when ws.prof is enabled
+// (algorithmic flops 8 M N K per launch).
+int timed_update(Workspace& ws, GemmFn fn, int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
 // Sum of recorded update launches since the last read (synchronises the stream).
 int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms);
 

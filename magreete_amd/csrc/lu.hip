@@ -490,26 +490,8 @@ int getrf(Workspace& ws, void* A, int64_t n, int64_t lda, int* ipiv, int* info, 
   auto trail = [&](int64_t k, int64_t jb, int64_t c0, int64_t nc) -> int {
     const int64_t r0 = k + jb, r = n - r0;
     if (r <= 0 || nc <= 0) return MG_OK;
-    UpdateProfile& pr = ws.prof;
-    if (pr.enabled) {
-      if (pr.used * 2 + 2 > pr.ev.size()) {
-        hipEvent_t a, b;
-        MG_HIP(hipEventCreate(&a));
-        MG_HIP(hipEventCreate(&b));
-        pr.ev.push_back(a);
-        pr.ev.push_back(b);
-      }
-      MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
-    }
-    MG_TRY(fn(r, nc, jb, cd{-1, 0}, Ap + r0 * lda + k, lda, Ap + k * lda + c0, lda, cd{1, 0}, Ap + r0 * lda + c0, lda,
-              s));
-    if (pr.enabled) {
-      MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
-      if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
-      pr.flops[pr.used] = 8.0 * (double)r * (double)nc * (double)jb;
-      ++pr.used;
-    }
-    return MG_OK;
+    return timed_update(ws, fn, r, nc, jb, cd{-1, 0}, Ap + r0 * lda + k, lda, Ap + k * lda + c0, lda, cd{1, 0},
+                        Ap + r0 * lda + c0, lda, s);
   };
   // Look-ahead of depth one: once the next panel's columns are updated, that
   // panel is factored on the workspace's second stream while the rest of the
@@ -614,6 +596,29 @@ void workspace_free(Workspace& ws) {
   if (ws.evA) (void)hipEventDestroy(ws.evA);
   if (ws.evB) (void)hipEventDestroy(ws.evB);
   ws = Workspace();
+}
+
+int timed_update(Workspace& ws, GemmFn fn, int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s) {
+  UpdateProfile& pr = ws.prof;
+  if (pr.enabled) {
+    if (pr.used * 2 + 2 > pr.ev.size()) {
+      hipEvent_t a, b;
+      MG_HIP(hipEventCreate(&a));
+      MG_HIP(hipEventCreate(&b));
+      pr.ev.push_back(a);
+      pr.ev.push_back(b);
+    }
+    MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
+  }
+  MG_TRY(fn(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s));
+  if (pr.enabled) {
+    MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
+    if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
+    pr.flops[pr.used] = 8.0 * (double)M * (double)N * (double)K;
+    ++pr.used;
+  }
+  return MG_OK;
 }
 
 int workspace_leaf_blocks(Workspace& ws, int g) {
