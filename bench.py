@@ -24,7 +24,7 @@ PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s per prod
 # the factorisation's trailing-update launches (timed with HIP events on their stream)
 KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
           "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
-          "bf16x3": "k_cgemm_bf16x3 (block-LDU trailing update launches)"}
+          "bf16x3": "k_cgemm_bf16x3<1> (block-LDU trailing update; BF16x3 MFMA, 64x128 tiles)"}
 # HBM bytes per trailing-update launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
 # counters' reading for a copy kernel of known size (profiles/); None where not measured
 TRAFFIC = {}

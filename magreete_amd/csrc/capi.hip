@@ -91,7 +91,7 @@ int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv,
   switch (prec) {
     case MG_C128: return mg::getrf<double>(h->ws, A, n, lda, ipiv, info, s, 256, nullptr);
     case MG_C64: return mg::getrf<float>(h->ws, A, n, lda, ipiv, info, s, 256, nullptr);
-    case MG_BF16X3: return mg::getrf<float>(h->ws, A, n, lda, ipiv, info, s, 512, &mg::cgemm_bf16x3);
+    case MG_BF16X3: return mg::getrf<float>(h->ws, A, n, lda, ipiv, info, s, 512, &mg::cgemm_bf16x3_trail);
     default: return MG_ERR_ARG;
   }
 }
@@ -148,8 +148,9 @@ int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* 
   if (!h || !A || !dinv || !work || !info) return MG_ERR_ARG;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (prec) {
-    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr);
-    case MG_BF16X3: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3);
+    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr, nullptr);
+    case MG_BF16X3:
+      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3, &mg::cgemm_bf16x3_trail);
     default: return MG_ERR_ARG;
   }
 }

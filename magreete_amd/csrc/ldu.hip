@@ -207,7 +207,7 @@ int64_t ldu_dinv_elems(int64_t n) { return ((n + kIB - 1) / kIB) * (int64_t)kIB 
 int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)kIB * kIB; }
 
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
-              GemmFn gemm) {
+              GemmFn gemm, GemmFn update) {
   if (n < 0 || lda < n) return MG_ERR_ARG;
   hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(1), 0, s, info);
   if (n == 0) return MG_OK;
@@ -216,7 +216,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   float2* D = reinterpret_cast<float2*>(dinv);
   float2* W = reinterpret_cast<float2*>(work);
   const GemmFn g = gemm ? gemm : &cgemm<float>;
-  const GemmFn gt = gemm ? gemm : &cgemm_trail<float>;
+  const GemmFn gt = update ? update : (gemm ? gemm : &cgemm_trail<float>);
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb;
   // scratch copies of A12 (h x r) and A21 (r x h) per step, double-buffered by step parity
   auto s12 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n; };

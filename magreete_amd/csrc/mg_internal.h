@@ -47,6 +47,9 @@ int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
 // bf16 MFMA products each (hi*hi + hi*lo + lo*hi).
 int cgemm_bf16x3(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
                  int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
+// The same GEMM under its own kernel symbol (factorisation trailing updates).
+int cgemm_bf16x3_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
+                       int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
 
 // Compute units of the current device (cached per device).
 int cu_count();
