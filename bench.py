@@ -25,9 +25,10 @@ PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s per prod
 KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
           "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
           "bf16x3": "k_cgemm_bf16x3<1> (block-LDU trailing update; BF16x3 MFMA, 64x128 tiles)"}
-# HBM bytes per trailing-update launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
-# counters' reading for a copy kernel of known size (profiles/); None where not measured
-TRAFFIC = {}
+# HBM bytes per trailing-update launch (average over one factorisation of this workload), measured
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the
+# counters' reading for that 2 GiB copy (fetch x2.000, write x1.000)
+TRAFFIC = {"fp64": 2.483e9, "c64": 3.881e8, "bf16x3": 3.685e8}
 
 
 def colerr(x, ref):
@@ -51,6 +52,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256,
                     help="configurations of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--serving", type=int, default=40, help="requests per serving load level (0: skip)")
+    ap.add_argument("--ldos-sites", type=int, default=0,
+                    help="2D TE perturbed-lattice LDOS (sites + 4096 probes) at about this many sites "
+                         "(0: skip; 20000 is the full configuration)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="GiB copied device to device once before the workload (untimed): a transfer of known "
                          "size that calibrates profiler memory counters")
@@ -173,6 +177,7 @@ def main():
                 "ms_per_step": r["update_ms"]}
 
     roof = roofline(best)
+    ldos = run_ldos(args, dev) if args.ldos_sites > 0 else None
     ens = run_ensemble(args, dev) if args.ensemble > 0 else None
     srv = run_serving(args, dev) if args.serving > 0 else None
     line = {
@@ -186,9 +191,43 @@ def main():
             "seed": args.seed, "tol": args.tol},
         "roofline": roof, "roofline_fp64": roofline("fp64") if best != "fp64" else None,
         "cpu_baseline": cpu, "methods": results, "errors": errors,
-        "ensemble": ens, "serving": srv,
+        "ensemble": ens, "serving": srv, "ldos_lattice": ldos,
     }
     print(json.dumps(line))
+
+
+def run_ldos(args, dev):
+    """LDOS of a 2D TE perturbed triangular lattice (unit spacing, sigma 0.1, k0 a = 0.6) at every site
+    and at 4096 probes in the disk, through the drop-in's batch_LDOS_measurements (FP64 LU, batches
+    of 1000 points, self_interaction=False: alpha is already dressed); with the CPU oracle timed at
+    1500 sites and scaled by n^3 when the CPU baseline is on."""
+    from magreete_amd import workloads as W
+    from magreete_amd.Transmission2D import Transmission2D_vector
+    w = W.cfg_2d_te_lattice(n_sites=args.ldos_sites, k0a=0.6, n_angles=8, n_probes=4096, seed=args.seed)
+    n = 2 * w.points.shape[0]
+    pts = torch.cat([w.points, w.probes]).to(dev)
+    cls = Transmission2D_vector(w.points.to(dev), "plane")
+    batches = torch.split(pts, 1000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ld = cls.batch_LDOS_measurements(batches, w.k0, w.alpha, 0.1, self_interaction=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"workload": "2d_te lattice, %d sites (n=%d), LDOS at sites + 4096 probes, k0 a = 0.6" % (
+        w.points.shape[0], n), "seconds": dt, "points": int(pts.shape[0]), "finite": bool(torch.isfinite(ld).all())}
+    print("# ldos lattice n=%d: %.1f s" % (n, dt), file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        from oracle import transmission as O
+        ws = W.cfg_2d_te_lattice(n_sites=1500, k0a=0.6, n_angles=8, n_probes=4096, seed=args.seed)
+        o = O.Transmission2D_vector(ws.points, "plane")
+        t0 = time.perf_counter()
+        o.batch_LDOS_measurements(torch.split(torch.cat([ws.points, ws.probes]), 1000), ws.k0, ws.alpha, 0.1,
+                                  self_interaction=False)
+        tc = time.perf_counter() - t0
+        ns = 2 * ws.points.shape[0]
+        out["cpu_oracle_seconds_at_n%d" % ns] = tc
+        out["cpu_oracle_seconds_scaled_n3"] = tc * (n / ns) ** 3
+    return out
 
 
 def run_ensemble(args, dev):

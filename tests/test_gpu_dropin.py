@@ -145,6 +145,22 @@ def test_ldos_and_dos(kind):
     assert bl.shape == br.shape and (bl - br).abs().max() <= 1e-9 * br.abs().max()
 
 
+@pytest.mark.parametrize("k0a", [0.6, 1.5])
+def test_cfg3_lattice_ldos_reduced(k0a):
+    """The 2D TE perturbed-lattice LDOS flow (sites + probes, one batch of each) at ~400 sites."""
+    say("cfg3 reduced", k0a)
+    w = W.cfg_2d_te_lattice(n_sites=400, k0a=k0a, n_angles=8, n_probes=64, seed=2)
+    dut = T2.Transmission2D_vector(w.points, 'plane')
+    ora = O.Transmission2D_vector(w.points, 'plane')
+    radius = 0.1
+    pts = torch.cat([w.points, w.probes])
+    batches = torch.split(pts, 128)
+    a = dut.batch_LDOS_measurements(batches, w.k0, w.alpha, radius)
+    b = ora.batch_LDOS_measurements(batches, w.k0, w.alpha, radius)
+    assert a.shape == b.shape == (pts.shape[0], 1)
+    assert (a - b).abs().max() <= 1e-9 * b.abs().max()
+
+
 @pytest.mark.parametrize("kind", list(DUT))
 def test_G0_and_greens(kind):
     say("G0", kind)
