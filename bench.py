@@ -20,11 +20,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "3D N=4096 solves/s: mixed-prec+IR vs FP64 @≤1e-9 err; serving p50/p99 latency"
-PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0}   # TFLOP/s per product form
+PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0, "bf16x6": 2500.0 / 6.0}   # TFLOP/s per product form
 # the factorisation's trailing-update launches (timed with HIP events on their stream)
 KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
           "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
-          "bf16x3": "k_cgemm_bf16x3<1> (block-LDU trailing update; BF16x3 MFMA, 64x128 tiles)"}
+          "bf16x3": "k_cgemm_split<2, .., 1> (block-LDU trailing update; BF16x3 MFMA, 128x128 tiles)",
+          "bf16x6": "k_cgemm_split<3, .., 1> (block-LDU trailing update; BF16x6 MFMA, 128x128 tiles)"}
 # HBM bytes per trailing-update launch (average over one factorisation of this workload), measured
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the
 # counters' reading for that 2 GiB copy (fetch x2.000, write x1.000)
@@ -42,7 +43,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--methods", default="fp64,c64,bf16x3")
+    ap.add_argument("--methods", default="fp64,c64,bf16x3,bf16x6")
     ap.add_argument("--n-points", type=int, default=4096)
     ap.add_argument("--n-dirs", type=int, default=128)
     ap.add_argument("--n-measure", type=int, default=2048)
@@ -156,7 +157,7 @@ def main():
 
     # headline: fastest method whose error vs the FP64 path is <= 1e-9 (FP64 itself otherwise)
     best = "fp64" if "solves_per_s" in results.get("fp64", {}) else None
-    for m in ("bf16x3", "c64"):
+    for m in ("bf16x6", "bf16x3", "c64"):
         r = results.get(m, {})
         e = errors.get(m + "_vs_fp64")
         if "solves_per_s" in r and e and max(e.values()) <= 1e-9 and not r["refine"].get("fallback"):
@@ -184,7 +185,8 @@ def main():
         "metric": METRIC, "value": rb["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": rb["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": {"fp64": "complex128", "c64": "complex64 LU + complex128 GMRES-IR",
-                                       "bf16x3": "complex64 (bf16x3 products) LU + complex128 GMRES-IR"}[best],
+                                       "bf16x3": "complex64 (bf16x3 products) LU + complex128 GMRES-IR",
+                                       "bf16x6": "complex64 (bf16x6 products) LU + complex128 GMRES-IR"}[best],
         "data": "synthetic (seeded uniform ball, Fibonacci directions)",
         "config": {"workload": "3d_vector N=%d (n=%d) T=%d (%d dirs x 2 pol) measure=%d" % (
             args.n_points, n, w.T, args.n_dirs, args.n_measure), "method": best, "k0": w.k0,

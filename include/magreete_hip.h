@@ -25,8 +25,10 @@ typedef struct mg_handle mg_handle;
 
 /* Green's-function kinds (one per reference class) */
 enum { MG_2D_TM = 0, MG_2D_TE = 1, MG_3D_VECTOR = 2, MG_3D_SCALAR = 3 };
-/* element precisions: complex128; complex64; complex64 with bf16x3 products */
-enum { MG_C128 = 0, MG_C64 = 1, MG_BF16X3 = 2 };
+/* element precisions: complex128; complex64; complex64 with products from
+ * two-part bf16 splits (three bf16 MFMA products, ~16 bits) or three-part
+ * splits (six products, ~24 bits: FP32-level) */
+enum { MG_C128 = 0, MG_C64 = 1, MG_BF16X3 = 2, MG_BF16X6 = 3 };
 /* assembly flags */
 enum {
   MG_F_REGULARIZE = 1, MG_F_ZERO_SELF = 2, MG_F_DIAG = 4, MG_F_NANFILL = 8,
@@ -58,7 +60,7 @@ int mg_assemble(int kind, const double* rows, int64_t nrows, const double* cols,
                 int flags, double radius, void* out_c128, int64_t ld_c128, void* out_c64, int64_t ld_c64,
                 void* stream);
 
-/* C = alpha A B + beta C, row-major complex (prec MG_C128, MG_C64, MG_BF16X3). */
+/* C = alpha A B + beta C, row-major complex (prec MG_C128, MG_C64, MG_BF16X3, MG_BF16X6). */
 int mg_cgemm(int prec, int64_t m, int64_t n, int64_t k, double alpha_re, double alpha_im, const void* A,
              int64_t lda, const void* B, int64_t ldb, double beta_re, double beta_im, void* C, int64_t ldc,
              void* stream);
@@ -80,7 +82,7 @@ int mg_getrf_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
 int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv,
                   void* B, int64_t ldb, int64_t nrhs, void* work, void* stream);
 /* Block LDU factorisation without pivoting, in place (complex float; prec
- * MG_C64 or MG_BF16X3): A = Lh D Uh with 256 x 256 diagonal blocks; dinv
+ * MG_C64, MG_BF16X3 or MG_BF16X6): A = Lh D Uh with 256 x 256 diagonal blocks; dinv
  * (mg_ldu_dinv_elems(n)) receives the inverses of the D blocks, A the scaled
  * off-diagonal blocks; work holds mg_ldu_work_elems(n) elements; info: first
  * exactly-zero pivot or 0.  The preconditioner of the mixed-precision

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libmagreete_hip.so")
 # enums (include/magreete_hip.h)
 MG_OK = 0
 MG_2D_TM, MG_2D_TE, MG_3D_VECTOR, MG_3D_SCALAR = 0, 1, 2, 3
-MG_C128, MG_C64, MG_BF16X3 = 0, 1, 2
+MG_C128, MG_C64, MG_BF16X3, MG_BF16X6 = 0, 1, 2, 3
 MG_F_REGULARIZE, MG_F_ZERO_SELF, MG_F_DIAG, MG_F_NANFILL = 1, 2, 4, 8
 MG_F_PERIODIC_X, MG_F_PERIODIC_Y, MG_F_PERIODIC_Z = 16, 32, 64
 MG_F_BESSEL_EXACT = 128

@@ -81,9 +81,9 @@ int mg_destroy(mg_handle* h) {
   return MG_OK;
 }
 
-// LU factorisation in place.  prec MG_C128: A is complex double; MG_C64 and
-// MG_BF16X3: A is complex float (BF16X3 runs the trailing updates as three
-// bf16 MFMA products).  ipiv: n device ints; info: one device int (first zero
+// LU factorisation in place.  prec MG_C128: A is complex double; MG_C64,
+// MG_BF16X3 and MG_BF16X6: A is complex float (the BF16 forms run the
+// trailing updates on split bf16 MFMA products).  ipiv: n device ints; info: one device int (first zero
 // pivot, 1-based, or 0).
 int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv, int* info, void* stream) {
   if (!h || !A || !ipiv || !info) return MG_ERR_ARG;
@@ -92,6 +92,7 @@ int mg_getrf(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, int* ipiv,
     case MG_C128: return mg::getrf<double>(h->ws, A, n, lda, ipiv, info, s, 256, nullptr);
     case MG_C64: return mg::getrf<float>(h->ws, A, n, lda, ipiv, info, s, 256, nullptr);
     case MG_BF16X3: return mg::getrf<float>(h->ws, A, n, lda, ipiv, info, s, 512, &mg::cgemm_bf16x3_trail);
+    case MG_BF16X6: return mg::getrf<float>(h->ws, A, n, lda, ipiv, info, s, 512, &mg::cgemm_bf16x6_trail);
     default: return MG_ERR_ARG;
   }
 }
@@ -104,7 +105,8 @@ int mg_getrs(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, con
   switch (prec) {
     case MG_C128: return mg::getrs<double>(LU, n, lda, ipiv, B, ldb, nrhs, s);
     case MG_C64:
-    case MG_BF16X3: return mg::getrs<float>(LU, n, lda, ipiv, B, ldb, nrhs, s);
+    case MG_BF16X3:
+    case MG_BF16X6: return mg::getrs<float>(LU, n, lda, ipiv, B, ldb, nrhs, s);
     default: return MG_ERR_ARG;
   }
 }
@@ -119,7 +121,8 @@ int mg_getrf_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
   switch (prec) {
     case MG_C128: return mg::diag_inverses<double>(LU, n, lda, dinv, s);
     case MG_C64:
-    case MG_BF16X3: return mg::diag_inverses<float>(LU, n, lda, dinv, s);
+    case MG_BF16X3:
+    case MG_BF16X6: return mg::diag_inverses<float>(LU, n, lda, dinv, s);
     default: return MG_ERR_ARG;
   }
 }
@@ -134,6 +137,7 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
     case MG_C128: return mg::getrs_dinv<double>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
     case MG_C64: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s);
     case MG_BF16X3: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3);
+    case MG_BF16X6: return mg::getrs_dinv<float>(LU, n, lda, ipiv, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x6);
     default: return MG_ERR_ARG;
   }
 }
@@ -142,15 +146,18 @@ int64_t mg_ldu_dinv_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_dinv_elems(n);
 int64_t mg_ldu_work_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_work_elems(n); }
 
 // Block LDU factorisation without pivoting (complex float; MG_C64: FP32 MFMA
-// GEMMs, MG_BF16X3: BF16x3 GEMMs).
+// GEMMs, MG_BF16X3 / MG_BF16X6: split-bf16 GEMMs).  The FP32-level forms
+// refine each diagonal-block inverse by one FP32 Newton-Schulz step.
 int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info,
                  void* stream) {
   if (!h || !A || !dinv || !work || !info) return MG_ERR_ARG;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (prec) {
-    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr, nullptr);
+    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr, nullptr, true);
     case MG_BF16X3:
-      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3, &mg::cgemm_bf16x3_trail);
+      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3, &mg::cgemm_bf16x3_trail, false);
+    case MG_BF16X6:
+      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x6, &mg::cgemm_bf16x6_trail, true);
     default: return MG_ERR_ARG;
   }
 }
@@ -162,6 +169,7 @@ int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda
   switch (prec) {
     case MG_C64: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, nullptr);
     case MG_BF16X3: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3);
+    case MG_BF16X6: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x6);
     default: return MG_ERR_ARG;
   }
 }

@@ -139,6 +139,7 @@ extern "C" int mg_cgemm(int prec, int64_t m, int64_t n, int64_t k, double alpha_
     case MG_C128: return mg::cgemm<double>(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
     case MG_C64: return mg::cgemm<float>(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
     case MG_BF16X3: return mg::cgemm_bf16x3(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
+    case MG_BF16X6: return mg::cgemm_bf16x6(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
     default: return MG_ERR_ARG;
   }
 }

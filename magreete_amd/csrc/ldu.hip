@@ -207,7 +207,7 @@ int64_t ldu_dinv_elems(int64_t n) { return ((n + kIB - 1) / kIB) * (int64_t)kIB 
 int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)kIB * kIB; }
 
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
-              GemmFn gemm, GemmFn update) {
+              GemmFn gemm, GemmFn update, bool newton) {
   if (n < 0 || lda < n) return MG_ERR_ARG;
   hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(1), 0, s, info);
   if (n == 0) return MG_OK;
@@ -222,11 +222,10 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   auto s12 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n; };
   auto s21 = [&](int64_t k) { return W + (k & 1) * 2 * nb * n + nb * n; };
   // step k's inverse and panels, on stream st
-  // With FP32 products the BF16x3 block inverse would limit the factor's
-  // accuracy: one Newton-Schulz step X <- X (2I - S X) in FP32 squares its
-  // error.  X0 and R live after the panel copies in the work buffer (panels
-  // run one at a time).
-  const bool newton = (gemm == nullptr);
+  // With FP32-level products the BF16x3 block inverse would limit the
+  // factor's accuracy (newton): one Newton-Schulz step X <- X (2I - S X) in
+  // FP32 squares its error.  X0 and R live after the panel copies in the work
+  // buffer (panels run one at a time).
   float2* X0 = W + 4 * nb * n;
   float2* R = X0 + nb * nb;
   auto panel = [&](int64_t k, hipStream_t st) -> int {

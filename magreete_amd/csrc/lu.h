@@ -86,11 +86,12 @@ int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_
 // diagonal blocks of the Schur complements in dinv (ldu_dinv_elems), scaled
 // off-diagonal blocks in place, work: ldu_work_elems(n) elements; the solve
 // needs n x nrhs work elements.  gemm: GEMM of the panels and block solves,
-// update: GEMM of the trailing updates (nullptr: the FP32 MFMA GEMMs).
+// update: GEMM of the trailing updates (nullptr: the FP32 MFMA GEMMs);
+// newton: refine each block inverse by one FP32 Newton-Schulz step.
 int64_t ldu_dinv_elems(int64_t n);
 int64_t ldu_work_elems(int64_t n);
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
-              GemmFn gemm, GemmFn update);
+              GemmFn gemm, GemmFn update, bool newton);
 int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb, int64_t nrhs,
               void* work, hipStream_t s, GemmFn gemm);
 

@@ -36,6 +36,7 @@ enum {
   MG_C128 = 0,   // complex double, interleaved
   MG_C64 = 1,    // complex float, interleaved
   MG_BF16X3 = 2, // complex float storage, products as three bf16 MFMAs
+  MG_BF16X6 = 3, // complex float storage, products as six bf16 MFMAs (three-part splits)
 };
 
 // Assembly flags.

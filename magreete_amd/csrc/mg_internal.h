@@ -50,6 +50,12 @@ int cgemm_bf16x3(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64
 // The same GEMM under its own kernel symbol (factorisation trailing updates).
 int cgemm_bf16x3_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
                        int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
+// BF16x6: three-part bf16 splits and the six products with i + j <= 2 per
+// real product (FP32-level accuracy).
+int cgemm_bf16x6(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
+                 int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
+int cgemm_bf16x6_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B,
+                       int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
 
 // Compute units of the current device (cached per device).
 int cu_count();

@@ -15,7 +15,7 @@ from . import _lib as L
 KINDS = {"2d_tm": L.MG_2D_TM, "2d_te": L.MG_2D_TE, "3d_vector": L.MG_3D_VECTOR, "3d_scalar": L.MG_3D_SCALAR}
 DIM = {"2d_tm": 1, "2d_te": 2, "3d_vector": 3, "3d_scalar": 1}
 SPACE = {"2d_tm": 2, "2d_te": 2, "3d_vector": 3, "3d_scalar": 3}
-PRECS = {"c128": L.MG_C128, "c64": L.MG_C64, "bf16x3": L.MG_BF16X3}
+PRECS = {"c128": L.MG_C128, "c64": L.MG_C64, "bf16x3": L.MG_BF16X3, "bf16x6": L.MG_BF16X6}
 
 _handles = {}
 

@@ -21,8 +21,9 @@ def crand(shape, rng, dtype=np.complex128):
     return (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(dtype)
 
 
-@pytest.mark.parametrize("prec", ["c128", "c64", "bf16x3"])
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 33, 9), (64, 64, 8), (130, 70, 257), (300, 200, 100), (5, 700, 3)])
+@pytest.mark.parametrize("prec", ["c128", "c64", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 33, 9), (64, 64, 8), (130, 70, 257), (300, 200, 100), (5, 700, 3),
+                                 (4200, 4300, 70)])
 def test_cgemm(prec, mnk):
     m, n, k = mnk
     say("cgemm", prec, mnk)
@@ -35,8 +36,8 @@ def test_cgemm(prec, mnk):
     E.cgemm(torch.from_numpy(A).to(DEV), torch.from_numpy(B).to(DEV), Cd, alpha, beta, prec)
     got = Cd.cpu().numpy().astype(np.complex128)
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    # BF16x3 keeps ~16 significant bits per product (the lo x lo term is dropped)
-    assert err < {"c128": 1e-14, "c64": 2e-6, "bf16x3": 5e-5}[prec], err
+    # BF16x3 keeps ~16 significant bits per product, BF16x6 ~24 (FP32-level)
+    assert err < {"c128": 1e-14, "c64": 2e-6, "bf16x3": 5e-5, "bf16x6": 2e-6}[prec], err
 
 
 def test_cgemm_strided_views():

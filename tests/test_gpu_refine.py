@@ -23,7 +23,7 @@ def crand(shape, rng):
     return rng.normal(size=shape) + 1j * rng.normal(size=shape)
 
 
-@pytest.mark.parametrize("prec", ["c64", "bf16x3"])
+@pytest.mark.parametrize("prec", ["c64", "bf16x3", "bf16x6"])
 @pytest.mark.parametrize("n", [7, 256, 300, 512, 777, 1100])
 def test_ldu_factor_solve(n, prec):
     """Unpivoted block LDU (the preconditioner): one solve against LAPACK."""
@@ -39,7 +39,7 @@ def test_ldu_factor_solve(n, prec):
     x = Bd.cpu().numpy().astype(np.complex128)
     # bf16x3: every product carries ~16 significant bits (about 1e-5 each);
     # c64: FP32 products and Newton-refined block inverses
-    tol = 1e-5 if prec == "c64" else 3e-4
+    tol = 3e-4 if prec == "bf16x3" else 1e-5
     assert (np.linalg.norm(x - ref, axis=0) / np.linalg.norm(ref, axis=0)).max() < tol
 
 
@@ -57,7 +57,7 @@ def colerr(x, ref):
     return (torch.linalg.norm(x - ref, dim=0) / torch.linalg.norm(ref, dim=0)).max().item()
 
 
-@pytest.mark.parametrize("method", ["c64", "bf16x3"])
+@pytest.mark.parametrize("method", ["c64", "bf16x3", "bf16x6"])
 def test_mixed_matches_fp64_and_oracle(method):
     say("mixed", method)
     w, prob = _problem()
