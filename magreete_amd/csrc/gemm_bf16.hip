@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_split(int64_t M, int64_t
         rb[p][e] = v;
       }
   };
-  auto stash = [&]() {
+  auto stash = [&](SplitSmem<NS, BM, BN>& sm) {
     unsigned short s[2 * NS][8];
 #pragma unroll
     for (int p = 0; p < APASS; ++p) {
@@ -143,13 +143,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_split(int64_t M, int64_t
 #pragma unroll
       for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0.f; ci[a][b][r] = 0.f; }
 
-  const int64_t KT = (K + BK - 1) / BK;
-  fetch(0);
-  for (int64_t kt = 0; kt < KT; ++kt) {
-    __syncthreads();   // previous tile's reads are done
-    stash();
-    __syncthreads();
-    if (kt + 1 < KT) fetch((kt + 1) * BK);
+  auto compute = [&](const SplitSmem<NS, BM, BN>& sm) {
     bf16x8 afr[TM][NS], afi[TM][NS], afn[TM][NS];
 #pragma unroll
     for (int a = 0; a < TM; ++a) {
@@ -173,6 +167,16 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_split(int64_t M, int64_t
 #pragma unroll
       for (int a = 0; a < TM; ++a) cmac_split<NS>(afr[a], afi[a], afn[a], bfr, bfi, cr[a][b], ci[a][b]);
     }
+  };
+
+  const int64_t KT = (K + BK - 1) / BK;
+  fetch(0);
+  for (int64_t kt = 0; kt < KT; ++kt) {
+    __syncthreads();   // previous tile's reads are done
+    stash(sm);
+    __syncthreads();
+    if (kt + 1 < KT) fetch((kt + 1) * BK);
+    compute(sm);
   }
 
   const bool beta0 = beta.x == 0.f && beta.y == 0.f;
@@ -203,7 +207,8 @@ template <int NS, int BM, int BN, int WM, int WN, int TRAIL>
 int launch_split(int64_t M, int64_t N, int64_t K, mg::cd alpha, const void* A, int64_t lda, const void* B,
                  int64_t ldb, mg::cd beta, void* C, int64_t ldc, hipStream_t s) {
   const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
-  hipLaunchKernelGGL((k_cgemm_split<NS, BM, BN, WM, WN, TRAIL>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N, K,
+  hipLaunchKernelGGL((k_cgemm_split<NS, BM, BN, WM, WN, TRAIL>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N,
+                     K,
                      make_float2((float)alpha.re, (float)alpha.im), reinterpret_cast<const float2*>(A), lda,
                      reinterpret_cast<const float2*>(B), ldb, make_float2((float)beta.re, (float)beta.im),
                      reinterpret_cast<float2*>(C), ldc, nbm, nbn);
