@@ -10,11 +10,9 @@
 //                       products are FP32;
 //   Uh_k = S_k^{-1} A12, Lh_k = A21 S_k^{-1}     stored over A12 and A21;
 //   A22 -= A21 S_k^{-1} A12                      the trailing update.
-// A solve is then GEMMs only: per 1024-row super-block, one product with
-// the inverse of its unit-triangular diagonal part (formed once per factor)
-// and one update of the rows below (Lh, forward) or above (Uh, backward),
-// with one batched product with the S_k^{-1} in between; no triangular
-// kernels and no per-column dependency chains.
+// A solve is then GEMMs only: one per block column with Lh (forward), one
+// batched product with the S_k^{-1}, one per block column with Uh
+// (backward); no triangular kernels and no per-column dependency chains.
 // Look-ahead: once step k has updated block row and column k+1, step k+1's
 // inverse and panels run on the workspace's second stream while the rest of
 // step k's trailing update runs on the caller's stream.
@@ -28,7 +26,6 @@
 namespace {
 
 constexpr int kIB = 256;          // diagonal block order (= kInvBlock)
-constexpr int kSB = 1024;         // super-block order of the solves (4 diagonal blocks)
 constexpr int kPW = 32;           // Gauss-Jordan panel width
 constexpr int kPK = kPW + 8;      // padded k stride of the bf16 planes
 constexpr int kInvThreads = 512;  // 8 waves; wave w owns rows 32w .. 32w+31
@@ -194,14 +191,6 @@ __global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __r
 
 __global__ void k_zero_info(int* p) { *p = 0; }
 
-// count consecutive kSB x kSB identities
-__global__ void k_eye_super(float2* X, int64_t count) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= count * kSB * kSB) return;
-  const int64_t w = e % ((int64_t)kSB * kSB);
-  X[e] = make_float2(w / kSB == w % kSB ? 1.f : 0.f, 0.f);
-}
-
 // R[i][j] = 2 delta_ij for i, j < h (row stride kIB)
 __global__ void k_two_eye(float2* R, int h) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -214,50 +203,7 @@ __global__ void k_two_eye(float2* R, int h) {
 
 namespace mg {
 
-// D^{-1} blocks, then the inverses of the unit-triangular kSB x kSB diagonal
-// super-blocks of Lh and of Uh
-int64_t ldu_dinv_elems(int64_t n) {
-  return ((n + kIB - 1) / kIB) * (int64_t)kIB * kIB + 2 * ((n + kSB - 1) / kSB) * (int64_t)kSB * kSB;
-}
-
-// Inverses of the diagonal super-blocks of Lh (unit lower) and Uh (unit
-// upper) by block substitution on identities, FP32 products, batched over
-// the full super-blocks: the solves then take one GEMM per super-block for
-// its diagonal part instead of one per block column.
-static int super_inverses(const float2* A, int64_t n, int64_t lda, float2* SL, float2* SU, hipStream_t s) {
-  const int64_t nsb = (n + kSB - 1) / kSB, full = n / kSB, ss = (int64_t)kSB * kSB, sa = kSB * lda + kSB;
-  hipLaunchKernelGGL(k_eye_super, dim3((unsigned)((2 * nsb * ss + 255) / 256)), dim3(256), 0, s, SL, 2 * nsb);
-  MG_HIP(hipGetLastError());
-  // X L = I within a super-block of S rows starting at r0: block rows b+1.. -= L[b+1.., b] X[b]
-  auto lower = [&](int64_t r0, int64_t S, int64_t batch, float2* X) -> int {
-    const int64_t nbs = (S + kIB - 1) / kIB;
-    for (int64_t b = 0; b + 1 < nbs; ++b) {
-      const int64_t o = (b + 1) * kIB;
-      MG_TRY(cgemm_batched<float>(S - o, o, kIB, cd{-1, 0}, A + (r0 + o) * lda + r0 + b * kIB, lda, sa,
-                                  X + b * kIB * kSB, kSB, ss, cd{1, 0}, X + o * kSB, kSB, ss, batch, s));
-    }
-    return MG_OK;
-  };
-  auto upper = [&](int64_t r0, int64_t S, int64_t batch, float2* X) -> int {
-    const int64_t nbs = (S + kIB - 1) / kIB;
-    for (int64_t b = nbs - 1; b > 0; --b) {
-      const int64_t o = b * kIB, hb = (S - o < kIB) ? S - o : kIB;
-      MG_TRY(cgemm_batched<float>(o, S - o, hb, cd{-1, 0}, A + r0 * lda + r0 + o, lda, sa, X + o * kSB + o, kSB, ss,
-                                  cd{1, 0}, X + o, kSB, ss, batch, s));
-    }
-    return MG_OK;
-  };
-  if (full > 0) {
-    MG_TRY(lower(0, kSB, full, SL));
-    MG_TRY(upper(0, kSB, full, SU));
-  }
-  if (full < nsb) {
-    const int64_t r0 = full * kSB, S = n - r0;
-    MG_TRY(lower(r0, S, 1, SL + full * ss));
-    MG_TRY(upper(r0, S, 1, SU + full * ss));
-  }
-  return MG_OK;
-}
+int64_t ldu_dinv_elems(int64_t n) { return ((n + kIB - 1) / kIB) * (int64_t)kIB * kIB; }
 int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)kIB * kIB; }
 
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
@@ -332,8 +278,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
       MG_TRY(panel(k + 1, s));
     }
   }
-  float2* SL = D + nblk * nb * nb;
-  return super_inverses(Ap, n, lda, SL, SL + ((n + kSB - 1) / kSB) * (int64_t)kSB * kSB, s);
+  return MG_OK;
 }
 
 int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb, int64_t nrhs,
@@ -346,31 +291,25 @@ int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B
   float2* Bp = reinterpret_cast<float2*>(B);
   float2* W = reinterpret_cast<float2*>(work);
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
-  const int64_t nsb = (n + kSB - 1) / kSB, ss = (int64_t)kSB * kSB;
-  const float2* SL = D + nblk * nb * nb;
-  const float2* SU = SL + nsb * ss;
-  // forward over super-blocks: W_t = Lh_tt^{-1} B_t, B[r1:] -= Lh[r1:, t] W_t
-  for (int64_t t = 0; t < nsb; ++t) {
-    const int64_t r0 = t * kSB, S = (n - r0 < kSB) ? n - r0 : kSB, r1 = r0 + S;
-    MG_TRY(g(S, nrhs, S, cd{1, 0}, SL + t * ss, kSB, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * nrhs, nrhs, s));
-    if (r1 < n)
-      MG_TRY(g(n - r1, nrhs, S, cd{-1, 0}, Ap + r1 * lda + r0, lda, W + r0 * nrhs, nrhs, cd{1, 0}, Bp + r1 * ldb,
-               ldb, s));
+  // forward: B[r1:] -= Lh[r1:, block k] B[block k]
+  for (int64_t k = 0; k + 1 < nblk; ++k) {
+    const int64_t r0 = k * nb, r1 = r0 + nb;
+    MG_TRY(g(n - r1, nrhs, nb, cd{-1, 0}, Ap + r1 * lda + r0, lda, Bp + r0 * ldb, ldb, cd{1, 0}, Bp + r1 * ldb, ldb,
+             s));
   }
-  // B = D^{-1} W, block by block (FP32 products)
+  // W = D^{-1} B, block by block (FP32 products)
   if (full > 0)
-    MG_TRY(cgemm_batched<float>(nb, nrhs, nb, cd{1, 0}, D, nb, nb * nb, W, nrhs, nb * nrhs, cd{0, 0}, Bp, ldb,
-                                nb * ldb, full, s));
+    MG_TRY(cgemm_batched<float>(nb, nrhs, nb, cd{1, 0}, D, nb, nb * nb, Bp, ldb, nb * ldb, cd{0, 0}, W, nrhs,
+                                nb * nrhs, full, s));
   if (full < nblk) {
     const int64_t r0 = full * nb, h = n - r0;
-    MG_TRY(cgemm<float>(h, nrhs, h, cd{1, 0}, D + full * nb * nb, nb, W + r0 * nrhs, nrhs, cd{0, 0}, Bp + r0 * ldb,
-                        ldb, s));
+    MG_TRY(cgemm<float>(h, nrhs, h, cd{1, 0}, D + full * nb * nb, nb, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * nrhs,
+                        nrhs, s));
   }
-  // backward over super-blocks: W_t = Uh_tt^{-1} B_t, B[:r0] -= Uh[:r0, t] W_t
-  for (int64_t t = nsb - 1; t >= 0; --t) {
-    const int64_t r0 = t * kSB, S = (n - r0 < kSB) ? n - r0 : kSB;
-    MG_TRY(g(S, nrhs, S, cd{1, 0}, SU + t * ss, kSB, Bp + r0 * ldb, ldb, cd{0, 0}, W + r0 * nrhs, nrhs, s));
-    if (r0 > 0) MG_TRY(g(r0, nrhs, S, cd{-1, 0}, Ap + r0, lda, W + r0 * nrhs, nrhs, cd{1, 0}, Bp, ldb, s));
+  // backward: W[:r0] -= Uh[:r0, block k] W[block k]
+  for (int64_t k = nblk - 1; k > 0; --k) {
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb;
+    MG_TRY(g(r0, nrhs, h, cd{-1, 0}, Ap + r0, lda, W + r0 * nrhs, nrhs, cd{1, 0}, W, nrhs, s));
   }
   MG_HIP(hipMemcpy2DAsync(Bp, ldb * sizeof(float2), W, nrhs * sizeof(float2), nrhs * sizeof(float2), n,
                           hipMemcpyDeviceToDevice, s));

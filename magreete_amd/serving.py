@@ -125,7 +125,7 @@ def resident_bytes(n, method):
     """Bytes that stay resident for a factorised system of order n."""
     if method == "fp64":
         return 16 * n * n + 16 * 256 * 256 * 2 * ((n + 255) // 256)
-    return 16 * n * n + 8 * n * n + 8 * (256 * 256 * ((n + 255) // 256) + 2 * 1024 * 1024 * ((n + 1023) // 1024))
+    return 16 * n * n + 8 * n * n + 8 * 256 * 256 * ((n + 255) // 256)
 
 
 def transient_bytes(n, T, n_meas, method, restart=12):
