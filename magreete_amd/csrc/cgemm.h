@@ -207,17 +207,6 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
   const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
   const int li = lane & 15, lk = lane >> 4;
 
-  // 4M: acc[0] = Cr, acc[1] = Ci.  3M: acc[0..2] = t1, t2, t3.
-  ACC acc[NP == 3 ? 3 : 2][TM][TN];
-#pragma unroll
-  for (int q = 0; q < (NP == 3 ? 3 : 2); ++q)
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-      for (int b = 0; b < TN; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[q][a][b][r] = 0;
-
   const int64_t KT = (K + BK - 1) / BK;
   constexpr bool kMem = std::is_same<ASrc, MemA>::value;
   ARegs<T, BM, BK, NT> ar;
@@ -231,6 +220,51 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
     a_green<ASrc::kind, T, BM, BN, BK, NT, NP>(sm[0], asrc, i0, 0);
   }
   b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, 0, j0, br);
+
+  // Accumulator item r of lane l holds column l%16 and row l/16 + 4r (f64
+  // MFMA) or 4(l/16) + r (f32 MFMA), as measured on gfx950.  The
+  // accumulators start from (beta / alpha) C, loaded here so that its
+  // latency hides behind the first k-steps; the epilogue then only scales by
+  // alpha and writes (alpha = 0 keeps C = beta C).
+  // 4M: acc[0] = Cr, acc[1] = Ci.  3M: acc[0..2] = t1, t2, t3, started at
+  // t1 = cr, t2 = 0, t3 = cr + ci so that t1 - t2 and t3 - t1 - t2 carry c.
+  T2* C = reinterpret_cast<T2*>(Cp);
+  constexpr bool kRowsStrided = std::is_same<T, double>::value;
+  const bool alpha0 = (alpha.x == 0 && alpha.y == 0);
+  const bool cload = !(beta.x == 0 && beta.y == 0) && !alpha0;
+  T2 q;
+  q.x = 0;
+  q.y = 0;
+  if (cload) {
+    const T d = alpha.x * alpha.x + alpha.y * alpha.y;
+    q.x = (beta.x * alpha.x + beta.y * alpha.y) / d;
+    q.y = (beta.y * alpha.x - beta.x * alpha.y) / d;
+  }
+  ACC acc[NP == 3 ? 3 : 2][TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a) {
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const int64_t col = j0 + wn * (BN / WN) + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = i0 + wm * (BM / WM) + a * 16 + (kRowsStrided ? lk + 4 * r : lk * 4 + r);
+        T2 c;
+        c.x = 0;
+        c.y = 0;
+        if (cload && row < M && col < N) c = C[row * ldc + col];
+        const T cr = q.x * c.x - q.y * c.y, ci = q.x * c.y + q.y * c.x;
+        acc[0][a][b][r] = cr;
+        if constexpr (NP == 3) {
+          acc[1][a][b][r] = 0;
+          acc[2][a][b][r] = cr + ci;
+        } else {
+          acc[1][a][b][r] = ci;
+        }
+      }
+    }
+  }
+
   b_store<T, BM, BN, BK, NT, NP>(sm[0], br);
   __syncthreads();
 
@@ -278,12 +312,7 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
     __syncthreads();
   }
 
-  // epilogue: C = alpha*acc + beta*C.  Accumulator item r of lane l holds
-  // column l%16 and row l/16 + 4r (f64 MFMA) or 4(l/16) + r (f32 MFMA),
-  // as measured on gfx950.
-  T2* C = reinterpret_cast<T2*>(Cp);
-  const bool beta0 = (beta.x == 0 && beta.y == 0);
-  constexpr bool kRowsStrided = std::is_same<T, double>::value;
+  // epilogue: C = alpha * acc (acc already holds (beta / alpha) C).
 #pragma unroll
   for (int a = 0; a < TM; ++a) {
 #pragma unroll
@@ -305,10 +334,10 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
           T2 out;
           out.x = alpha.x * xr - alpha.y * xi;
           out.y = alpha.x * xi + alpha.y * xr;
-          if (!beta0) {
+          if (alpha0) {
             const T2 c = C[row * ldc + col];
-            out.x += beta.x * c.x - beta.y * c.y;
-            out.y += beta.x * c.y + beta.y * c.x;
+            out.x = beta.x * c.x - beta.y * c.y;
+            out.y = beta.x * c.y + beta.y * c.x;
           }
           C[row * ldc + col] = out;
         }

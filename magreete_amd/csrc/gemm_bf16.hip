@@ -10,9 +10,7 @@
 // 16 TM x 16 TN), k-steps of 32 complex.  The next k-tile is fetched into
 // registers while the current one is multiplied; LDS holds it split into
 // 2 NS bf16 planes (re parts, im parts) with k contiguous, so each lane reads
-// its 8 k-values of a plane as one 16-byte load.  64 x 128 tiles (4 waves)
-// for small grids, 128 x 128 (8 waves of 32 x 64) for large ones: the larger
-// tile halves the operand reads per flop, which bound the small one.
+// its 8 k-values of a plane as one 16-byte load.
 #include "bf16x3.h"
 #include "cgemm.h"
 #include "mg_internal.h"
@@ -135,13 +133,31 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_split(int64_t M, int64_t
     }
   };
 
+  // The accumulators start from (beta / alpha) C, loaded before the k-loop so
+  // that its latency hides behind the first k-steps; the epilogue then only
+  // scales by alpha and writes.  (alpha = 0 keeps C = beta C.)
+  const bool alpha0 = alpha.x == 0.f && alpha.y == 0.f;
+  const bool cload = !(beta.x == 0.f && beta.y == 0.f) && !alpha0;
+  float2 q = make_float2(0.f, 0.f);   // beta / alpha
+  if (cload) {
+    const float d = alpha.x * alpha.x + alpha.y * alpha.y;
+    q = make_float2((beta.x * alpha.x + beta.y * alpha.y) / d, (beta.y * alpha.x - beta.x * alpha.y) / d);
+  }
   v4f cr[TM][TN], ci[TM][TN];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < TN; ++b)
+    for (int b = 0; b < TN; ++b) {
+      const int64_t col = j0 + wn * (BN / WN) + b * 16 + li;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0.f; ci[a][b][r] = 0.f; }
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = i0 + wm * (BM / WM) + a * 16 + lg * 4 + r;
+        float2 c = make_float2(0.f, 0.f);
+        if (cload && row < M && col < N) c = C[row * ldc + col];
+        cr[a][b][r] = q.x * c.x - q.y * c.y;
+        ci[a][b][r] = q.x * c.y + q.y * c.x;
+      }
+    }
 
   auto compute = [&](const SplitSmem<NS, BM, BN>& sm) {
     bf16x8 afr[TM][NS], afi[TM][NS], afn[TM][NS];
@@ -179,7 +195,6 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_split(int64_t M, int64_t
     compute(sm);
   }
 
-  const bool beta0 = beta.x == 0.f && beta.y == 0.f;
 #pragma unroll
   for (int a = 0; a < TM; ++a) {
 #pragma unroll
@@ -191,10 +206,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_split(int64_t M, int64_t
         if (row < M && col < N) {
           const float xr = cr[a][b][r], xi = ci[a][b][r];
           float2 o = make_float2(alpha.x * xr - alpha.y * xi, alpha.x * xi + alpha.y * xr);
-          if (!beta0) {
+          if (alpha0) {
             const float2 c = C[row * ldc + col];
-            o.x += beta.x * c.x - beta.y * c.y;
-            o.y += beta.x * c.y + beta.y * c.x;
+            o = make_float2(beta.x * c.x - beta.y * c.y, beta.x * c.y + beta.y * c.x);
           }
           C[row * ldc + col] = o;
         }
@@ -216,7 +230,10 @@ int launch_split(int64_t M, int64_t N, int64_t K, mg::cd alpha, const void* A, i
   return MG_OK;
 }
 
-// 128 x 128 tiles when the grid still gives every CU a tile, else 64 x 128.
+// 128 x 128 tiles (8 waves of 32 x 64) when the grid still gives every CU a
+// tile, else 64 x 128 (4 waves): the larger tile halves the operand reads per
+// flop, which bound the small one.  With three-part splits a 64 x 128 tile
+// needs 90 KiB of LDS, one workgroup per CU; 64 x 64 (60 KiB) keeps two.
 template <int NS, int TRAIL>
 int dispatch_split(int64_t M, int64_t N, int64_t K, mg::cd alpha, const void* A, int64_t lda, const void* B,
                    int64_t ldb, mg::cd beta, void* C, int64_t ldc, hipStream_t s) {
@@ -225,7 +242,10 @@ int dispatch_split(int64_t M, int64_t N, int64_t K, mg::cd alpha, const void* A,
   const int64_t big = ((M + 127) / 128) * ((N + 127) / 128);
   if (big >= (int64_t)mg::cu_count())
     return launch_split<NS, 128, 128, 4, 2, TRAIL>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  return launch_split<NS, 64, 128, 2, 2, TRAIL>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if constexpr (NS == 3)
+    return launch_split<NS, 64, 64, 2, 2, TRAIL>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  else
+    return launch_split<NS, 64, 128, 2, 2, TRAIL>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 }  // namespace
