@@ -65,6 +65,17 @@ int mg_cgemm(int prec, int64_t m, int64_t n, int64_t k, double alpha_re, double 
              int64_t lda, const void* B, int64_t ldb, double beta_re, double beta_im, void* C, int64_t ldc,
              void* stream);
 
+/* mg_cgemm for tall-skinny products (the residual and Krylov products
+ * M V of Transmission3D.py:205 / Transmission2D.py:230's systems when solved
+ * iteratively): K is split over a workspace of mg_cgemm_work_elems(prec, m,
+ * n, k) elements of C's type so that the tile grid fills the CUs, and the
+ * partial products are summed in a fixed order.  work may be NULL (or the
+ * count 0), which runs mg_cgemm; MG_BF16X3 / MG_BF16X6 never split. */
+int64_t mg_cgemm_work_elems(int prec, int64_t m, int64_t n, int64_t k);
+int mg_cgemm_ws(int prec, int64_t m, int64_t n, int64_t k, double alpha_re, double alpha_im, const void* A,
+                int64_t lda, const void* B, int64_t ldb, double beta_re, double beta_im, void* C, int64_t ldc,
+                void* work, int64_t work_elems, void* stream);
+
 /* LU with partial pivoting in place; the factor step of torch.linalg.solve
  * (Transmission2D.py:230, 742; Transmission3D.py:205, 717).  ipiv: n device
  * ints (0-based row swapped with row k); info: one device int. */

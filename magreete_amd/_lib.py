@@ -32,6 +32,9 @@ SIGNATURES = {
     "mg_assemble": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _dbl,
                     _vp, _i64, _vp, _i64, _vp],
     "mg_cgemm": [_int, _i64, _i64, _i64, _dbl, _dbl, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _i64, _vp],
+    "mg_cgemm_work_elems": [_int, _i64, _i64, _i64],
+    "mg_cgemm_ws": [_int, _i64, _i64, _i64, _dbl, _dbl, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _i64, _vp, _i64,
+                    _vp],
     "mg_getrf": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp],
     "mg_getrs": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp],
     "mg_check": [_vp, _vp],
@@ -55,7 +58,7 @@ SIGNATURES = {
     "mg_mdot": [_vp, _i64, _i64, _int, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _vp],
     "mg_maxpy": [_vp, _i64, _i64, _int, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _vp],
 }
-_RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64, "mg_mdot_work_elems": _c.c_int64,
+_RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64, "mg_cgemm_work_elems": _c.c_int64, "mg_mdot_work_elems": _c.c_int64,
              "mg_ldu_dinv_elems": _c.c_int64, "mg_ldu_work_elems": _c.c_int64}
 
 _lib = None

@@ -78,6 +78,46 @@ int dispatch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t l
                                                   s);
 }
 
+// C = alpha sum_s W_s + beta C over the S split-K partials W_s (M x N, dense),
+// summed in the fixed order s = 0 .. S-1 so that results do not depend on
+// scheduling.
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_reduce(int64_t M, int64_t N, int S, const typename Cplx<T>::t* W,
+                                                       typename Cplx<T>::t alpha, typename Cplx<T>::t beta,
+                                                       typename Cplx<T>::t* C, int64_t ldc) {
+  typedef typename Cplx<T>::t T2;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int64_t i = idx / N, j = idx % N;
+  T2 acc = W[idx];
+  for (int q = 1; q < S; ++q) {
+    const T2 w = W[(int64_t)q * M * N + idx];
+    acc.x += w.x;
+    acc.y += w.y;
+  }
+  T2 out;
+  out.x = alpha.x * acc.x - alpha.y * acc.y;
+  out.y = alpha.x * acc.y + alpha.y * acc.x;
+  if (beta.x != 0 || beta.y != 0) {
+    const T2 c = C[i * ldc + j];
+    out.x += beta.x * c.x - beta.y * c.y;
+    out.y += beta.x * c.y + beta.y * c.x;
+  }
+  C[i * ldc + j] = out;
+}
+
+// Number of K-splits for a tall-skinny product whose 128 x 64 tile grid
+// leaves CUs idle: enough splits for two workgroups per CU, each split at
+// least 512 deep.  1 = no split.
+int splitk_parts(int64_t M, int64_t N, int64_t K) {
+  const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64);
+  const int64_t want = 2 * (int64_t)mg::cu_count();
+  if (tiles >= want || K < 1024) return 1;
+  int64_t S = (want + tiles - 1) / tiles;
+  S = std::min<int64_t>(S, std::min<int64_t>(K / 512, 16));
+  return (int)std::max<int64_t>(S, 1);
+}
+
 }  // namespace
 
 namespace mg {
@@ -117,6 +157,45 @@ int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
   return dispatch<T, true>(M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, s);
 }
 
+template <typename T>
+int64_t cgemm_work_elems(int64_t M, int64_t N, int64_t K) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  const int S = splitk_parts(M, N, K);
+  return S > 1 ? (int64_t)S * M * N : 0;
+}
+
+// As cgemm, with K split over the batch dimension into a workspace of
+// cgemm_work_elems(M, N, K) elements (no split when work is too small).
+template <typename T>
+int cgemm_ws(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+             cd beta, void* C, int64_t ldc, void* work, int64_t work_elems, hipStream_t s) {
+  if (M < 0 || N < 0 || K < 0) return MG_ERR_ARG;
+  if (M == 0 || N == 0) return MG_OK;
+  const int S = splitk_parts(M, N, K);
+  if (S <= 1 || work == nullptr || work_elems < (int64_t)S * M * N)
+    return cgemm<T>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  typedef typename Cplx<T>::t T2;
+  const int64_t Ks = ((K + S - 1) / S + 15) / 16 * 16;   // split depth, a multiple of the k-tile
+  const int full = (int)std::min<int64_t>(S, K / Ks);     // splits of depth Ks
+  const int64_t rest = K - full * Ks;                      // depth of the one shorter split (may be 0)
+  const int parts = full + (rest > 0 ? 1 : 0);
+  T2* W = reinterpret_cast<T2*>(work);
+  const cd one{1, 0}, zero{0, 0};
+  int st = dispatch<T, false>(M, N, Ks, one, A, lda, Ks, B, ldb, Ks * ldb, zero, W, N, M * N, full, s);
+  if (st != MG_OK) return st;
+  if (rest > 0) {
+    st = dispatch<T, false>(M, N, rest, one, reinterpret_cast<const T2*>(A) + full * Ks, lda, 0,
+                            reinterpret_cast<const T2*>(B) + full * Ks * ldb, ldb, 0, zero, W + (int64_t)full * M * N,
+                            N, 0, 1, s);
+    if (st != MG_OK) return st;
+  }
+  const int64_t total = M * N;
+  hipLaunchKernelGGL((k_splitk_reduce<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, M, N, parts, W,
+                     to_t2<T>(alpha), to_t2<T>(beta), reinterpret_cast<T2*>(C), ldc);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
 #define MG_INST(T)                                                                                                 \
   template int cgemm_batched<T>(int64_t, int64_t, int64_t, cd, const void*, int64_t, int64_t, const void*,         \
                                 int64_t, int64_t, cd, void*, int64_t, int64_t, int64_t, hipStream_t);              \
@@ -126,6 +205,12 @@ int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
                               void*, int64_t, hipStream_t);
 MG_INST(double)
 MG_INST(float)
+template int64_t cgemm_work_elems<double>(int64_t, int64_t, int64_t);
+template int64_t cgemm_work_elems<float>(int64_t, int64_t, int64_t);
+template int cgemm_ws<double>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,
+                              int64_t, void*, int64_t, hipStream_t);
+template int cgemm_ws<float>(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*,
+                             int64_t, void*, int64_t, hipStream_t);
 #undef MG_INST
 
 }  // namespace mg
@@ -141,5 +226,25 @@ extern "C" int mg_cgemm(int prec, int64_t m, int64_t n, int64_t k, double alpha_
     case MG_BF16X3: return mg::cgemm_bf16x3(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
     case MG_BF16X6: return mg::cgemm_bf16x6(m, n, k, al, A, lda, B, ldb, be, C, ldc, s);
     default: return MG_ERR_ARG;
+  }
+}
+
+extern "C" int64_t mg_cgemm_work_elems(int prec, int64_t m, int64_t n, int64_t k) {
+  switch (prec) {
+    case MG_C128: return mg::cgemm_work_elems<double>(m, n, k);
+    case MG_C64: return mg::cgemm_work_elems<float>(m, n, k);
+    default: return 0;
+  }
+}
+
+extern "C" int mg_cgemm_ws(int prec, int64_t m, int64_t n, int64_t k, double alpha_re, double alpha_im,
+                           const void* A, int64_t lda, const void* B, int64_t ldb, double beta_re, double beta_im,
+                           void* C, int64_t ldc, void* work, int64_t work_elems, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  mg::cd al{alpha_re, alpha_im}, be{beta_re, beta_im};
+  switch (prec) {
+    case MG_C128: return mg::cgemm_ws<double>(m, n, k, al, A, lda, B, ldb, be, C, ldc, work, work_elems, s);
+    case MG_C64: return mg::cgemm_ws<float>(m, n, k, al, A, lda, B, ldb, be, C, ldc, work, work_elems, s);
+    default: return mg_cgemm(prec, m, n, k, alpha_re, alpha_im, A, lda, B, ldb, beta_re, beta_im, C, ldc, stream);
   }
 }

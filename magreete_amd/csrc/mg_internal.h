@@ -38,6 +38,16 @@ int cgemm_batched(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int6
                   int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch,
                   hipStream_t s);
 
+// Workspace (elements of T2) with which cgemm_ws splits K for a tall-skinny
+// product whose tile grid would leave CUs idle; 0 when it does not split.
+template <typename T>
+int64_t cgemm_work_elems(int64_t M, int64_t N, int64_t K);
+// cgemm with that split: partial products into work, then a fixed-order
+// reduction into C.  Without enough workspace it is cgemm.
+template <typename T>
+int cgemm_ws(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+             cd beta, void* C, int64_t ldc, void* work, int64_t work_elems, hipStream_t s);
+
 // The same GEMM launched under a separate kernel symbol (LU trailing update).
 template <typename T>
 int cgemm_trail(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, const void* B, int64_t ldb,

@@ -242,14 +242,23 @@ def check_barriers(device):
         L.check(lib.mg_check(handle(device), _stream(device)), "mg_check")
 
 
-def cgemm(A, B, C, alpha=1.0, beta=0.0, prec="c128"):
-    """C <- alpha A B + beta C (row-major)."""
+def cgemm(A, B, C, alpha=1.0, beta=0.0, prec="c128", split=False):
+    """C <- alpha A B + beta C (row-major).  split: a tall-skinny product may
+    split K over a workspace (torch's caching allocator) so that its tile
+    grid fills the CUs; the partial sums are reduced in a fixed order."""
     lib = L.load()
     dev = C.device
     alpha, beta = complex(alpha), complex(beta)
+    m, n, k = C.shape[0], C.shape[1], A.shape[1]
     with torch.cuda.device(dev):
-        st = lib.mg_cgemm(PRECS[prec], C.shape[0], C.shape[1], A.shape[1], alpha.real, alpha.imag, _p(A), _ld(A),
-                          _p(B), _ld(B), beta.real, beta.imag, _p(C), _ld(C), _stream(dev))
+        if split:
+            elems = lib.mg_cgemm_work_elems(PRECS[prec], m, n, k)
+            work = torch.empty(max(int(elems), 1), dtype=C.dtype, device=dev)
+            st = lib.mg_cgemm_ws(PRECS[prec], m, n, k, alpha.real, alpha.imag, _p(A), _ld(A), _p(B), _ld(B),
+                                 beta.real, beta.imag, _p(C), _ld(C), _p(work), int(elems), _stream(dev))
+        else:
+            st = lib.mg_cgemm(PRECS[prec], m, n, k, alpha.real, alpha.imag, _p(A), _ld(A), _p(B), _ld(B),
+                              beta.real, beta.imag, _p(C), _ld(C), _stream(dev))
     L.check(st, "mg_cgemm")
     return C
 

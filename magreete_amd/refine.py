@@ -75,7 +75,7 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
         for j in range(m):
             precond(V[j], Z[j])
             w = V[j + 1]
-            E.cgemm(M, Z[j], w, alpha=1.0, beta=0.0)          # w = A z_j (FP64)
+            E.cgemm(M, Z[j], w, alpha=1.0, beta=0.0, split=True)    # w = A z_j (FP64)
             iters += 1
             for _ in range(2):                                # classical Gram-Schmidt, twice
                 h = E.mdot(V[: j + 1], w, work=work)
@@ -173,7 +173,7 @@ class MixedSolver:
         X.copy_(B)
         self.F.solve(X)
         R = B.clone()
-        E.cgemm(self.M, X, R, alpha=-1.0, beta=1.0)
+        E.cgemm(self.M, X, R, alpha=-1.0, beta=1.0, split=True)
         rel = (torch.linalg.norm(R, dim=0) / bnorm).max().item()
         return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel}
 

@@ -133,7 +133,10 @@ def transient_bytes(n, T, n_meas, method, restart=12):
     rhs = 16 * n * T * 4 + 16 * n_meas * T * 2
     if method == "fp64":
         return rhs + 8 * 256 * n * 4
-    return rhs + 16 * n * T * (2 * restart + 2) + 8 * n * T * 2 + 8 * 256 * n * 4
+    # split-K partials of the FP64 Krylov products: at most two workgroups'
+    # worth of 128 x 64 tiles per CU (1024 on 256 CUs), or 16 splits
+    splitk = 16 * min(16 * n * T, 1024 * 128 * 64)
+    return rhs + 16 * n * T * (2 * restart + 2) + 8 * n * T * 2 + 8 * 256 * n * 4 + splitk
 
 
 class _Entry:

@@ -40,6 +40,43 @@ def test_cgemm(prec, mnk):
     assert err < {"c128": 1e-14, "c64": 2e-6, "bf16x3": 5e-5, "bf16x6": 2e-6}[prec], err
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("mnk", [(3000, 40, 4096), (2000, 70, 1500), (12288 // 4, 256, 3072)])
+def test_cgemm_split_k(prec, mnk):
+    """Split-K tall-skinny products: partials over K in a workspace, summed in
+    a fixed order; the same result as the unsplit GEMM to rounding, and
+    repeatable bit for bit."""
+    m, n, k = mnk
+    say("cgemm split-K", prec, mnk)
+    lib = L.load()
+    elems = lib.mg_cgemm_work_elems(E.PRECS[prec], m, n, k)
+    assert elems >= 2 * m * n                     # these shapes split
+    assert lib.mg_cgemm_work_elems(E.PRECS[prec], 128 * 40, 64 * 40, k) == 0
+    rng = np.random.default_rng(m + n + k)
+    dt = np.complex128 if prec == "c128" else np.complex64
+    # A as a strided view (lda > k), C with ldc > n
+    Abig = torch.from_numpy(crand((m, k + 3), rng, dt)).to(DEV)
+    A = Abig[:, 1:k + 1]
+    B = torch.from_numpy(crand((k, n), rng, dt)).to(DEV)
+    Cbig = torch.from_numpy(crand((m, n + 5), rng, dt)).to(DEV)
+    alpha, beta = -1.0 + 0.25j, 0.5 - 1.0j
+    C1 = Cbig.clone()
+    E.cgemm(A, B, C1[:, 2:n + 2], alpha, beta, prec, split=True)
+    C2 = Cbig.clone()
+    E.cgemm(A, B, C2[:, 2:n + 2], alpha, beta, prec, split=False)
+    C3 = Cbig.clone()
+    E.cgemm(A, B, C3[:, 2:n + 2], alpha, beta, prec, split=True)
+    assert torch.equal(C1, C3)                    # deterministic
+    assert torch.equal(C1[:, :2], Cbig[:, :2]) and torch.equal(C1[:, n + 2:], Cbig[:, n + 2:])
+    ref = alpha * (A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128)) \
+        + beta * Cbig[:, 2:n + 2].cpu().numpy().astype(np.complex128)
+    # FP32 accumulation over K terms: rounding grows like sqrt(K)
+    tol = 1e-14 if prec == "c128" else 1e-7 * np.sqrt(k) * 4
+    for C in (C1, C2):
+        got = C[:, 2:n + 2].cpu().numpy().astype(np.complex128)
+        assert np.abs(got - ref).max() / np.abs(ref).max() < tol
+
+
 def test_cgemm_strided_views():
     say("cgemm strided")
     rng = np.random.default_rng(7)
