@@ -167,9 +167,9 @@ int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda
   if (!h || !LDU || !dinv || !B || !work) return MG_ERR_ARG;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (prec) {
-    case MG_C64: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, nullptr);
-    case MG_BF16X3: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3);
-    case MG_BF16X6: return mg::getrs_ldu(LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x6);
+    case MG_C64: return mg::getrs_ldu(&h->ws, LDU, n, lda, dinv, B, ldb, nrhs, work, s, nullptr, 0);
+    case MG_BF16X3: return mg::getrs_ldu(&h->ws, LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x3, 2);
+    case MG_BF16X6: return mg::getrs_ldu(&h->ws, LDU, n, lda, dinv, B, ldb, nrhs, work, s, &mg::cgemm_bf16x6, 3);
     default: return MG_ERR_ARG;
   }
 }

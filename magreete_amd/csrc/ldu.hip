@@ -10,9 +10,11 @@
 //                       products are FP32;
 //   Uh_k = S_k^{-1} A12, Lh_k = A21 S_k^{-1}     stored over A12 and A21;
 //   A22 -= A21 S_k^{-1} A12                      the trailing update.
-// A solve is then GEMMs only: one per block column with Lh (forward), one
-// batched product with the S_k^{-1}, one per block column with Uh
-// (backward); no triangular kernels and no per-column dependency chains.
+// A solve is then GEMMs only: the forward sweep with Lh, one batched
+// product with the S_k^{-1}, the backward sweep with Uh; no triangular
+// kernels and no per-column dependency chains.  Each sweep is either one
+// GEMM launch per block column or, with split-bf16 products, one dataflow
+// launch (k_ldu_sweep).
 // Look-ahead: once step k has updated block row and column k+1, step k+1's
 // inverse and panels run on the workspace's second stream while the rest of
 // step k's trailing update runs on the caller's stream.
@@ -22,6 +24,7 @@
 #include "bf16x3.h"
 #include "lu.h"
 #include "mg_internal.h"
+#include "split_gemm.h"
 
 namespace {
 
@@ -191,6 +194,149 @@ __global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __r
 
 __global__ void k_zero_info(int* p) { *p = 0; }
 
+__global__ void k_zero_ints(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
+
+// One block-triangular sweep of the LDU solve over the right-hand sides X
+// (n x nrhs, row-major, ldx), in one launch:
+//   forward (FWD):  x_i = b_i - sum_{j < i} Lh_ij x_j,   i = 0 .. nblk-1,
+//   backward:       x_i = w_i - sum_{j > i} Uh_ij x_j,   i = nblk-1 .. 0,
+// Lh_ij / Uh_ij the 256 x 256 blocks of LDU (row-major, lda) and b_i / w_i
+// read from Bin (Bin may be X).  One workgroup per BM x BN tile of X; it
+// starts from -b_i, and for each j in dependency order waits until every
+// tile of block row j is final, then adds Lh_ij x_j (split-bf16 MFMA).
+// ctr[0] hands out tickets in start order and ctr[1 + i] counts the final
+// tiles of block row i; tickets map to block rows in dependency order, so a
+// workgroup only waits on workgroups that started before it and the sweep
+// needs no co-residency.  Final tiles are written with agent-scope stores and
+// read with agent-scope loads (coherent across XCDs); each wave waits for its
+// stores to complete before the barrier that precedes the counter increment.  A wait that exceeds the spin
+// bound fills its tile with NaN and stops waiting, so every workgroup ends.
+template <int NS, int BM, int BN, int WM, int WN, bool FWD>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(const float2* __restrict__ LDU, int64_t lda,
+                                                            const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
+                                                            int64_t n, int64_t nrhs, int nblk, int ct, int* ctr) {
+  typedef mg_sg::Tile<NS, BM, BN, WM, WN> Tile;
+  __shared__ typename Tile::S sm;
+  __shared__ int s_ticket, s_ok;
+  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int tpr = (kIB / BM) * ct;   // tiles per block row
+  const int blk = s_ticket / tpr, w = s_ticket % tpr;
+  const int i = FWD ? blk : nblk - 1 - blk;
+  int* done = ctr + 1;
+  const int64_t i0 = (int64_t)i * kIB + (int64_t)(w / ct) * BM, j0 = (int64_t)(w % ct) * BN;
+  const int64_t M = n - i0, N = nrhs - j0;   // valid extents (M <= 0: a row tile past a short last block)
+  if (M > 0) {
+    Tile tile;
+    tile.init(Bin + i0 * ldbin + j0, ldbin, M, N, make_float2(-1.f, 0.f), true);
+    bool ok = true;
+    const int steps = FWD ? i : nblk - 1 - i;
+    // One k-loop over the k-tiles of all steps; the wait for block row j
+    // comes before the fetch of its first k-tile.
+    auto wait_row = [&](int j) {
+      if (threadIdx.x == 0) {
+        long long spins = 0;
+        int good = 1;
+        while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tpr) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > kSweepSpin) {
+            good = 0;
+            break;
+          }
+        }
+        s_ok = good;
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      return s_ok != 0;
+    };
+    const float2* Aj = nullptr;
+    const float2* Xj = nullptr;
+    int64_t kj = 0, KT = 0;
+    auto start_step = [&](int st) {
+      const int j = FWD ? st : nblk - 1 - st;
+      kj = (n - (int64_t)j * kIB < kIB) ? n - (int64_t)j * kIB : kIB;
+      KT = (kj + mg_sg::BK - 1) / mg_sg::BK;
+      Aj = LDU + i0 * lda + (int64_t)j * kIB;
+      Xj = X + (int64_t)j * kIB * ldx + j0;
+      return wait_row(j);
+    };
+    int st = 0;
+    int64_t kt = 0;
+    if (steps > 0) {
+      ok = start_step(0);
+      if (ok) tile.template fetch<true>(Aj, lda, Xj, ldx, M, N, kj, 0);
+    }
+    while (ok && st < steps) {
+      __syncthreads();   // previous k-tile's LDS reads are done
+      tile.stash(sm);
+      __syncthreads();
+      if (++kt == KT) {
+        kt = 0;
+        if (++st < steps) ok = start_step(st);
+      }
+      if (ok && st < steps) tile.template fetch<true>(Aj, lda, Xj, ldx, M, N, kj, kt * mg_sg::BK);
+      tile.compute(sm);
+    }
+    if (!ok) {
+      const float nan = __builtin_nanf("");
+#pragma unroll
+      for (int a = 0; a < Tile::TM; ++a)
+#pragma unroll
+        for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) tile.cr[a][b][r] = tile.ci[a][b][r] = nan;
+    }
+    tile.template store<true>(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+  }
+  // every wave's tile stores complete before the barrier after which the
+  // row counter is bumped (a workgroup fence emits no such wait on gfx950)
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[i], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sweep launch: counters zeroed by a kernel (no memset node), grid = block
+// rows x tiles per block row.
+template <int NS, int BM, int BN, int WM, int WN, bool FWD>
+int launch_sweep(int* ctr, const float2* LDU, int64_t lda, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
+                 int64_t n, int64_t nrhs, hipStream_t s) {
+  const int nblk = (int)((n + kIB - 1) / kIB);
+  const int ct = (int)((nrhs + BN - 1) / BN);
+  const int64_t grid = (int64_t)nblk * (kIB / BM) * ct;
+  if (grid > 0x7fffffffLL) return MG_ERR_ARG;
+  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((nblk + 1 + 255) / 256)), dim3(256), 0, s, ctr, nblk + 1);
+  hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, LDU,
+                     lda, Bin, ldbin, X, ldx, n, nrhs, nblk, ct, ctr);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+template <int NS>
+int ldu_sweeps(int* ctr, const float2* LDU, int64_t lda, const float2* D, float2* B, int64_t ldb, int64_t nrhs,
+               float2* W, int64_t n, hipStream_t s) {
+  // BF16x3: 64 x 128 tiles; BF16x6: 64 x 64 (the 64 x 128 tile's LDS would
+  // leave one workgroup per CU)
+  constexpr int BN = NS == 2 ? 128 : 64;
+  const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
+  MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, LDU, lda, B, ldb, B, ldb, n, nrhs, s)));
+  if (full > 0)
+    MG_TRY(mg::cgemm_batched<float>(nb, nrhs, nb, mg::cd{1, 0}, D, nb, nb * nb, B, ldb, nb * ldb, mg::cd{0, 0}, W,
+                                    nrhs, nb * nrhs, full, s));
+  if (full < nblk) {
+    const int64_t r0 = full * nb, h = n - r0;
+    MG_TRY(mg::cgemm<float>(h, nrhs, h, mg::cd{1, 0}, D + full * nb * nb, nb, B + r0 * ldb, ldb, mg::cd{0, 0},
+                            W + r0 * nrhs, nrhs, s));
+  }
+  return launch_sweep<NS, 64, BN, 2, 2, false>(ctr, LDU, lda, W, nrhs, B, ldb, n, nrhs, s);
+}
+
 // R[i][j] = 2 delta_ij for i, j < h (row stride kIB)
 __global__ void k_two_eye(float2* R, int h) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -281,8 +427,8 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   return MG_OK;
 }
 
-int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb, int64_t nrhs,
-              void* work, hipStream_t s, GemmFn gemm) {
+int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb,
+              int64_t nrhs, void* work, hipStream_t s, GemmFn gemm, int sweep_ns) {
   if (n < 0 || nrhs < 0 || lda < n || ldb < nrhs) return MG_ERR_ARG;
   if (n == 0 || nrhs == 0) return MG_OK;
   const GemmFn g = gemm ? gemm : &cgemm<float>;
@@ -291,6 +437,20 @@ int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B
   float2* Bp = reinterpret_cast<float2*>(B);
   float2* W = reinterpret_cast<float2*>(work);
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
+  if (ws && (sweep_ns == 2 || sweep_ns == 3)) {
+    if (ws->sweep_cap < nblk + 1) {
+      // grows rarely (a larger system than any before on this handle)
+      MG_HIP(hipStreamSynchronize(s));
+      if (ws->sweep) MG_HIP(hipFree(ws->sweep));
+      ws->sweep = nullptr;
+      ws->sweep_cap = 0;
+      const int64_t cap = ((nblk + 1 + 255) / 256) * 256;
+      MG_HIP(hipMalloc(&ws->sweep, sizeof(int) * cap));
+      ws->sweep_cap = cap;
+    }
+    if (sweep_ns == 2) return ldu_sweeps<2>(ws->sweep, Ap, lda, D, Bp, ldb, nrhs, W, n, s);
+    return ldu_sweeps<3>(ws->sweep, Ap, lda, D, Bp, ldb, nrhs, W, n, s);
+  }
   // forward: B[r1:] -= Lh[r1:, block k] B[block k]
   for (int64_t k = 0; k + 1 < nblk; ++k) {
     const int64_t r0 = k * nb, r1 = r0 + nb;

@@ -39,6 +39,8 @@ struct Workspace {
   UpdateProfile prof;
   hipStream_t s2 = nullptr;        // look-ahead panel stream (default priority)
   hipEvent_t evA = nullptr, evB = nullptr;
+  int* sweep = nullptr;            // ticket + per-block-row counters of the LDU solve sweeps
+  int64_t sweep_cap = 0;           // ints allocated at sweep
 };
 
 typedef int (*GemmFn)(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*, int64_t,
@@ -92,8 +94,11 @@ int64_t ldu_dinv_elems(int64_t n);
 int64_t ldu_work_elems(int64_t n);
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
               GemmFn gemm, GemmFn update, bool newton);
-int getrs_ldu(const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb, int64_t nrhs,
-              void* work, hipStream_t s, GemmFn gemm);
+// sweep_ns: 2 / 3 run the forward and backward solves each as one dataflow
+// launch with BF16x3 / BF16x6 products (counters in ws); 0 runs one GEMM
+// launch per block column (gemm, nullptr: the FP32 MFMA GEMM).
+int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb,
+              int64_t nrhs, void* work, hipStream_t s, GemmFn gemm, int sweep_ns);
 
 // gemm: GEMM of the block solves (nullptr: the FP64/FP32 MFMA GEMM of T).
 template <typename T>

@@ -81,14 +81,17 @@ template <typename T2>
 __device__ inline T2 pub_load2(const T2* p) { T2 v; v.x = pub_load(&p->x); v.y = pub_load(&p->y); return v; }
 
 // Grid barrier number `it` over all G workgroups of a cooperative launch.
-// The workgroup fences wait for this wave's exchange stores to complete
-// before the arrival is counted.  With more than 64 workgroups, arrivals go
+// Every wave waits for its outstanding memory operations (the agent-scope
+// exchange stores) to complete before the workgroup barrier after which the
+// arrival is counted: a workgroup-scope fence alone emits no such wait on
+// gfx950, and the stores could otherwise become visible after the arrival.  With more than 64 workgroups, arrivals go
 // to 8 group counters (g % 8) and the last arriver of a group bumps the top
 // counter, which keeps contention on one address bounded.  All counters
 // start at 0.  A wait that exceeds the spin limit, or sees another
 // workgroup's failure flag, sets the flag and returns, so every workgroup
 // always reaches the end of the kernel.
 __device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err) {
+  __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -592,6 +595,7 @@ void workspace_free(Workspace& ws) {
   (void)hipFree(ws.getf2.rowj);
   (void)hipFree(ws.getf2.bar);
   (void)hipFree(ws.getf2.err);
+  if (ws.sweep) (void)hipFree(ws.sweep);
   if (ws.s2) (void)hipStreamDestroy(ws.s2);
   if (ws.evA) (void)hipEventDestroy(ws.evA);
   if (ws.evB) (void)hipEventDestroy(ws.evB);
