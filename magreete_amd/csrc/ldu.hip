@@ -4,7 +4,8 @@
 // With 256 x 256 diagonal blocks, A = Lh D Uh, Lh (Uh) block unit lower
 // (upper) triangular and D = blockdiag(S_k), S_k the k-th diagonal block of
 // the successive Schur complements.  Step k of the factorisation:
-//   S_k^{-1}            one workgroup, in-place Gauss-Jordan with 32-column
+//   S_k^{-1}            eight workgroups (one 32-column slab each, held in
+//                       registers), in-place Gauss-Jordan with 32-column
 //                       panels (BF16x3 MFMA rank-32 updates, FP32 pivots),
 //                       plus one FP32 Newton-Schulz step when the other
 //                       products are FP32;
@@ -31,15 +32,7 @@ namespace {
 constexpr int kIB = 256;          // diagonal block order (= kInvBlock)
 constexpr int kPW = 32;           // Gauss-Jordan panel width
 constexpr int kPK = kPW + 8;      // padded k stride of the bf16 planes
-constexpr int kInvThreads = 512;  // 8 waves; wave w owns rows 32w .. 32w+31
-constexpr int kGrp = 2;           // column tiles per load group of the rank update
-static_assert(2 * kInvThreads == kPW * kPW && kIB == 8 * 32, "thread layout of k_block_inverse");
-
-struct alignas(16) InvSmem {
-  unsigned short r[4][kIB][kPK];   // panel rows as MFMA B operand: [plane][column][k]
-  unsigned short pa[4][kPW][kPK];  // inverse pivot block as MFMA A operand: [plane][row][k]
-  float2 p[2][kPW][kPW + 1];       // pivot block, double-buffered Gauss-Jordan steps
-};
+static_assert(kIB == 8 * kPW, "slab layout of k_block_inverse");
 
 __device__ inline float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline float2 crecipf(float2 a) {
@@ -53,143 +46,227 @@ __device__ inline void split4(float2 v, unsigned short& rh, unsigned short& rl, 
   mg_split(v.y, ih, il);
 }
 
-// X (256 x 256, row stride 256) <- inverse of the h x h block A padded with
-// the identity.  One workgroup of 512 threads.
-__global__ __launch_bounds__(kInvThreads) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
-                                                               float2* __restrict__ X, int* info, int64_t row0) {
-  __shared__ InvSmem sm;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int li = lane & 15, lg = lane >> 4;
-  for (int e = tid; e < kIB * kIB; e += kInvThreads) {
-    const int i = e >> 8, j = e & (kIB - 1);
-    X[e] = (i < h && j < h) ? A[(int64_t)i * lda + j] : make_float2(i == j ? 1.f : 0.f, 0.f);
+constexpr int kSlabs = kIB / kPW;     // workgroups of the block inverse: one per 32-column slab
+constexpr int kMwThreads = 256;       // 4 waves; wave w holds slab rows 64w .. 64w+63
+constexpr long long kInvSpin = 1LL << 22;
+
+struct alignas(16) InvSmem {
+  unsigned short pa[4][kPW][kPK];     // P^{-1} as MFMA A operand: [plane][row][k]
+  unsigned short xb[4][kPW][kPK];     // the slab's pivot rows as B operand: [plane][column][k]
+  unsigned short rb[4][kPW][kPK];     // R (the slab's new pivot rows) as B operand
+  float2 pf[kPW][kPW + 1];            // P^{-1}
+  float2 rf[kPW][kPW + 1];            // R
+};
+
+__device__ inline void split_planes(float2 v, unsigned short* p0, unsigned short* p1, unsigned short* p2,
+                                    unsigned short* p3) {
+  split4(v, *p0, *p1, *p2, *p3);
+}
+
+// Barrier over the kSlabs workgroups of k_block_inverse (counter bar, target
+// the arrivals so far).  Every wave's stores complete before the arrival; a
+// wait past the spin bound (or another workgroup's) sets err and returns.
+__device__ inline void inv_barrier(unsigned* bar, unsigned target, int* err) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long long spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++spins > kInvSpin) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
   }
   __syncthreads();
-  for (int p = 0; p < kIB / kPW; ++p) {
-    const int c0 = p * kPW;
-    // (1) pivot block and the panel rows as bf16 planes
-    for (int e = tid; e < kPW * kPW; e += kInvThreads) {
-      const int i = e >> 5, j = e & (kPW - 1);
-      sm.p[0][i][j] = X[(c0 + i) * kIB + c0 + j];
-    }
-    for (int e = tid; e < kPW * kIB; e += kInvThreads) {
-      const int k = e >> 8, col = e & (kIB - 1);
-      split4(X[(c0 + k) * kIB + col], sm.r[0][col][k], sm.r[1][col][k], sm.r[2][col][k], sm.r[3][col][k]);
-    }
-    __syncthreads();
-    // (2) Gauss-Jordan inversion of the pivot block (no pivoting), one barrier
-    // per step; thread t updates elements (t/32, t%32) and (t/32 + 16, t%32)
-    {
-      const int i0 = tid >> 5, i1 = i0 + 16, j = tid & (kPW - 1);
-      for (int s = 0; s < kPW; ++s) {
-        const float2(*src)[kPW + 1] = sm.p[s & 1];
-        float2(*dst)[kPW + 1] = sm.p[(s + 1) & 1];
-        const float2 piv = src[s][s], rs = src[s][j];
-        const float2 q0 = src[i0][s], q1 = src[i1][s], a0 = src[i0][j], a1 = src[i1][j];
-        if (tid == 0 && piv.x == 0.f && piv.y == 0.f && c0 + s < h) atomicCAS(info, 0, (int)(row0 + c0 + s + 1));
-        const float rd = __builtin_amdgcn_rcpf(piv.x * piv.x + piv.y * piv.y);
-        const float2 inv = make_float2(piv.x * rd, -piv.y * rd);
-        const float2 u = (j == s) ? inv : cmulf(rs, inv);   // row s of the result
-        const float2 t0 = cmulf(q0, (j == s) ? inv : u), t1 = cmulf(q1, (j == s) ? inv : u);
-        dst[i0][j] = (i0 == s) ? u : (j == s) ? make_float2(-t0.x, -t0.y) : make_float2(a0.x - t0.x, a0.y - t0.y);
-        dst[i1][j] = (i1 == s) ? u : (j == s) ? make_float2(-t1.x, -t1.y) : make_float2(a1.x - t1.x, a1.y - t1.y);
-        __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// X (256 x 256, row stride 256) <- inverse of the h x h block A padded with
+// the identity, by in-place Gauss-Jordan without pivoting over 32-column
+// panels.  kSlabs workgroups; workgroup q keeps columns 32q .. 32q+31 of X
+// in its MFMA accumulators for the whole inversion.  Panel p:
+//   slab p, published by its owner to exch (coherent stores), gives the
+//   pivot block P (rows 32p..) and the column panel C;
+//   wave 0 of every workgroup inverts P in registers (FP32, lane = column,
+//   rows through cross-lane shuffles);
+//   R = P^{-1} X_p (the slab's pivot rows; R = P^{-1} on slab p) becomes the
+//   slab's pivot rows, and every other row gets X_i <- X_i - C_i R (slab p:
+//   X_i <- -C_i P^{-1}) as BF16x3 MFMA products.
+// One barrier per panel; zero pivots are reported in info (first, 1-based).
+__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
+                                                              float2* __restrict__ X, int* info, int64_t row0,
+                                                              float2* exch, unsigned* bar, int* err) {
+  __shared__ InvSmem sm;
+  const int q = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  // slab item r of tile (a, b): row 64 wave + 16 a + 4 lg + r, column 32 q + 16 b + li
+  v4f xr[4][2], xi[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 64 * wave + 16 * a + 4 * lg + r, col = kPW * q + 16 * b + li;
+        float2 v = make_float2(row == col ? 1.f : 0.f, 0.f);
+        if (row < h && col < h) v = A[(int64_t)row * lda + col];
+        xr[a][b][r] = v.x;
+        xi[a][b][r] = v.y;
+      }
+  auto publish = [&](float2* buf) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          mg_sg::coherent_store(&buf[(64 * wave + 16 * a + 4 * lg + r) * kPW + 16 * b + li],
+                                make_float2(xr[a][b][r], xi[a][b][r]));
+  };
+  unsigned arrivals = 0;
+  if (q == 0) publish(exch);
+  inv_barrier(bar, (++arrivals) * kSlabs, err);
+
+  for (int p = 0; p < kSlabs; ++p) {
+    const float2* C = exch + (p & 1) * (kIB * kPW);   // slab p: 256 x 32, row stride 32
+    // (1) P^{-1}: lane (j = lane % 32, half hh) holds P[16 hh + t][j], t = 0..15
+    if (wave == 0) {
+      const int j = lane & 31, hh = lane >> 5;
+      float2 v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) v[t] = mg_sg::coherent_load(&C[(kPW * p + 16 * hh + t) * kPW + j]);
+#pragma unroll
+      for (int st = 0; st < kPW; ++st) {
+        const int hs = st >> 4, ts = st & 15;
+        const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[ts].x), hs * 32 + st));
+        const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[ts].y), hs * 32 + st));
+        if (q == 0 && lane == 0 && px == 0.f && py == 0.f && kPW * p + st < h)
+          atomicCAS(info, 0, (int)(row0 + kPW * p + st + 1));
+        const float rd = __builtin_amdgcn_rcpf(px * px + py * py);
+        const float2 inv = make_float2(px * rd, -py * rd);
+        const float2 rowv = make_float2(__shfl(v[ts].x, hs * 32 + j), __shfl(v[ts].y, hs * 32 + j));
+        const float2 u = (j == st) ? inv : cmulf(rowv, inv);   // row st of the result
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const float2 cv = make_float2(__shfl(v[t].x, hh * 32 + st), __shfl(v[t].y, hh * 32 + st));
+          const float2 tq = cmulf(cv, (j == st) ? inv : u);
+          const int i = 16 * hh + t;
+          v[t] = (i == st) ? u : (j == st) ? make_float2(-tq.x, -tq.y) : make_float2(v[t].x - tq.x, v[t].y - tq.y);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int i = 16 * hh + t;
+        sm.pf[i][j] = v[t];
+        split_planes(v[t], &sm.pa[0][i][j], &sm.pa[1][i][j], &sm.pa[2][i][j], &sm.pa[3][i][j]);
       }
     }
-    // kPW steps (even): the inverse pivot block is in sm.p[0]
-    for (int e = tid; e < kPW * kPW; e += kInvThreads) {
-      const int i = e >> 5, k = e & (kPW - 1);
-      split4(sm.p[0][i][k], sm.pa[0][i][k], sm.pa[1][i][k], sm.pa[2][i][k], sm.pa[3][i][k]);
-    }
-    __syncthreads();
-    // (3) panel rows <- P^{-1} rows; the pivot block's own columns get P^{-1}.
-    // Wave w computes column tiles 2w and 2w+1 and rewrites their planes.
-    {
-      bf16x8 pa[2][4];
+    // the slab's pivot rows (wave pw, row tiles pa0 and pa0 + 1) as B operand
+    const int pw = p >> 1, pa0 = 2 * (p & 1);
+    if (q != p && wave == pw) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) pa[a][q] = mg_ld8(&sm.pa[q][a * 16 + li][lg * 8]);
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = (2 * wave + t) * 16 + li;
-        const bool own = col >= c0 && col < c0 + kPW;
-        const bf16x8 brh = mg_ld8(&sm.r[0][col][lg * 8]), brl = mg_ld8(&sm.r[1][col][lg * 8]);
-        const bf16x8 bih = mg_ld8(&sm.r[2][col][lg * 8]), bil = mg_ld8(&sm.r[3][col][lg * 8]);
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          v4f r = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-          mg_cmac12(pa[a][0], pa[a][1], pa[a][2], pa[a][3], brh, brl, bih, bil, r, im);
-#pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const int row = a * 16 + 4 * lg + rr;
-            const float2 pv = sm.p[0][row][(col - c0) & (kPW - 1)];
-            const float2 v = own ? pv : make_float2(r[rr], im[rr]);
-            X[(c0 + row) * kIB + col] = v;
-            split4(v, sm.r[0][col][row], sm.r[1][col][row], sm.r[2][col][row], sm.r[3][col][row]);
+          for (int r = 0; r < 4; ++r) {
+            const int k = 16 * t + 4 * lg + r, c = 16 * b + li;
+            const int a = pa0 + t;
+            split_planes(make_float2(xr[a][b][r], xi[a][b][r]), &sm.xb[0][c][k], &sm.xb[1][c][k], &sm.xb[2][c][k],
+                         &sm.xb[3][c][k]);
           }
-        }
+    }
+    __syncthreads();
+    // (2) this wave's rows of C (the A operand of the update), loaded after the
+    // barrier so that their registers are not live across the pivot inversion
+    float2 cf[4][8];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) cf[a][e] = mg_sg::coherent_load(&C[(64 * wave + 16 * a + li) * kPW + 8 * lg + e]);
+    // (3) R = P^{-1} X_p (q != p) or P^{-1} (q == p); wave w computes tile (w / 2, w % 2)
+    {
+      const int ti = wave >> 1, tj = wave & 1;
+      float2 rv[4];
+      if (q == p) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rv[r] = sm.pf[16 * ti + 4 * lg + r][16 * tj + li];
+      } else {
+        const int ar = 16 * ti + li, bc = 16 * tj + li;
+        v4f r = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+        mg_cmac12(mg_ld8(&sm.pa[0][ar][lg * 8]), mg_ld8(&sm.pa[1][ar][lg * 8]), mg_ld8(&sm.pa[2][ar][lg * 8]),
+                  mg_ld8(&sm.pa[3][ar][lg * 8]), mg_ld8(&sm.xb[0][bc][lg * 8]), mg_ld8(&sm.xb[1][bc][lg * 8]),
+                  mg_ld8(&sm.xb[2][bc][lg * 8]), mg_ld8(&sm.xb[3][bc][lg * 8]), r, im);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) rv[rr] = make_float2(r[rr], im[rr]);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 16 * ti + 4 * lg + r, c = 16 * tj + li;
+        sm.rf[k][c] = rv[r];
+        split_planes(rv[r], &sm.rb[0][c][k], &sm.rb[1][c][k], &sm.rb[2][c][k], &sm.rb[3][c][k]);
       }
     }
     __syncthreads();
-    // (4) rows outside the panel: X[I][:] = X[I][:] (0 on the pivot columns) - X[I][K] R
-    if (wave != p) {
-      bf16x8 na[2][4];   // -X[I][K] as A operand planes
+    // (4) slab update: pivot rows <- R; others X_i <- X_i - C_i R (slab p: -C_i P^{-1})
+    bf16x8 bp[2][4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int row = wave * 32 + a * 16 + li;
-        unsigned short ph[4][8];
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const float2 v = X[row * kIB + c0 + lg * 8 + e];
-          split4(make_float2(-v.x, -v.y), ph[0][e], ph[1][e], ph[2][e], ph[3][e]);
-        }
+      for (int pl = 0; pl < 4; ++pl) bp[b][pl] = mg_ld8(&sm.rb[pl][16 * b + li][lg * 8]);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) na[a][q] = mg_as_bf16x8(mg_pack8(ph[q]));
-      }
-      // kGrp column tiles per group: their loads are issued together
-#pragma unroll 1
-      for (int jg = 0; jg < kIB / (16 * kGrp); ++jg) {
-        float2 c[kGrp][2][4];
+    for (int a = 0; a < 4; ++a) {
+      if (wave == pw && (a == pa0 || a == pa0 + 1)) {
 #pragma unroll
-        for (int t = 0; t < kGrp; ++t) {
-          const int jt = jg * kGrp + t, col = jt * 16 + li;
-          const bool pivcols = (jt >> 1) == p;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              const int row = wave * 32 + a * 16 + 4 * lg + rr;
-              const float2 v = X[row * kIB + col];   // unconditional: keeps the loads in flight together
-              c[t][a][rr] = pivcols ? make_float2(0.f, 0.f) : v;
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < kGrp; ++t) {
-          const int col = (jg * kGrp + t) * 16 + li;
-          const bf16x8 brh = mg_ld8(&sm.r[0][col][lg * 8]), brl = mg_ld8(&sm.r[1][col][lg * 8]);
-          const bf16x8 bih = mg_ld8(&sm.r[2][col][lg * 8]), bil = mg_ld8(&sm.r[3][col][lg * 8]);
-#pragma unroll
-          for (int a = 0; a < 2; ++a) {
-            v4f r, im;
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              r[rr] = c[t][a][rr].x;
-              im[rr] = c[t][a][rr].y;
-            }
-            mg_cmac12(na[a][0], na[a][1], na[a][2], na[a][3], brh, brl, bih, bil, r, im);
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              const int row = wave * 32 + a * 16 + 4 * lg + rr;
-              X[row * kIB + col] = make_float2(r[rr], im[rr]);
-            }
+          for (int r = 0; r < 4; ++r) {
+            const float2 v = sm.rf[16 * (a - pa0) + 4 * lg + r][16 * b + li];
+            xr[a][b][r] = v.x;
+            xi[a][b][r] = v.y;
           }
+        continue;
+      }
+      unsigned short ph[4][8];   // -C rows of tile a as bf16x3 planes
+#pragma unroll
+      for (int e = 0; e < 8; ++e) split4(make_float2(-cf[a][e].x, -cf[a][e].y), ph[0][e], ph[1][e], ph[2][e], ph[3][e]);
+      bf16x8 na[4];
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) na[pl] = mg_as_bf16x8(mg_pack8(ph[pl]));
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (q == p) {
+          xr[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+          xi[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
         }
+        mg_cmac12(na[0], na[1], na[2], na[3], bp[b][0], bp[b][1], bp[b][2], bp[b][3], xr[a][b], xi[a][b]);
       }
     }
-    __syncthreads();
-    // end of panel p
+    // (5) the next panel's slab, published by its owner
+    if (p + 1 < kSlabs) {
+      if (q == p + 1) publish(exch + ((p + 1) & 1) * (kIB * kPW));
+      inv_barrier(bar, (++arrivals) * kSlabs, err);
+    } else {
+      __syncthreads();
+    }
   }
+  const bool bad = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+  const float nan = __builtin_nanf("");
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 64 * wave + 16 * a + 4 * lg + r, col = kPW * q + 16 * b + li;
+        X[row * kIB + col] = bad ? make_float2(nan, nan) : make_float2(xr[a][b][r], xi[a][b][r]);
+      }
 }
 
 __global__ void k_zero_info(int* p) { *p = 0; }
@@ -355,6 +432,7 @@ int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)k
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
               GemmFn gemm, GemmFn update, bool newton) {
   if (n < 0 || lda < n) return MG_ERR_ARG;
+  MG_TRY(workspace_init(ws));
   hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(1), 0, s, info);
   if (n == 0) return MG_OK;
   static_assert(kIB == kInvBlock, "diagonal block order");
@@ -378,8 +456,9 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
     float2* Dk = D + k * nb * nb;
     const float2* S = Ap + r0 * lda + r0;
-    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(kInvThreads), 0, st, S, lda, (int)h, newton ? X0 : Dk, info,
-                       r0);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, ws.inv_ctl, 64);
+    hipLaunchKernelGGL(k_block_inverse, dim3(kSlabs), dim3(kMwThreads), 0, st, S, lda, (int)h, newton ? X0 : Dk,
+                       info, r0, ws.inv_exch, reinterpret_cast<unsigned*>(ws.inv_ctl), ws.inv_ctl + 32);
     MG_HIP(hipGetLastError());
     if (newton) {
       hipLaunchKernelGGL(k_two_eye, dim3((unsigned)((h * nb + 255) / 256)), dim3(256), 0, st, R, (int)h);

@@ -39,6 +39,8 @@ struct Workspace {
   UpdateProfile prof;
   hipStream_t s2 = nullptr;        // look-ahead panel stream (default priority)
   hipEvent_t evA = nullptr, evB = nullptr;
+  float2* inv_exch = nullptr;      // LDU block inverse: two 256 x 32 published slabs
+  int* inv_ctl = nullptr;          // LDU block inverse: barrier counter [0] and failure flag [32]
   int* sweep = nullptr;            // ticket + per-block-row counters of the LDU solve sweeps
   int64_t sweep_cap = 0;           // ints allocated at sweep
 };

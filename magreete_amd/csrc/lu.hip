@@ -559,6 +559,8 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipMalloc(&ws.getf2.cand_data, sizeof(double2) * 2 * G * kLeaf));
   MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kLeaf));
   MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * kBarWords));
+  MG_HIP(hipMalloc(&ws.inv_exch, sizeof(float2) * 2 * 256 * 32));
+  MG_HIP(hipMalloc(&ws.inv_ctl, sizeof(int) * 64));
   MG_HIP(hipMalloc(&ws.getf2.err, sizeof(int) * 64));
   MG_HIP(hipMemset(ws.getf2.err, 0, sizeof(int) * 64));
   if (ws.coop_blocks > G) ws.coop_blocks = G;
@@ -596,6 +598,8 @@ void workspace_free(Workspace& ws) {
   (void)hipFree(ws.getf2.bar);
   (void)hipFree(ws.getf2.err);
   if (ws.sweep) (void)hipFree(ws.sweep);
+  if (ws.inv_exch) (void)hipFree(ws.inv_exch);
+  if (ws.inv_ctl) (void)hipFree(ws.inv_ctl);
   if (ws.s2) (void)hipStreamDestroy(ws.s2);
   if (ws.evA) (void)hipEventDestroy(ws.evA);
   if (ws.evB) (void)hipEventDestroy(ws.evB);

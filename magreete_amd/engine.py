@@ -202,20 +202,28 @@ class Factor:
             del work          # stream-ordered: the factorisation's last step waits on this stream
         self._work = None
 
-    def solve(self, B):
-        """In place: B <- A^{-1} B (B n x nrhs, the factor's dtype, row-major)."""
+    def solve(self, B, work=None):
+        """In place: B <- A^{-1} B (B n x nrhs, the factor's dtype, row-major).
+        work: n * nrhs elements of the factor's dtype owned by the caller
+        (solves that run at the same time on one factor each bring their
+        own); default: a buffer kept by the factor."""
         lib = L.load()
         dev = self.A.device
-        if self._work is None or self._work.numel() < B.shape[0] * B.shape[1]:
-            self._work = torch.empty(B.shape[0] * B.shape[1], dtype=self.A.dtype, device=dev)
+        need = B.shape[0] * B.shape[1]
+        if work is None:
+            if self._work is None or self._work.numel() < need:
+                self._work = torch.empty(need, dtype=self.A.dtype, device=dev)
+            work = self._work
+        elif work.numel() < need or work.dtype != self.A.dtype:
+            raise ValueError("work needs %d elements of %s" % (need, self.A.dtype))
         with torch.cuda.device(dev):
             if self.pivot:
                 st = lib.mg_getrs_dinv(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A),
-                                       _p(self.ipiv), _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(self._work),
+                                       _p(self.ipiv), _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(work),
                                        _stream(dev))
             else:
                 st = lib.mg_getrs_ldu(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A),
-                                      _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(self._work), _stream(dev))
+                                      _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(work), _stream(dev))
         L.check(st, "mg_getrs")
         return B
 

@@ -38,12 +38,13 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
     dev = M.device
     n, T = B.shape
     lowp = torch.empty((n, T), dtype=torch.complex64, device=dev)
+    fwork = torch.empty(n * T, dtype=F.A.dtype, device=dev)
     m_max = max(1, min(restart, max_iters))
     work = torch.empty(L.load().mg_mdot_work_elems(m_max + 1, T), dtype=torch.complex128, device=dev)
 
     def precond(v, out):
         E.convert(v, lowp)
-        F.solve(lowp)
+        F.solve(lowp, work=fwork)
         E.convert(lowp, out)
         return out
 
