@@ -285,21 +285,27 @@ constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
 // Lh_ij / Uh_ij the 256 x 256 blocks of LDU (row-major, lda) and b_i / w_i
 // read from Bin (Bin may be X).  One workgroup per BM x BN tile of X; it
 // starts from -b_i, and for each j in dependency order waits until every
-// tile of block row j is final, then adds Lh_ij x_j (split-bf16 MFMA).
+// tile of block row j is final, then adds Lh_ij x_j (split-bf16 MFMA).  The
+// workgroup holds KS thread groups of WM x WN waves, each with its own
+// accumulators and LDS planes; group g takes k-tiles g, g + KS, ... of every
+// block, which divides the chain of dependent k-tiles per block by KS, and
+// the groups' sums are added through LDS at the end.
 // ctr[0] hands out tickets in start order and ctr[1 + i] counts the final
 // tiles of block row i; tickets map to block rows in dependency order, so a
 // workgroup only waits on workgroups that started before it and the sweep
 // needs no co-residency.  Final tiles are written with agent-scope stores and
 // read with agent-scope loads (coherent across XCDs); each wave waits for its
-// stores to complete before the barrier that precedes the counter increment.  A wait that exceeds the spin
-// bound fills its tile with NaN and stops waiting, so every workgroup ends.
-template <int NS, int BM, int BN, int WM, int WN, bool FWD>
-__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(const float2* __restrict__ LDU, int64_t lda,
-                                                            const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
-                                                            int64_t n, int64_t nrhs, int nblk, int ct, int* ctr) {
+// stores to complete before the barrier that precedes the counter increment.
+// A wait that exceeds the spin bound fills its tile with NaN and stops
+// waiting, so every workgroup ends.
+template <int NS, int BM, int BN, int WM, int WN, int KS, bool FWD>
+__global__ __launch_bounds__(64 * WM * WN * KS) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(
+    const float2* __restrict__ LDU, int64_t lda, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx, int64_t n,
+    int64_t nrhs, int nblk, int ct, int* ctr) {
   typedef mg_sg::Tile<NS, BM, BN, WM, WN> Tile;
-  __shared__ typename Tile::S sm;
+  __shared__ typename Tile::S sm[KS];
   __shared__ int s_ticket, s_ok;
+  const int grp = threadIdx.x / Tile::NT;
   if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int tpr = (kIB / BM) * ct;   // tiles per block row
@@ -310,11 +316,10 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
   const int64_t M = n - i0, N = nrhs - j0;   // valid extents (M <= 0: a row tile past a short last block)
   if (M > 0) {
     Tile tile;
-    tile.init(Bin + i0 * ldbin + j0, ldbin, M, N, make_float2(-1.f, 0.f), true);
+    // group 0 starts from -b_i, the others from 0
+    tile.init(Bin + i0 * ldbin + j0, ldbin, M, N, make_float2(-1.f, 0.f), grp == 0);
     bool ok = true;
     const int steps = FWD ? i : nblk - 1 - i;
-    // One k-loop over the k-tiles of all steps; the wait for block row j
-    // comes before the fetch of its first k-tile.
     auto wait_row = [&](int j) {
       if (threadIdx.x == 0) {
         long long spins = 0;
@@ -343,33 +348,79 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
       Xj = X + (int64_t)j * kIB * ldx + j0;
       return wait_row(j);
     };
+    // One loop over the k-tiles of all steps, KS at a time (group g: k-tile
+    // kt + g); the wait for block row j comes before its first fetch.  A
+    // group past the block's last k-tile fetches nothing (zeros).
+    auto fetch = [&](int64_t kt) {
+      tile.template fetch<true>(Aj, lda, Xj, ldx, M, N, kj, (kt + grp) * mg_sg::BK);
+    };
     int st = 0;
     int64_t kt = 0;
     if (steps > 0) {
       ok = start_step(0);
-      if (ok) tile.template fetch<true>(Aj, lda, Xj, ldx, M, N, kj, 0);
+      if (ok) fetch(0);
     }
     while (ok && st < steps) {
-      __syncthreads();   // previous k-tile's LDS reads are done
-      tile.stash(sm);
+      __syncthreads();   // previous k-tiles' LDS reads are done
+      tile.stash(sm[grp]);
       __syncthreads();
-      if (++kt == KT) {
+      kt += KS;
+      if (kt >= KT) {
         kt = 0;
         if (++st < steps) ok = start_step(st);
       }
-      if (ok && st < steps) tile.template fetch<true>(Aj, lda, Xj, ldx, M, N, kj, kt * mg_sg::BK);
-      tile.compute(sm);
+      if (ok && st < steps) fetch(kt);
+      tile.compute(sm[grp]);
     }
-    if (!ok) {
-      const float nan = __builtin_nanf("");
+    // group sums into group 0 through LDS, real parts then imaginary parts
+    // (each group's own planes hold its partial sums)
+    if constexpr (KS > 1) {
+      static_assert(sizeof(typename Tile::S) >= sizeof(float) * BM * BN, "the group sums reuse a group's LDS");
 #pragma unroll
-      for (int a = 0; a < Tile::TM; ++a)
+      for (int part = 0; part < 2; ++part) {
+        __syncthreads();
+        if (grp > 0) {
+          float* buf = reinterpret_cast<float*>(&sm[grp]);
 #pragma unroll
-        for (int b = 0; b < Tile::TN; ++b)
+          for (int a = 0; a < Tile::TM; ++a)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) tile.cr[a][b][r] = tile.ci[a][b][r] = nan;
+            for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                buf[Tile::row_of(a, r) * BN + Tile::col_of(b)] = part ? tile.ci[a][b][r] : tile.cr[a][b][r];
+        }
+        __syncthreads();
+        if (grp == 0) {
+#pragma unroll
+          for (int g = 1; g < KS; ++g) {
+            const float* buf = reinterpret_cast<const float*>(&sm[g]);
+#pragma unroll
+            for (int a = 0; a < Tile::TM; ++a)
+#pragma unroll
+              for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  const float v = buf[Tile::row_of(a, r) * BN + Tile::col_of(b)];
+                  if (part) tile.ci[a][b][r] += v;
+                  else tile.cr[a][b][r] += v;
+                }
+          }
+        }
+      }
     }
-    tile.template store<true>(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+    if (grp == 0) {
+      if (!ok) {
+        const float nan = __builtin_nanf("");
+#pragma unroll
+        for (int a = 0; a < Tile::TM; ++a)
+#pragma unroll
+          for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tile.cr[a][b][r] = tile.ci[a][b][r] = nan;
+      }
+      tile.template store<true>(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f),
+                                false);
+    }
   }
   // every wave's tile stores complete before the barrier after which the
   // row counter is bumped (a workgroup fence emits no such wait on gfx950)
@@ -381,7 +432,7 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
 
 // Sweep launch: counters zeroed by a kernel (no memset node), grid = block
 // rows x tiles per block row.
-template <int NS, int BM, int BN, int WM, int WN, bool FWD>
+template <int NS, int BM, int BN, int WM, int WN, int KS, bool FWD>
 int launch_sweep(int* ctr, const float2* LDU, int64_t lda, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
                  int64_t n, int64_t nrhs, hipStream_t s) {
   const int nblk = (int)((n + kIB - 1) / kIB);
@@ -389,8 +440,8 @@ int launch_sweep(int* ctr, const float2* LDU, int64_t lda, const float2* Bin, in
   const int64_t grid = (int64_t)nblk * (kIB / BM) * ct;
   if (grid > 0x7fffffffLL) return MG_ERR_ARG;
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((nblk + 1 + 255) / 256)), dim3(256), 0, s, ctr, nblk + 1);
-  hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, LDU,
-                     lda, Bin, ldbin, X, ldx, n, nrhs, nblk, ct, ctr);
+  hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, KS, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN * KS), 0, s,
+                     LDU, lda, Bin, ldbin, X, ldx, n, nrhs, nblk, ct, ctr);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
@@ -401,8 +452,9 @@ int ldu_sweeps(int* ctr, const float2* LDU, int64_t lda, const float2* D, float2
   // BF16x3: 64 x 128 tiles; BF16x6: 64 x 64 (the 64 x 128 tile's LDS would
   // leave one workgroup per CU)
   constexpr int BN = NS == 2 ? 128 : 64;
+  constexpr int KS = 1;   // 2 measured slower: at twice the LDS only one workgroup fits per CU
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
-  MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, LDU, lda, B, ldb, B, ldb, n, nrhs, s)));
+  MG_TRY((launch_sweep<NS, 64, BN, 2, 2, KS, true>(ctr, LDU, lda, B, ldb, B, ldb, n, nrhs, s)));
   if (full > 0)
     MG_TRY(mg::cgemm_batched<float>(nb, nrhs, nb, mg::cd{1, 0}, D, nb, nb * nb, B, ldb, nb * ldb, mg::cd{0, 0}, W,
                                     nrhs, nb * nrhs, full, s));
@@ -411,7 +463,7 @@ int ldu_sweeps(int* ctr, const float2* LDU, int64_t lda, const float2* D, float2
     MG_TRY(mg::cgemm<float>(h, nrhs, h, mg::cd{1, 0}, D + full * nb * nb, nb, B + r0 * ldb, ldb, mg::cd{0, 0},
                             W + r0 * nrhs, nrhs, s));
   }
-  return launch_sweep<NS, 64, BN, 2, 2, false>(ctr, LDU, lda, W, nrhs, B, ldb, n, nrhs, s);
+  return launch_sweep<NS, 64, BN, 2, 2, KS, false>(ctr, LDU, lda, W, nrhs, B, ldb, n, nrhs, s);
 }
 
 // R[i][j] = 2 delta_ij for i, j < h (row stride kIB)

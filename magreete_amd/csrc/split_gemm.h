@@ -76,13 +76,17 @@ struct Tile {
 
   v4f cr[TM][TN], ci[TM][TN];
 
+  // Thread index within the tile's NT threads (a workgroup may hold several
+  // tiles' thread groups, each NT consecutive threads).
+  __device__ static int ltid() { return (int)threadIdx.x % NT; }
+
   // Tile row / column of accumulator item (a, b, r) of this lane.
   __device__ static int row_of(int a, int r) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = ltid() & 63, wave = ltid() >> 6;
     return (wave / WN) * (BM / WM) + a * 16 + (lane >> 4) * 4 + r;
   }
   __device__ static int col_of(int b) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = ltid() & 63, wave = ltid() >> 6;
     return (wave % WN) * (BN / WN) + b * 16 + (lane & 15);
   }
 
@@ -129,7 +133,7 @@ struct Tile {
   template <bool COHERENT_B = false>
   __device__ void fetch(const float2* __restrict__ A, int64_t lda, const float2* B, int64_t ldb, int64_t M, int64_t N,
                         int64_t K, int64_t k0) {
-    const int tid = threadIdx.x;
+    const int tid = ltid();
     const int ar = tid / (BK / 8), ak = (tid % (BK / 8)) * 8;
     const int bc = tid % BN, bk = (tid / BN) * 8;
     const bool kin = K - k0 >= BK;   // the whole k-tile is inside K
@@ -162,7 +166,7 @@ struct Tile {
 
   // The fetched k-tile, split into bf16 planes, into LDS.
   __device__ void stash(S& s) const {
-    const int tid = threadIdx.x;
+    const int tid = ltid();
     const int ar = tid / (BK / 8), ak = (tid % (BK / 8)) * 8;
     const int bc = tid % BN, bk = (tid / BN) * 8;
     unsigned short t2[2 * NS][8];
@@ -200,7 +204,7 @@ struct Tile {
 
   // acc += the k-tile in LDS.
   __device__ void compute(const S& s) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = ltid() & 63, wave = ltid() >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const int li = lane & 15, lg = lane >> 4;
     bf16x8 afr[TM][NS], afi[TM][NS], afn[TM][NS];
