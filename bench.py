@@ -47,7 +47,9 @@ def main():
     ap.add_argument("--n-points", type=int, default=4096)
     ap.add_argument("--n-dirs", type=int, default=128)
     ap.add_argument("--n-measure", type=int, default=2048)
-    ap.add_argument("--tol", type=float, default=1e-12)
+    ap.add_argument("--tol", type=float, default=1e-9,
+                    help="relative FP64 residual per column at which the refinement stops (the error of every "
+                         "method against the FP64 path is measured and reported beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ensemble", type=int, default=256,

@@ -26,7 +26,7 @@ from . import _lib as L
 from . import engine as E
 
 
-def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
+def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None):
     """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F.
     The first Krylov direction is P^{-1} b, so the first cycle starts with no
     FP64 product.  Every cycle ends with the residual b - M x formed from the
@@ -34,6 +34,8 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
     with the unrotated Hessenberg matrix: it equals the FP64 product b - M x
     up to the rounding of the FP64 products M z_j already taken, at the cost
     of one pass over the basis instead of another product.
+    history: optional list that receives the largest per-column relative
+    residual estimate |g_{j+1}| / ||b|| after every FP64 product.
     Returns (converged, FP64 products, cycles, residual)."""
     dev = M.device
     n, T = B.shape
@@ -102,6 +104,8 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart):
             k = j + 1
             # |g_{j+1}| is the norm of the Arnoldi residual formed below
             est = (g[j + 1].abs() / bnorm).max().item()
+            if history is not None:
+                history.append(est)
             if est <= tol:
                 break
         Hk = H[:k, :k].permute(2, 0, 1).contiguous()
@@ -163,12 +167,14 @@ class MixedSolver:
         bnorm = torch.linalg.norm(B, dim=0)
         bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
         total = 0
+        hist = []
         while self.tier != "fp64":
-            ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart)
+            ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart,
+                                               hist)
             total += iters
             if ok:
                 return {"tier": self.tier, "fallback": self.fell, "iters": total, "cycles": cycles,
-                        "residual": rel}
+                        "residual": rel, "history": hist}
             self._escalate()
         # refinement stalled in low precision: complex128 LU
         X.copy_(B)
