@@ -94,13 +94,18 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
                   void* B, int64_t ldb, int64_t nrhs, void* work, void* stream);
 /* Block LDU factorisation without pivoting, in place (complex float; prec
  * MG_C64, MG_BF16X3 or MG_BF16X6): A = Lh D Uh with 256 x 256 diagonal blocks; dinv
- * (mg_ldu_dinv_elems(n)) receives the inverses of the D blocks, A the scaled
- * off-diagonal blocks; work holds mg_ldu_work_elems(n) elements; info: first
- * exactly-zero pivot or 0.  The preconditioner of the mixed-precision
- * refinement (no counterpart in the reference).  mg_getrs_ldu solves with it
- * by GEMMs only (work: n x nrhs elements). */
+ * (mg_ldu_factor_elems(prec, n) elements, 16-byte aligned) receives the
+ * inverses of the D blocks (the first mg_ldu_dinv_elems(n)) and, for the
+ * split precisions, the off-diagonal panels split into bf16 parts; A receives
+ * the scaled off-diagonal blocks; work holds mg_ldu_work_elems(n) elements;
+ * info: first exactly-zero pivot or 0.  The preconditioner of the
+ * mixed-precision refinement (no counterpart in the reference).
+ * mg_getrs_ldu solves with it by GEMMs only (work:
+ * mg_ldu_solve_work_elems(prec, n, nrhs) elements, 16-byte aligned). */
 int64_t mg_ldu_dinv_elems(int64_t n);
+int64_t mg_ldu_factor_elems(int prec, int64_t n);
 int64_t mg_ldu_work_elems(int64_t n);
+int64_t mg_ldu_solve_work_elems(int prec, int64_t n, int64_t nrhs);
 int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info,
                  void* stream);
 int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B,

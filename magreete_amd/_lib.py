@@ -52,6 +52,8 @@ SIGNATURES = {
     "mg_sources": [_int, _int, _vp, _i64, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _vp],
     "mg_ldu_dinv_elems": [_i64],
     "mg_ldu_work_elems": [_i64],
+    "mg_ldu_factor_elems": [_int, _i64],
+    "mg_ldu_solve_work_elems": [_int, _i64, _i64],
     "mg_getrf_ldu": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "mg_getrs_ldu": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp],
     "mg_mdot_work_elems": [_i64, _i64],
@@ -59,7 +61,8 @@ SIGNATURES = {
     "mg_maxpy": [_vp, _i64, _i64, _int, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _vp],
 }
 _RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64, "mg_cgemm_work_elems": _c.c_int64, "mg_mdot_work_elems": _c.c_int64,
-             "mg_ldu_dinv_elems": _c.c_int64, "mg_ldu_work_elems": _c.c_int64}
+             "mg_ldu_dinv_elems": _c.c_int64, "mg_ldu_work_elems": _c.c_int64,
+             "mg_ldu_factor_elems": _c.c_int64, "mg_ldu_solve_work_elems": _c.c_int64}
 
 _lib = None
 

@@ -145,6 +145,27 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
 int64_t mg_ldu_dinv_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_dinv_elems(n); }
 int64_t mg_ldu_work_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_work_elems(n); }
 
+namespace {
+// bf16 parts per float of a split precision (0: FP32 products); -1: not an LDU precision
+int split_parts(int prec) {
+  switch (prec) {
+    case MG_C64: return 0;
+    case MG_BF16X3: return 2;
+    case MG_BF16X6: return 3;
+    default: return -1;
+  }
+}
+}  // namespace
+
+int64_t mg_ldu_factor_elems(int prec, int64_t n) {
+  const int ns = split_parts(prec);
+  return (n < 0 || ns < 0) ? 0 : mg::ldu_factor_elems(ns, n);
+}
+int64_t mg_ldu_solve_work_elems(int prec, int64_t n, int64_t nrhs) {
+  const int ns = split_parts(prec);
+  return (n < 0 || nrhs < 0 || ns < 0) ? 0 : mg::ldu_solve_work_elems(ns, n, nrhs);
+}
+
 // Block LDU factorisation without pivoting (complex float; MG_C64: FP32 MFMA
 // GEMMs, MG_BF16X3 / MG_BF16X6: split-bf16 GEMMs).  The FP32-level forms
 // refine each diagonal-block inverse by one FP32 Newton-Schulz step.
@@ -153,11 +174,13 @@ int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* 
   if (!h || !A || !dinv || !work || !info) return MG_ERR_ARG;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (prec) {
-    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr, nullptr, true);
+    case MG_C64: return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, nullptr, nullptr, true, 0);
     case MG_BF16X3:
-      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3, &mg::cgemm_bf16x3_trail, false);
+      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x3, &mg::cgemm_bf16x3_trail, false,
+                           2);
     case MG_BF16X6:
-      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x6, &mg::cgemm_bf16x6_trail, true);
+      return mg::getrf_ldu(h->ws, A, n, lda, dinv, work, info, s, &mg::cgemm_bf16x6, &mg::cgemm_bf16x6_trail, true,
+                           3);
     default: return MG_ERR_ARG;
   }
 }

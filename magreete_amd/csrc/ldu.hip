@@ -23,8 +23,10 @@
 // info (first one, 1-based row), and the refinement that uses these factors
 // falls back to a pivoted factorisation when they are not good enough.
 #include "bf16x3.h"
+#include "cgemm.h"
 #include "lu.h"
 #include "mg_internal.h"
+#include "psplit.h"
 #include "split_gemm.h"
 
 namespace {
@@ -276,36 +278,170 @@ __global__ void k_zero_ints(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Split precisions (BF16x3, BF16x6): the factor's panels are stored pre-split
+// (psplit.h), once per factorisation, after the D inverses.  With the matrix
+// padded to np = nblk 256 rows, step k < nblk - 1 has rows(k) = np - (k+1) 256
+// and two stored operands of rows(k) 256 elements each:
+//   SPL(k):  Lh's block column k (rows (k+1) 256 .. np), A-operand units,
+//            32 groups per row (the forward sweep's and the trailing update's
+//            A operand);
+//   SPUr(k): Uh's block row k (256 rows; k = columns (k+1) 256 .. np),
+//            A-operand units, rows(k) / 8 groups per row (the backward
+//            sweep's A operand).
+// Each is 32 U rows(k) units at unit offset 32 U sum_{k' < k} rows(k')
+// (U = 2 NS units per row and group); all of SPL comes first, then SPUr.
+struct SplitLayout {
+  int64_t np, nblk;
+  int U;
+  __host__ __device__ int64_t rows(int64_t k) const { return np - (k + 1) * kIB; }
+  __host__ __device__ int64_t off(int64_t k) const {
+    return (int64_t)(kIB / 8) * U * (k * np - (int64_t)kIB * k * (k + 1) / 2);
+  }
+  __host__ __device__ int64_t total() const { return off(nblk); }
+};
+
+// A-operand units of rows [0, rows_pad) x groups [0, ldg) from A (rows x K,
+// row-major lda); zeros outside rows x K.
+template <int NS>
+__global__ void k_pack_rows(const float2* __restrict__ A, int64_t lda, int64_t rows, int64_t K, int64_t rows_pad,
+                            int64_t ldg, u32x4* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows_pad * ldg) return;
+  const int64_t r = t / ldg, g = t - r * ldg;
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = g * 8 + e;
+    v[e] = (r < rows && k < K) ? A[r * lda + k] : make_float2(0.f, 0.f);
+  }
+  u32x4 u[2 * NS];
+  mg_ps::split8<NS>(v, u);
+#pragma unroll
+  for (int q = 0; q < 2 * NS; ++q) out[t * 2 * NS + q] = u[q];
+}
+
+// B-operand units of columns [0, cols_pad) x groups [0, ldg) from B (K x
+// cols, row-major ldb); zeros outside K x cols.  Consecutive threads take
+// consecutive columns (coalesced reads).
+template <int NS>
+__global__ void k_pack_cols(const float2* __restrict__ B, int64_t ldb, int64_t K, int64_t cols, int64_t cols_pad,
+                            int64_t ldg, u32x4* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cols_pad * ldg) return;
+  const int64_t g = t / cols_pad, c = t - g * cols_pad;
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = g * 8 + e;
+    v[e] = (c < cols && k < K) ? B[k * ldb + c] : make_float2(0.f, 0.f);
+  }
+  u32x4 u[2 * NS];
+  mg_ps::split8<NS>(v, u);
+#pragma unroll
+  for (int q = 0; q < 2 * NS; ++q) out[(c * ldg + g) * 2 * NS + q] = u[q];
+}
+
+template <int NS>
+int pack_rows(const float2* A, int64_t lda, int64_t rows, int64_t K, int64_t rows_pad, int64_t ldg, u32x4* out,
+              hipStream_t s) {
+  const int64_t t = rows_pad * ldg;
+  if (t <= 0) return MG_OK;
+  hipLaunchKernelGGL((k_pack_rows<NS>), dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, A, lda, rows, K, rows_pad,
+                     ldg, out);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+template <int NS>
+int pack_cols(const float2* B, int64_t ldb, int64_t K, int64_t cols, int64_t cols_pad, int64_t ldg, u32x4* out,
+              hipStream_t s) {
+  const int64_t t = cols_pad * ldg;
+  if (t <= 0) return MG_OK;
+  hipLaunchKernelGGL((k_pack_cols<NS>), dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, B, ldb, K, cols, cols_pad,
+                     ldg, out);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+// Trailing update of the split factorisation, C -= A_0 B_0 (+ A_1 B_1) with
+// every A (M rows) and B (N columns) pre-split over one 256-block of k (the
+// panels of one or two steps); C complex float row-major.  The accumulators
+// start from -C (loaded before the k-loop).
+template <int NS, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_ldu_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0,
+                                                             const u32x4* A1, const u32x4* B1, int nsrc,
+                                                             float2* __restrict__ C, int64_t ldc, int nbm, int nbn) {
+  typedef mg_ps::PTile<NS, BM, BN, WM, WN> Tile;
+  constexpr int U = 2 * NS, KG = kIB / 8;
+  __shared__ typename Tile::S sm;
+  int tm, tn;
+  mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
+  const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
+  const u32x4* A[2] = {A0 + i0 * KG * U, A1 + i0 * KG * U};
+  const u32x4* B[2] = {B0 + j0 * KG * U, B1 + j0 * KG * U};
+  Tile t;
+  t.init(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), true);
+  t.pmac_n(sm, A, B, nsrc, KG, M - i0, N - j0, KG);
+  t.store(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+}
+
+template <int NS, int BM, int BN, int WM, int WN>
+int launch_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0, const u32x4* A1, const u32x4* B1, int nsrc,
+                  float2* C, int64_t ldc, hipStream_t s) {
+  const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
+  hipLaunchKernelGGL((k_ldu_update<NS, BM, BN, WM, WN>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N, A0, B0,
+                     A1 ? A1 : A0, B1 ? B1 : B0, nsrc, C, ldc, nbm, nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+// C -= A_0 B_0 (+ A_1 B_1: nsrc = 2).  BF16x3: 64 x 128 tiles (two
+// workgroups per CU; 12032^2 x 256 measured 1.07 ms against 1.21 ms with
+// 128 x 128); BF16x6: 128 x 128 when the grid gives every CU a tile, else
+// 64 x 64 (its 64 x 128 tile's LDS would leave one workgroup per CU).
+template <int NS>
+int ldu_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0, const u32x4* A1, const u32x4* B1, int nsrc,
+               float2* C, int64_t ldc, hipStream_t s) {
+  if (M <= 0 || N <= 0) return MG_OK;
+  if constexpr (NS == 2) {
+    return launch_update<NS, 64, 128, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
+  } else {
+    if (((M + 127) / 128) * ((N + 127) / 128) >= (int64_t)mg::cu_count())
+      return launch_update<NS, 128, 128, 4, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
+    return launch_update<NS, 64, 64, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
+  }
+}
+
 constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
 
 // One block-triangular sweep of the LDU solve over the right-hand sides X
 // (n x nrhs, row-major, ldx), in one launch:
 //   forward (FWD):  x_i = b_i - sum_{j < i} Lh_ij x_j,   i = 0 .. nblk-1,
 //   backward:       x_i = w_i - sum_{j > i} Uh_ij x_j,   i = nblk-1 .. 0,
-// Lh_ij / Uh_ij the 256 x 256 blocks of LDU (row-major, lda) and b_i / w_i
-// read from Bin (Bin may be X).  One workgroup per BM x BN tile of X; it
-// starts from -b_i, and for each j in dependency order waits until every
-// tile of block row j is final, then adds Lh_ij x_j (split-bf16 MFMA).  The
-// workgroup holds KS thread groups of WM x WN waves, each with its own
-// accumulators and LDS planes; group g takes k-tiles g, g + KS, ... of every
-// block, which divides the chain of dependent k-tiles per block by KS, and
-// the groups' sums are added through LDS at the end.
+// with b_i / w_i read from Bin (Bin may be X) and Lh_ij / Uh_ij from the
+// factor's pre-split panels SP (SPL forward, SPUr backward).  One workgroup
+// per BM x BN tile of X; it starts from -b_i and for each j in dependency
+// order waits until every tile of block row j is final, then adds Lh_ij x_j
+// (split-bf16 MFMA).  Finished tiles are published pre-split in XS (B-operand
+// units, one column per right-hand side, np / 8 groups per column; rows past
+// n as zeros) for the later block rows, and as complex float in X.
 // ctr[0] hands out tickets in start order and ctr[1 + i] counts the final
 // tiles of block row i; tickets map to block rows in dependency order, so a
 // workgroup only waits on workgroups that started before it and the sweep
-// needs no co-residency.  Final tiles are written with agent-scope stores and
-// read with agent-scope loads (coherent across XCDs); each wave waits for its
-// stores to complete before the barrier that precedes the counter increment.
-// A wait that exceeds the spin bound fills its tile with NaN and stops
-// waiting, so every workgroup ends.
-template <int NS, int BM, int BN, int WM, int WN, int KS, bool FWD>
-__global__ __launch_bounds__(64 * WM * WN * KS) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(
-    const float2* __restrict__ LDU, int64_t lda, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx, int64_t n,
-    int64_t nrhs, int nblk, int ct, int* ctr) {
-  typedef mg_sg::Tile<NS, BM, BN, WM, WN> Tile;
-  __shared__ typename Tile::S sm[KS];
+// needs no co-residency.  XS is written with agent-scope stores and read with
+// agent-scope loads (coherent across XCDs); each wave waits for its stores to
+// complete before the barrier that precedes the counter increment.  A wait
+// past the spin bound fills the tile with NaN and stops waiting, so every
+// workgroup ends.
+template <int NS, int BM, int BN, int WM, int WN, bool FWD>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(
+    const u32x4* __restrict__ SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx, u32x4* XS,
+    int64_t n, int64_t nrhs, int nblk, int ct, int* ctr) {
+  typedef mg_ps::PTile<NS, BM, BN, WM, WN> Tile;
+  constexpr int U = 2 * NS, KG = kIB / 8, KT = KG / 4;
+  __shared__ typename Tile::S sm;
   __shared__ int s_ticket, s_ok;
-  const int grp = threadIdx.x / Tile::NT;
   if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int tpr = (kIB / BM) * ct;   // tiles per block row
@@ -314,147 +450,128 @@ __global__ __launch_bounds__(64 * WM * WN * KS) __attribute__((amdgpu_waves_per_
   int* done = ctr + 1;
   const int64_t i0 = (int64_t)i * kIB + (int64_t)(w / ct) * BM, j0 = (int64_t)(w % ct) * BN;
   const int64_t M = n - i0, N = nrhs - j0;   // valid extents (M <= 0: a row tile past a short last block)
-  if (M > 0) {
-    Tile tile;
-    // group 0 starts from -b_i, the others from 0
-    tile.init(Bin + i0 * ldbin + j0, ldbin, M, N, make_float2(-1.f, 0.f), grp == 0);
-    bool ok = true;
-    const int steps = FWD ? i : nblk - 1 - i;
-    auto wait_row = [&](int j) {
-      if (threadIdx.x == 0) {
-        long long spins = 0;
-        int good = 1;
-        while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tpr) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > kSweepSpin) {
-            good = 0;
-            break;
-          }
+  const int64_t ldgx = np / 8;
+  const SplitLayout lay{np, nblk, U};
+  Tile tile;
+  tile.init(Bin + i0 * ldbin + j0, ldbin, M, N, make_float2(-1.f, 0.f), M > 0);
+  bool ok = true;
+  const int steps = M > 0 ? (FWD ? i : nblk - 1 - i) : 0;
+  // thread 0 polls block row j's counter (wait: until it is complete)
+  auto poll_row = [&](int j, bool wait) {
+    if (threadIdx.x == 0) {
+      long long spins = 0;
+      int st = 1;
+      while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tpr) {
+        if (!wait) {
+          st = 2;
+          break;
         }
-        s_ok = good;
-      }
-      __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      return s_ok != 0;
-    };
-    const float2* Aj = nullptr;
-    const float2* Xj = nullptr;
-    int64_t kj = 0, KT = 0;
-    auto start_step = [&](int st) {
-      const int j = FWD ? st : nblk - 1 - st;
-      kj = (n - (int64_t)j * kIB < kIB) ? n - (int64_t)j * kIB : kIB;
-      KT = (kj + mg_sg::BK - 1) / mg_sg::BK;
-      Aj = LDU + i0 * lda + (int64_t)j * kIB;
-      Xj = X + (int64_t)j * kIB * ldx + j0;
-      return wait_row(j);
-    };
-    // One loop over the k-tiles of all steps, KS at a time (group g: k-tile
-    // kt + g); the wait for block row j comes before its first fetch.  A
-    // group past the block's last k-tile fetches nothing (zeros).
-    auto fetch = [&](int64_t kt) {
-      tile.template fetch<true>(Aj, lda, Xj, ldx, M, N, kj, (kt + grp) * mg_sg::BK);
-    };
-    int st = 0;
-    int64_t kt = 0;
-    if (steps > 0) {
-      ok = start_step(0);
-      if (ok) fetch(0);
-    }
-    while (ok && st < steps) {
-      __syncthreads();   // previous k-tiles' LDS reads are done
-      tile.stash(sm[grp]);
-      __syncthreads();
-      kt += KS;
-      if (kt >= KT) {
-        kt = 0;
-        if (++st < steps) ok = start_step(st);
-      }
-      if (ok && st < steps) fetch(kt);
-      tile.compute(sm[grp]);
-    }
-    // group sums into group 0 through LDS, real parts then imaginary parts
-    // (each group's own planes hold its partial sums)
-    if constexpr (KS > 1) {
-      static_assert(sizeof(typename Tile::S) >= sizeof(float) * BM * BN, "the group sums reuse a group's LDS");
-#pragma unroll
-      for (int part = 0; part < 2; ++part) {
-        __syncthreads();
-        if (grp > 0) {
-          float* buf = reinterpret_cast<float*>(&sm[grp]);
-#pragma unroll
-          for (int a = 0; a < Tile::TM; ++a)
-#pragma unroll
-            for (int b = 0; b < Tile::TN; ++b)
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-                buf[Tile::row_of(a, r) * BN + Tile::col_of(b)] = part ? tile.ci[a][b][r] : tile.cr[a][b][r];
-        }
-        __syncthreads();
-        if (grp == 0) {
-#pragma unroll
-          for (int g = 1; g < KS; ++g) {
-            const float* buf = reinterpret_cast<const float*>(&sm[g]);
-#pragma unroll
-            for (int a = 0; a < Tile::TM; ++a)
-#pragma unroll
-              for (int b = 0; b < Tile::TN; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                  const float v = buf[Tile::row_of(a, r) * BN + Tile::col_of(b)];
-                  if (part) tile.ci[a][b][r] += v;
-                  else tile.cr[a][b][r] += v;
-                }
-          }
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSweepSpin) {
+          st = 0;
+          break;
         }
       }
+      s_ok = st;
     }
-    if (grp == 0) {
-      if (!ok) {
-        const float nan = __builtin_nanf("");
-#pragma unroll
-        for (int a = 0; a < Tile::TM; ++a)
-#pragma unroll
-          for (int b = 0; b < Tile::TN; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) tile.cr[a][b][r] = tile.ci[a][b][r] = nan;
-      }
-      tile.template store<true>(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f),
-                                false);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return s_ok;   // 1: complete, 0: timed out, 2: not yet (poll only)
+  };
+  const u32x4* Aj = nullptr;
+  const u32x4* Xj = nullptr;
+  int64_t ldga = KG;
+  int jcur = 0;
+  auto set_step = [&](int st) {
+    const int j = FWD ? st : nblk - 1 - st;
+    jcur = j;
+    if constexpr (FWD) {
+      ldga = KG;
+      Aj = SP + lay.off(j) + (i0 - (int64_t)(j + 1) * kIB) * KG * U;
+    } else {
+      ldga = lay.rows(i) / 8;
+      Aj = SP + lay.off(i) + ((i0 - (int64_t)i * kIB) * ldga + (int64_t)(j - i - 1) * KG) * U;
     }
+    Xj = XS + (j0 * ldgx + (int64_t)j * KG) * U;
+  };
+  // One loop over the k-tiles of all steps; the wait for block row j comes
+  // before its first fetch.  When the next block row is not complete yet,
+  // the current k-tile is multiplied before the wait.
+  int st = 0, kt = 0;
+  if (steps > 0) {
+    set_step(0);
+    ok = poll_row(jcur, true) == 1;
+    if (ok) tile.template pfetch<true>(Aj, ldga, Xj, ldgx, M, N, 0);
   }
-  // every wave's tile stores complete before the barrier after which the
-  // row counter is bumped (a workgroup fence emits no such wait on gfx950)
+  while (ok && st < steps) {
+    __syncthreads();   // previous k-tile's LDS reads are done
+    tile.pstash(sm);
+    __syncthreads();
+    bool computed = false;
+    if (++kt == KT) {
+      kt = 0;
+      if (++st < steps) {
+        set_step(st);
+        int r = poll_row(jcur, false);
+        if (r == 2) {
+          tile.compute(sm);
+          computed = true;
+          r = poll_row(jcur, true);
+        }
+        ok = r == 1;
+      }
+    }
+    if (ok && st < steps) tile.template pfetch<true>(Aj, ldga, Xj, ldgx, M, N, (int64_t)kt * 4);
+    if (!computed) tile.compute(sm);
+  }
+  if (!ok) {
+    const float nan = __builtin_nanf("");
+#pragma unroll
+    for (int a = 0; a < Tile::TM; ++a)
+#pragma unroll
+      for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tile.cr[a][b][r] = tile.ci[a][b][r] = nan;
+  }
+  tile.store_split_cols(XS + (j0 * ldgx + i0 / 8) * U, ldgx, M, N, make_float2(-1.f, 0.f));
+  // every wave's stores complete before the barrier after which the row
+  // counter is bumped (a workgroup fence emits no such wait on gfx950)
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[i], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // the complex-float result is read only after the launch: stored after the signal
+  if (M > 0)
+    tile.store(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
 
 // Sweep launch: counters zeroed by a kernel (no memset node), grid = block
 // rows x tiles per block row.
-template <int NS, int BM, int BN, int WM, int WN, int KS, bool FWD>
-int launch_sweep(int* ctr, const float2* LDU, int64_t lda, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
-                 int64_t n, int64_t nrhs, hipStream_t s) {
+template <int NS, int BM, int BN, int WM, int WN, bool FWD>
+int launch_sweep(int* ctr, const u32x4* SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
+                 u32x4* XS, int64_t n, int64_t nrhs, hipStream_t s) {
   const int nblk = (int)((n + kIB - 1) / kIB);
   const int ct = (int)((nrhs + BN - 1) / BN);
   const int64_t grid = (int64_t)nblk * (kIB / BM) * ct;
   if (grid > 0x7fffffffLL) return MG_ERR_ARG;
   hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((nblk + 1 + 255) / 256)), dim3(256), 0, s, ctr, nblk + 1);
-  hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, KS, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN * KS), 0, s,
-                     LDU, lda, Bin, ldbin, X, ldx, n, nrhs, nblk, ct, ctr);
+  hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, SP, np,
+                     Bin, ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
 
+// The LDU solve with pre-split panels: forward sweep, the batched product
+// with the D inverses (FP32 MFMA), backward sweep.  64 x 64 tiles: on the
+// 12288-row, 256-column solve they measured 1.15 ms per sweep against 1.4 ms
+// for 64 x 128 and 1.2-1.5 ms for 32 x 64, 64 x 32 and 32 x 32 (the sweep's
+// time is its chain of 48 dependent block rows, not its MFMA work).
 template <int NS>
-int ldu_sweeps(int* ctr, const float2* LDU, int64_t lda, const float2* D, float2* B, int64_t ldb, int64_t nrhs,
-               float2* W, int64_t n, hipStream_t s) {
-  // BF16x3: 64 x 128 tiles; BF16x6: 64 x 64 (the 64 x 128 tile's LDS would
-  // leave one workgroup per CU)
-  constexpr int BN = NS == 2 ? 128 : 64;
-  constexpr int KS = 1;   // 2 measured slower: at twice the LDS only one workgroup fits per CU
+int ldu_sweeps(int* ctr, const u32x4* SPL, const u32x4* SPUr, int64_t np, const float2* D, float2* B, int64_t ldb,
+               int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
+  constexpr int BN = 64;
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
-  MG_TRY((launch_sweep<NS, 64, BN, 2, 2, KS, true>(ctr, LDU, lda, B, ldb, B, ldb, n, nrhs, s)));
+  MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
   if (full > 0)
     MG_TRY(mg::cgemm_batched<float>(nb, nrhs, nb, mg::cd{1, 0}, D, nb, nb * nb, B, ldb, nb * ldb, mg::cd{0, 0}, W,
                                     nrhs, nb * nrhs, full, s));
@@ -463,7 +580,7 @@ int ldu_sweeps(int* ctr, const float2* LDU, int64_t lda, const float2* D, float2
     MG_TRY(mg::cgemm<float>(h, nrhs, h, mg::cd{1, 0}, D + full * nb * nb, nb, B + r0 * ldb, ldb, mg::cd{0, 0},
                             W + r0 * nrhs, nrhs, s));
   }
-  return launch_sweep<NS, 64, BN, 2, 2, KS, false>(ctr, LDU, lda, W, nrhs, B, ldb, n, nrhs, s);
+  return launch_sweep<NS, 64, BN, 2, 2, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
 }
 
 // R[i][j] = 2 delta_ij for i, j < h (row stride kIB)
@@ -479,10 +596,30 @@ __global__ void k_two_eye(float2* R, int h) {
 namespace mg {
 
 int64_t ldu_dinv_elems(int64_t n) { return ((n + kIB - 1) / kIB) * (int64_t)kIB * kIB; }
-int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)kIB * kIB; }
+
+namespace {
+int64_t padded(int64_t n) { return ((n + kIB - 1) / kIB) * kIB; }
+// complex-float elements (8 bytes) of the pre-split panels of a split precision
+int64_t split_elems(int ns, int64_t n) {
+  if (ns == 0) return 0;
+  const SplitLayout lay{padded(n), (n + kIB - 1) / kIB, 2 * ns};
+  return 2 * lay.total() * 2;   // SPL and SPUr, two float2 per 16-byte unit
+}
+// the double-buffered pre-split copy of A12 per step (widest split, 6 units)
+int64_t a12_split_elems(int64_t n) { return 2 * padded(n) * (kIB / 8) * 6 * 2; }
+}  // namespace
+
+int64_t ldu_factor_elems(int ns, int64_t n) { return ldu_dinv_elems(n) + split_elems(ns, n); }
+int64_t ldu_work_elems(int64_t n) { return 4 * (int64_t)kIB * n + 2 * (int64_t)kIB * kIB + a12_split_elems(n); }
+int64_t ldu_solve_work_elems(int ns, int64_t n, int64_t nrhs) {
+  const int64_t w = n * nrhs;
+  if (ns == 0) return w;
+  // W, then (16-byte aligned) the pre-split right-hand sides: nrhs columns of np / 8 groups
+  return ((w + 1) & ~(int64_t)1) + nrhs * (padded(n) / 8) * (2 * ns) * 2;
+}
 
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
-              GemmFn gemm, GemmFn update, bool newton) {
+              GemmFn gemm, GemmFn update, bool newton, int split_ns) {
   if (n < 0 || lda < n) return MG_ERR_ARG;
   MG_TRY(workspace_init(ws));
   hipLaunchKernelGGL(k_zero_info, dim3(1), dim3(1), 0, s, info);
@@ -504,8 +641,27 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   // buffer (panels run one at a time).
   float2* X0 = W + 4 * nb * n;
   float2* R = X0 + nb * nb;
-  auto panel = [&](int64_t k, hipStream_t st) -> int {
-    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
+  // split precisions: the pre-split panels (factor data after the D
+  // inverses) and the pre-split A12 of the current step (work, after R)
+  const int U = 2 * split_ns;
+  const SplitLayout lay{padded(n), nblk, U};
+  u32x4* SPL = split_ns ? reinterpret_cast<u32x4*>(D + ldu_dinv_elems(n)) : nullptr;
+  u32x4* SPUr = split_ns ? SPL + lay.total() : nullptr;
+  u32x4* A12s = reinterpret_cast<u32x4*>(R + nb * nb);
+  auto spa12 = [&](int64_t k) { return A12s + (k & 1) * lay.np * (kIB / 8) * U; };
+  auto pack_panels = [&](int64_t k, const float2* S12, hipStream_t st) -> int {
+    const int64_t r1 = (k + 1) * nb, r = n - r1, rk = lay.rows(k);
+    if (split_ns == 2) {
+      MG_TRY(pack_rows<2>(Ap + r1 * lda + k * nb, lda, r, nb, rk, kIB / 8, SPL + lay.off(k), st));
+      MG_TRY(pack_rows<2>(Ap + k * nb * lda + r1, lda, nb, r, nb, rk / 8, SPUr + lay.off(k), st));
+      return pack_cols<2>(S12, r, nb, r, rk, kIB / 8, spa12(k), st);
+    }
+    MG_TRY(pack_rows<3>(Ap + r1 * lda + k * nb, lda, r, nb, rk, kIB / 8, SPL + lay.off(k), st));
+    MG_TRY(pack_rows<3>(Ap + k * nb * lda + r1, lda, nb, r, nb, rk / 8, SPUr + lay.off(k), st));
+    return pack_cols<3>(S12, r, nb, r, rk, kIB / 8, spa12(k), st);
+  };
+  auto inverse = [&](int64_t k, hipStream_t st) -> int {
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb;
     float2* Dk = D + k * nb * nb;
     const float2* S = Ap + r0 * lda + r0;
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, ws.inv_ctl, 64);
@@ -518,7 +674,13 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
       MG_TRY(cgemm<float>(h, h, h, cd{-1, 0}, S, lda, X0, nb, cd{1, 0}, R, nb, st));    // R = 2I - S X0
       MG_TRY(cgemm<float>(h, h, h, cd{1, 0}, X0, nb, R, nb, cd{0, 0}, Dk, nb, st));     // D_k = X0 R
     }
+    return MG_OK;
+  };
+  // step k's panels from its inverse (and, split precisions, their pre-split copies)
+  auto panel_rest = [&](int64_t k, hipStream_t st) -> int {
+    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
     if (r == 0) return MG_OK;
+    const float2* Dk = D + k * nb * nb;
     float2 *S12 = s12(k), *S21 = s21(k);
     MG_HIP(hipMemcpy2DAsync(S12, r * sizeof(float2), Ap + r0 * lda + r1, lda * sizeof(float2), r * sizeof(float2),
                             h, hipMemcpyDeviceToDevice, st));
@@ -526,6 +688,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
                             r, hipMemcpyDeviceToDevice, st));
     MG_TRY(g(h, r, h, cd{1, 0}, Dk, nb, S12, r, cd{0, 0}, Ap + r0 * lda + r1, lda, st));   // Uh = S^-1 A12
     MG_TRY(g(r, h, h, cd{1, 0}, S21, h, Dk, nb, cd{0, 0}, Ap + r1 * lda + r0, lda, st));   // Lh = A21 S^-1
+    if (split_ns) MG_TRY(pack_panels(k, S12, st));
     return MG_OK;
   };
   // A22[rows i0.., cols j0..j0+nc] -= A21 S^-1 A12 for step k (A21 from the scratch copy, Uh in place)
@@ -535,26 +698,88 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     return timed_update(ws, gt, mr, nc, h, cd{-1, 0}, s21(k) + (i0 - r1) * h, h, Ap + r0 * lda + j0, lda, cd{1, 0},
                         Ap + i0 * lda + j0, lda, st);
   };
-  MG_TRY(panel(0, s));
+  // split precisions: A22[rows i0.., cols j0..] -= Lh_k A12_k (+ Lh_{k+1} A12_{k+1} when both), pre-split
+  auto trail_ps = [&](int64_t k, bool both, int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) -> int {
+    if (mr <= 0 || nc <= 0) return MG_OK;
+    constexpr int64_t KG = kIB / 8;
+    const u32x4* A[2];
+    const u32x4* B[2];
+    const int nsrc = both ? 2 : 1;
+    for (int q = 0; q < nsrc; ++q) {
+      const int64_t r1 = (k + q + 1) * nb;
+      A[q] = SPL + lay.off(k + q) + (i0 - r1) * KG * U;
+      B[q] = spa12(k + q) + (j0 - r1) * KG * U;
+    }
+    const double flops = 8.0 * (double)mr * (double)nc * (double)(nsrc * nb);
+    return timed_launch(ws, flops, st, [&] {
+      return split_ns == 2 ? ldu_update<2>(mr, nc, A[0], B[0], nsrc > 1 ? A[1] : nullptr, nsrc > 1 ? B[1] : nullptr,
+                                           nsrc, Ap + i0 * lda + j0, lda, st)
+                           : ldu_update<3>(mr, nc, A[0], B[0], nsrc > 1 ? A[1] : nullptr, nsrc > 1 ? B[1] : nullptr,
+                                           nsrc, Ap + i0 * lda + j0, lda, st);
+    });
+  };
+  MG_TRY(inverse(0, s));
+  MG_TRY(panel_rest(0, s));
+  if (!split_ns || !ws.s2) {
+    // Look-ahead of depth one (with a second stream): block row and column
+    // k+1 are updated first, step k+1's inverse and panels run on the second
+    // stream while the rest of step k's trailing update runs here.
+    for (int64_t k = 0; k < nblk; ++k) {
+      const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
+      if (r == 0) break;
+      const int64_t h2 = (r < nb) ? r : nb;   // next block's order
+      auto upd = [&](int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) {
+        return split_ns ? trail_ps(k, false, i0, mr, j0, nc, st) : trail(k, i0, mr, j0, nc, st);
+      };
+      MG_TRY(upd(r1, h2, r1, r, s));
+      MG_TRY(upd(r1 + h2, r - h2, r1, h2, s));
+      if (ws.s2) {
+        MG_HIP(hipEventRecord(ws.evA, s));
+        MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
+        MG_TRY(inverse(k + 1, ws.s2));
+        MG_TRY(panel_rest(k + 1, ws.s2));
+        MG_HIP(hipEventRecord(ws.evB, ws.s2));
+        MG_TRY(upd(r1 + h2, r - h2, r1 + h2, r - h2, s));
+        MG_HIP(hipStreamWaitEvent(s, ws.evB, 0));
+      } else {
+        MG_TRY(upd(r1 + h2, r - h2, r1 + h2, r - h2, s));
+        MG_TRY(inverse(k + 1, s));
+        MG_TRY(panel_rest(k + 1, s));
+      }
+    }
+    return MG_OK;
+  }
+  // Split precisions: the second stream carries the factorisation's
+  // critical path.  Step k updates the next diagonal block S_{k+1} there and
+  // inverts it at once, while the rest of block row and column k+1 is
+  // updated on the caller's stream; step k+1's panels follow on the second
+  // stream once those are final, and the rest of step k's trailing update
+  // runs beside them.  That rest leaves out the block S_{k+2}, which the next
+  // step's diagonal update takes with both steps' panels (so that no two
+  // launches update one block at the same time).  evA: step k's panels done;
+  // evB: block row and column k+1 done.
+  MG_HIP(hipEventRecord(ws.evA, s));
   for (int64_t k = 0; k < nblk; ++k) {
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
     if (r == 0) break;
     const int64_t h2 = (r < nb) ? r : nb;   // next block's order
-    // block row and column k+1 first
-    MG_TRY(trail(k, r1, h2, r1, r, s));
-    MG_TRY(trail(k, r1 + h2, r - h2, r1, h2, s));
-    if (ws.s2) {
-      MG_HIP(hipEventRecord(ws.evA, s));
-      MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
-      MG_TRY(panel(k + 1, ws.s2));
-      MG_HIP(hipEventRecord(ws.evB, ws.s2));
-      MG_TRY(trail(k, r1 + h2, r - h2, r1 + h2, r - h2, s));
-      MG_HIP(hipStreamWaitEvent(s, ws.evB, 0));
-    } else {
-      MG_TRY(trail(k, r1 + h2, r - h2, r1 + h2, r - h2, s));
-      MG_TRY(panel(k + 1, s));
-    }
+    const int64_t r2 = r1 + h2, h3 = (n - r2 < nb) ? n - r2 : nb;   // block k+2
+    MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
+    if (k == 0) MG_TRY(trail_ps(0, false, r1, h2, r1, h2, ws.s2));
+    else MG_TRY(trail_ps(k - 1, true, r1, h2, r1, h2, ws.s2));   // S_{k+1}: steps k-1 and k
+    MG_TRY(inverse(k + 1, ws.s2));
+    MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));
+    MG_TRY(trail_ps(k, false, r1, h2, r2, n - r2, s));
+    MG_TRY(trail_ps(k, false, r2, n - r2, r1, h2, s));
+    MG_HIP(hipEventRecord(ws.evB, s));
+    MG_HIP(hipStreamWaitEvent(ws.s2, ws.evB, 0));
+    MG_TRY(panel_rest(k + 1, ws.s2));
+    MG_HIP(hipEventRecord(ws.evA, ws.s2));
+    // the rest of step k's update, without block S_{k+2}
+    MG_TRY(trail_ps(k, false, r2, h3, r2 + h3, n - r2 - h3, s));
+    MG_TRY(trail_ps(k, false, r2 + h3, n - r2 - h3, r2, n - r2, s));
   }
+  MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));   // the last inverse
   return MG_OK;
 }
 
@@ -568,7 +793,8 @@ int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void
   float2* Bp = reinterpret_cast<float2*>(B);
   float2* W = reinterpret_cast<float2*>(work);
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
-  if (ws && (sweep_ns == 2 || sweep_ns == 3)) {
+  if (sweep_ns == 2 || sweep_ns == 3) {
+    if (!ws) return MG_ERR_ARG;
     if (ws->sweep_cap < nblk + 1) {
       // grows rarely (a larger system than any before on this handle)
       MG_HIP(hipStreamSynchronize(s));
@@ -579,8 +805,12 @@ int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void
       MG_HIP(hipMalloc(&ws->sweep, sizeof(int) * cap));
       ws->sweep_cap = cap;
     }
-    if (sweep_ns == 2) return ldu_sweeps<2>(ws->sweep, Ap, lda, D, Bp, ldb, nrhs, W, n, s);
-    return ldu_sweeps<3>(ws->sweep, Ap, lda, D, Bp, ldb, nrhs, W, n, s);
+    const SplitLayout lay{padded(n), nblk, 2 * sweep_ns};
+    const u32x4* SPL = reinterpret_cast<const u32x4*>(D + ldu_dinv_elems(n));
+    const u32x4* SPUr = SPL + lay.total();
+    u32x4* XS = reinterpret_cast<u32x4*>(W + ((n * nrhs + 1) & ~(int64_t)1));
+    if (sweep_ns == 2) return ldu_sweeps<2>(ws->sweep, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
+    return ldu_sweeps<3>(ws->sweep, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
   }
   // forward: B[r1:] -= Lh[r1:, block k] B[block k]
   for (int64_t k = 0; k + 1 < nblk; ++k) {

@@ -52,6 +52,30 @@ typedef int (*GemmFn)(int64_t, int64_t, int64_t, cd, const void*, int64_t, const
 // (algorithmic flops 8 M N K per launch).
 int timed_update(Workspace& ws, GemmFn fn, int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda,
                  const void* B, int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
+// launch() on s, timed with HIP events on s when ws.prof is enabled (flops:
+// the launch's algorithmic count).
+template <typename F>
+int timed_launch(Workspace& ws, double flops, hipStream_t s, F&& launch) {
+  UpdateProfile& pr = ws.prof;
+  if (pr.enabled) {
+    if (pr.used * 2 + 2 > pr.ev.size()) {
+      hipEvent_t a, b;
+      MG_HIP(hipEventCreate(&a));
+      MG_HIP(hipEventCreate(&b));
+      pr.ev.push_back(a);
+      pr.ev.push_back(b);
+    }
+    MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
+  }
+  MG_TRY(launch());
+  if (pr.enabled) {
+    MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
+    if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
+    pr.flops[pr.used] = flops;
+    ++pr.used;
+  }
+  return MG_OK;
+}
 // Sum of recorded update launches since the last read (synchronises the stream).
 int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms);
 
@@ -92,13 +116,20 @@ int diag_inverses(const void* LU, int64_t n, int64_t lda, void* dinv, hipStream_
 // needs n x nrhs work elements.  gemm: GEMM of the panels and block solves,
 // update: GEMM of the trailing updates (nullptr: the FP32 MFMA GEMMs);
 // newton: refine each block inverse by one FP32 Newton-Schulz step.
+// split_ns (2: BF16x3, 3: BF16x6; 0 for FP32 products): the trailing
+// updates and the solves read the panels pre-split, which the factorisation
+// stores after the D inverses (dinv then holds ldu_factor_elems(split_ns, n)
+// elements) and the solve's work holds ldu_solve_work_elems(split_ns, n, nrhs).
 int64_t ldu_dinv_elems(int64_t n);
+int64_t ldu_factor_elems(int split_ns, int64_t n);
 int64_t ldu_work_elems(int64_t n);
+int64_t ldu_solve_work_elems(int split_ns, int64_t n, int64_t nrhs);
 int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* work, int* info, hipStream_t s,
-              GemmFn gemm, GemmFn update, bool newton);
+              GemmFn gemm, GemmFn update, bool newton, int split_ns);
 // sweep_ns: 2 / 3 run the forward and backward solves each as one dataflow
-// launch with BF16x3 / BF16x6 products (counters in ws); 0 runs one GEMM
-// launch per block column (gemm, nullptr: the FP32 MFMA GEMM).
+// launch with BF16x3 / BF16x6 products over the pre-split panels (counters
+// in ws); 0 runs one GEMM launch per block column (gemm, nullptr: the FP32
+// MFMA GEMM).
 int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb,
               int64_t nrhs, void* work, hipStream_t s, GemmFn gemm, int sweep_ns);
 

@@ -608,25 +608,8 @@ void workspace_free(Workspace& ws) {
 
 int timed_update(Workspace& ws, GemmFn fn, int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda,
                  const void* B, int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s) {
-  UpdateProfile& pr = ws.prof;
-  if (pr.enabled) {
-    if (pr.used * 2 + 2 > pr.ev.size()) {
-      hipEvent_t a, b;
-      MG_HIP(hipEventCreate(&a));
-      MG_HIP(hipEventCreate(&b));
-      pr.ev.push_back(a);
-      pr.ev.push_back(b);
-    }
-    MG_HIP(hipEventRecord(pr.ev[pr.used * 2], s));
-  }
-  MG_TRY(fn(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s));
-  if (pr.enabled) {
-    MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
-    if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
-    pr.flops[pr.used] = 8.0 * (double)M * (double)N * (double)K;
-    ++pr.used;
-  }
-  return MG_OK;
+  return timed_launch(ws, 8.0 * (double)M * (double)N * (double)K, s,
+                      [&] { return fn(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); });
 }
 
 int workspace_leaf_blocks(Workspace& ws, int g) {

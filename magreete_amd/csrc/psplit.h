@@ -1,0 +1,192 @@
+// Pre-split operands for the split-bf16 complex products.
+//
+// The split GEMM tile (split_gemm.h) converts every float operand it loads
+// into NS bf16 parts.  An operand that several products read (the LDU
+// factor's panels, the right-hand sides of a sweep) is cheaper to split once
+// and store split: the products then copy 16-byte units into LDS and spend no
+// VALU work on conversions.
+//
+// Format: for each row r of an A operand (k along the row), or each column c
+// of a B operand (k down the column), and each group g of 8 consecutive k,
+// the 2 NS planes of those 8 elements follow each other as 16-byte units:
+//   unit ((r * ldg + g) * 2 NS + q),  q < NS: real part q, q >= NS: imaginary
+//   part q - NS,
+// each part the bf16 rounding of what the earlier parts leave.  ldg (groups
+// per row or column) is a multiple of 4, so a k-tile of 32 is 4 groups; k
+// past the operand's extent holds zeros.
+#pragma once
+#include "split_gemm.h"
+
+namespace mg_ps {
+
+using mg_sg::BK;
+
+// Eight complex values -> their 2 NS units.
+template <int NS>
+__device__ inline void split8(const float2* v, u32x4* out) {
+  unsigned short t[2 * NS][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    unsigned short p[NS];
+    mg_sg::split_n<NS>(v[e].x, p);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) t[q][e] = p[q];
+    mg_sg::split_n<NS>(v[e].y, p);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) t[NS + q][e] = p[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 2 * NS; ++q) out[q] = mg_pack8(t[q]);
+}
+
+// Agent-scope (cross-XCD coherent) 16-byte load and 8-byte store, for units
+// that other workgroups of the same launch write.
+__device__ inline u32x4 coherent_load16(const u32x4* p) {
+  const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+  const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  u32x4 v;
+  v.x = (unsigned)a;
+  v.y = (unsigned)(a >> 32);
+  v.z = (unsigned)b;
+  v.w = (unsigned)(b >> 32);
+  return v;
+}
+__device__ inline void coherent_store8(void* p, unsigned long long v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The split GEMM tile fed from pre-split operands (the accumulators, compute
+// and store are the split tile's).
+template <int NS, int BM, int BN, int WM, int WN>
+struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
+  typedef mg_sg::Tile<NS, BM, BN, WM, WN> Base;
+  typedef typename Base::S S;
+  static constexpr int NT = Base::NT;
+  static constexpr int U = 2 * NS;                       // units per (row, group)
+  static constexpr int AU = BM * (BK / 8) * U / NT;      // units per thread and k-tile, A
+  static constexpr int BU = BN * (BK / 8) * U / NT;      // the same for B
+  static_assert(AU * NT == BM * (BK / 8) * U && BU * NT == BN * (BK / 8) * U, "staging layout");
+
+  u32x4 pa[AU], pb[BU];
+
+  // The k-tile of groups [g0, g0 + 4) of A (rows at the tile origin, ldga
+  // groups per row) and B (columns at the tile origin, ldgb groups per
+  // column); rows >= M and columns >= N read as 0.  A row's k-tile is 4 U
+  // consecutive units, and consecutive threads take consecutive units, so
+  // every load instruction reads contiguous memory across the wave.
+  // COHERENT_B: B with agent-scope loads (B written by other workgroups of
+  // this launch).
+  template <bool COHERENT_B = false>
+  __device__ void pfetch(const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
+                         int64_t N, int64_t g0) {
+    const int tid = Base::ltid();
+#pragma unroll
+    for (int c = 0; c < AU; ++c) {
+      const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U);
+      pa[c] = r < M ? A[((int64_t)r * ldga + g0) * U + e] : u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int c = 0; c < BU; ++c) {
+      const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U);
+      if (r < N) {
+        const u32x4* p = B + ((int64_t)r * ldgb + g0) * U + e;
+        if constexpr (COHERENT_B) pb[c] = coherent_load16(p);
+        else pb[c] = *p;
+      } else {
+        pb[c] = u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+  }
+
+  // The fetched k-tile into the LDS planes (no conversion).
+  __device__ void pstash(S& s) const {
+    const int tid = Base::ltid();
+#pragma unroll
+    for (int c = 0; c < AU; ++c) {
+      const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U), g = e / U, q = e - g * U;
+      *reinterpret_cast<u32x4*>(&s.a[q][r][g * 8]) = pa[c];
+    }
+#pragma unroll
+    for (int c = 0; c < BU; ++c) {
+      const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U), g = e / U, q = e - g * U;
+      *reinterpret_cast<u32x4*>(&s.b[q][r][g * 8]) = pb[c];
+    }
+  }
+
+  // acc += A B over KG groups (a multiple of 4) from group 0 of A's rows and
+  // B's columns.
+  template <bool COHERENT_B = false>
+  __device__ void pmac(S& sm, const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
+                       int64_t N, int64_t KG) {
+    const int64_t KT = KG / 4;
+    pfetch<COHERENT_B>(A, ldga, B, ldgb, M, N, 0);
+    for (int64_t kt = 0; kt < KT; ++kt) {
+      __syncthreads();   // previous tile's reads are done
+      pstash(sm);
+      __syncthreads();
+      if (kt + 1 < KT) pfetch<COHERENT_B>(A, ldga, B, ldgb, M, N, (kt + 1) * 4);
+      this->compute(sm);
+    }
+  }
+
+  // acc += A_0 B_0 + ... + A_{nsrc-1} B_{nsrc-1}, every source KG groups (a
+  // multiple of 4) with ldg groups per row of A and per column of B.
+  __device__ void pmac_n(S& sm, const u32x4* const* A, const u32x4* const* B, int nsrc, int64_t ldg, int64_t M,
+                         int64_t N, int64_t KG) {
+    const int64_t KT = KG / 4, T = nsrc * KT;
+    pfetch(A[0], ldg, B[0], ldg, M, N, 0);
+    for (int64_t t = 0; t < T; ++t) {
+      __syncthreads();   // previous tile's reads are done
+      pstash(sm);
+      __syncthreads();
+      if (t + 1 < T) {
+        const int64_t src = (t + 1) / KT, kt = (t + 1) - src * KT;
+        pfetch(A[src], ldg, B[src], ldg, M, N, kt * 4);
+      }
+      this->compute(sm);
+    }
+  }
+
+  // alpha * acc as split B-operand units of this tile's BM rows (k) and BN
+  // columns: XS at the tile origin (row group 0, column 0), ldg groups per
+  // column; rows >= M are written as 0, columns >= N not at all.  Agent-scope
+  // stores (read by other workgroups of the launch).  Each lane holds 4
+  // consecutive rows of a column: half a group, 8 bytes per plane.
+  __device__ void store_split_cols(u32x4* XS, int64_t ldg, int64_t M, int64_t N, float2 alpha) const {
+#pragma unroll
+    for (int a = 0; a < Base::TM; ++a)
+#pragma unroll
+      for (int b = 0; b < Base::TN; ++b) {
+        const int col = Base::col_of(b);
+        if (col >= N) continue;
+        const int row = Base::row_of(a, 0);   // multiple of 4
+        unsigned short t[2 * NS][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float xr = 0.f, xi = 0.f;
+          if (row + r < M) {
+            const float ar = this->cr[a][b][r], ai = this->ci[a][b][r];
+            xr = alpha.x * ar - alpha.y * ai;
+            xi = alpha.x * ai + alpha.y * ar;
+          }
+          unsigned short p[NS];
+          mg_sg::split_n<NS>(xr, p);
+#pragma unroll
+          for (int q = 0; q < NS; ++q) t[q][r] = p[q];
+          mg_sg::split_n<NS>(xi, p);
+#pragma unroll
+          for (int q = 0; q < NS; ++q) t[NS + q][r] = p[q];
+        }
+        u32x4* unit = XS + ((int64_t)col * ldg + (row >> 3)) * U;
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+          const unsigned long long v = (unsigned long long)(t[q][0] | ((unsigned)t[q][1] << 16)) |
+                                       ((unsigned long long)(t[q][2] | ((unsigned)t[q][3] << 16)) << 32);
+          coherent_store8(reinterpret_cast<unsigned long long*>(unit + q) + ((row >> 2) & 1), v);
+        }
+      }
+  }
+};
+
+}  // namespace mg_ps

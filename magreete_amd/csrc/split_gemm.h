@@ -29,10 +29,12 @@ __device__ inline void split_n(float x, unsigned short* p) {
   }
 }
 
+// One padding row per plane shifts consecutive planes by 20 LDS banks, so
+// that lanes storing the same (row, k) of different planes do not collide.
 template <int NS, int BM, int BN>
 struct alignas(16) Smem {
-  unsigned short a[2 * NS][BM][KP];   // planes: re part 0..NS-1, im part 0..NS-1
-  unsigned short b[2 * NS][BN][KP];
+  unsigned short a[2 * NS][BM + 1][KP];   // planes: re part 0..NS-1, im part 0..NS-1
+  unsigned short b[2 * NS][BN + 1][KP];
 };
 
 // (r, i) += A B for one 16 x 16 x 32 complex step from split planes:

@@ -193,7 +193,8 @@ class Factor:
                 raise ValueError("the unpivoted block LDU factors complex64 matrices")
             self.ipiv = None
             self.info = torch.zeros(1, dtype=torch.int32, device=dev)
-            self.dinv = torch.empty(lib.mg_ldu_dinv_elems(self.n), dtype=A.dtype, device=dev)
+            # D inverses, then (split precisions) the pre-split panels
+            self.dinv = torch.empty(lib.mg_ldu_factor_elems(PRECS[prec], self.n), dtype=A.dtype, device=dev)
             work = torch.empty(lib.mg_ldu_work_elems(self.n), dtype=A.dtype, device=dev)
             with torch.cuda.device(dev):
                 st = lib.mg_getrf_ldu(handle(dev), PRECS[prec], _p(A), self.n, _ld(A), _p(self.dinv), _p(work),
@@ -202,14 +203,20 @@ class Factor:
             del work          # stream-ordered: the factorisation's last step waits on this stream
         self._work = None
 
+    def work_elems(self, nrhs):
+        """Elements of the factor's dtype that a solve with nrhs columns needs as work."""
+        if self.pivot:
+            return self.n * nrhs
+        return int(L.load().mg_ldu_solve_work_elems(PRECS[self.prec], self.n, nrhs))
+
     def solve(self, B, work=None):
         """In place: B <- A^{-1} B (B n x nrhs, the factor's dtype, row-major).
-        work: n * nrhs elements of the factor's dtype owned by the caller
-        (solves that run at the same time on one factor each bring their
-        own); default: a buffer kept by the factor."""
+        work: work_elems(nrhs) elements of the factor's dtype owned by the
+        caller (solves that run at the same time on one factor each bring
+        their own); default: a buffer kept by the factor."""
         lib = L.load()
         dev = self.A.device
-        need = B.shape[0] * B.shape[1]
+        need = self.work_elems(B.shape[1])
         if work is None:
             if self._work is None or self._work.numel() < need:
                 self._work = torch.empty(need, dtype=self.A.dtype, device=dev)

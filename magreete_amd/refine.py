@@ -40,7 +40,7 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None):
     dev = M.device
     n, T = B.shape
     lowp = torch.empty((n, T), dtype=torch.complex64, device=dev)
-    fwork = torch.empty(n * T, dtype=F.A.dtype, device=dev)
+    fwork = torch.empty(F.work_elems(T), dtype=F.A.dtype, device=dev)
     m_max = max(1, min(restart, max_iters))
     work = torch.empty(L.load().mg_mdot_work_elems(m_max + 1, T), dtype=torch.complex128, device=dev)
 

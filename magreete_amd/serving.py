@@ -122,10 +122,17 @@ def workload(req):
 
 
 def resident_bytes(n, method):
-    """Bytes that stay resident for a factorised system of order n."""
+    """Bytes that stay resident for a factorised system of order n: M
+    (complex128) and the factors (complex128 LU with its diagonal-block
+    inverses, or the complex64 block LDU, its D inverses and, for the split
+    precisions, its panels stored split into bf16 parts: 4 bytes per part and
+    complex element, Lh and Uh)."""
+    nblk = (n + 255) // 256
     if method == "fp64":
-        return 16 * n * n + 16 * 256 * 256 * 2 * ((n + 255) // 256)
-    return 16 * n * n + 8 * n * n + 8 * 256 * 256 * ((n + 255) // 256)
+        return 16 * n * n + 16 * 256 * 256 * 2 * nblk
+    parts = {"bf16x3": 2, "bf16x6": 3}.get(method, 0)
+    split = 2 * 4 * parts * 256 * 256 * nblk * (nblk - 1) // 2
+    return 16 * n * n + 8 * n * n + 8 * 256 * 256 * nblk + split
 
 
 def transient_bytes(n, T, n_meas, method, restart=12):
