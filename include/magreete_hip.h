@@ -162,6 +162,15 @@ int mg_sources(int space, int type, const double* points, int64_t npts, const do
 int64_t mg_mdot_work_elems(int64_t k, int64_t T);
 int mg_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* w, int64_t ldw, int64_t n, int64_t T,
             void* work, void* out, int64_t ldo, void* stream);
+/* Arnoldi column j of the GMRES least-squares problems of all T columns (no
+ * reference counterpart): H ((m+1) x m x T complex128, entry (i, j) of column
+ * t at H[(i m + j) T + t]) holds the Gram-Schmidt coefficients of column j;
+ * hn (T doubles) the norms of the new vectors, which become h_{j+1,j}.  The
+ * unrotated column is copied to H0, the j earlier Givens rotations (cs, sn:
+ * m x T) and a new one are applied, g ((m+1) x T) is rotated, and est[0]
+ * receives max_t |g_{j+1,t}| / bnorm[t] (device pointers, stream-ordered). */
+int mg_givens(void* H, void* H0, void* cs, void* sn, void* g, const double* hn, const double* bnorm, int m, int j,
+              int64_t T, double* est, void* stream);
 int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
              int64_t ldw, int64_t n, int64_t T, void* stream);
 

@@ -53,6 +53,7 @@ SIGNATURES = {
     "mg_ldu_dinv_elems": [_i64],
     "mg_ldu_work_elems": [_i64],
     "mg_ldu_factor_elems": [_int, _i64],
+    "mg_givens": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _int, _i64, _vp, _vp],
     "mg_ldu_solve_work_elems": [_int, _i64, _i64],
     "mg_getrf_ldu": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp, _vp],
     "mg_getrs_ldu": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp],

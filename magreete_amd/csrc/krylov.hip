@@ -91,9 +91,76 @@ __global__ __launch_bounds__(256) void k_maxpy(const double2* __restrict__ V, in
   w[r * ldw + t] = o;
 }
 
+// Column j of the Hessenberg matrices of all right-hand sides, one thread
+// per column t (H: (m+1) x m x T, entry (i, j) of column t at
+// H[(i m + j) T + t]; cs, sn: m x T; g: (m+1) x T).  On entry H holds the
+// Gram-Schmidt coefficients h_ij, i <= j; hn[t] = ||w|| becomes h_{j+1,j}.
+// The unrotated column goes to H0, then the j earlier Givens rotations and a
+// new one (which zeroes h_{j+1,j}) are applied and g is rotated with it.
+// est[0] = max_t |g_{j+1,t}| / bnorm[t], the largest relative residual of the
+// least-squares problem (one workgroup: a fixed reduction order).
+__global__ __launch_bounds__(256) void k_givens(double2* __restrict__ H, double2* __restrict__ H0,
+                                                double2* __restrict__ cs, double2* __restrict__ sn,
+                                                double2* __restrict__ g, const double* __restrict__ hn,
+                                                const double* __restrict__ bnorm, int m, int j, int64_t T,
+                                                double* __restrict__ est) {
+  __shared__ double red[256];
+  double mx = 0.0;
+  for (int64_t t = threadIdx.x; t < T; t += blockDim.x) {
+    auto at = [&](int i) -> double2& { return H[((int64_t)i * m + j) * T + t]; };
+    at(j + 1) = make_double2(hn[t], 0.0);
+    for (int i = 0; i <= j + 1; ++i) H0[((int64_t)i * m + j) * T + t] = at(i);
+    for (int i = 0; i < j; ++i) {
+      const double2 c = cs[(int64_t)i * T + t], sv = sn[(int64_t)i * T + t];
+      const double2 a = at(i), b = at(i + 1);
+      // conj(c) a + conj(s) b  and  -s a + c b
+      at(i) = make_double2(c.x * a.x + c.y * a.y + sv.x * b.x + sv.y * b.y,
+                           c.x * a.y - c.y * a.x + sv.x * b.y - sv.y * b.x);
+      at(i + 1) = make_double2(-(sv.x * a.x - sv.y * a.y) + (c.x * b.x - c.y * b.y),
+                               -(sv.x * a.y + sv.y * a.x) + (c.x * b.y + c.y * b.x));
+    }
+    const double2 a = at(j), b = at(j + 1);
+    double den = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
+    if (den == 0.0) den = 1.0;
+    const double2 c = make_double2(a.x / den, a.y / den), sv = make_double2(b.x / den, b.y / den);
+    cs[(int64_t)j * T + t] = c;
+    sn[(int64_t)j * T + t] = sv;
+    at(j) = make_double2(den, 0.0);
+    at(j + 1) = make_double2(0.0, 0.0);
+    const double2 gj = g[(int64_t)j * T + t];
+    const double2 g1 = make_double2(-(sv.x * gj.x - sv.y * gj.y), -(sv.x * gj.y + sv.y * gj.x));
+    g[(int64_t)(j + 1) * T + t] = g1;
+    g[(int64_t)j * T + t] = make_double2(c.x * gj.x + c.y * gj.y, c.x * gj.y - c.y * gj.x);
+    const double e = sqrt(g1.x * g1.x + g1.y * g1.y) / bnorm[t];
+    mx = (e > mx || e != e) ? e : mx;   // a NaN wins, so that it reaches the caller
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      const double o = red[threadIdx.x + off];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) est[0] = red[0];
+}
+
 }  // namespace
 
 extern "C" {
+
+int mg_givens(void* H, void* H0, void* cs, void* sn, void* g, const double* hn, const double* bnorm, int m, int j,
+              int64_t T, double* est, void* stream) {
+  if (!H || !H0 || !cs || !sn || !g || !hn || !bnorm || !est || m < 1 || j < 0 || j >= m || T < 0)
+    return MG_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, s, reinterpret_cast<double2*>(H), reinterpret_cast<double2*>(H0),
+                     reinterpret_cast<double2*>(cs), reinterpret_cast<double2*>(sn), reinterpret_cast<double2*>(g), hn,
+                     bnorm, m, j, T, est);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
 
 int64_t mg_mdot_work_elems(int64_t k, int64_t T) { return (k < 0 || T < 0) ? 0 : (int64_t)kChunks * k * T; }
 

@@ -364,6 +364,61 @@ int pack_cols(const float2* B, int64_t ldb, int64_t K, int64_t cols, int64_t col
   return MG_OK;
 }
 
+// A panel of the split factorisation, C = A B (complex float operands, row-major;
+// entries outside M x N x K read as 0), written pre-split as A-operand units
+// (rows of C, k along its columns, ldg groups per row) over the padded
+// extent Mp x Np (Np a multiple of 8) of the grid, whose tiles past M x N hold
+// zeros.  The accumulators go through LDS so that every thread writes whole
+// 8-column groups.
+template <int NS>
+__global__ __launch_bounds__(256) void k_panel_split(int64_t M, int64_t N, int64_t K, const float2* __restrict__ A,
+                                                     int64_t lda, const float2* __restrict__ B, int64_t ldb,
+                                                     int64_t Mp, int64_t Np, u32x4* __restrict__ out, int64_t ldg,
+                                                     int nbn) {
+  constexpr int BM = 64, BN = 64;
+  typedef mg_sg::Tile<NS, BM, BN, 2, 2> Tile;
+  __shared__ typename Tile::S sm;
+  static_assert(sizeof(typename Tile::S) >= sizeof(float2) * BM * (BN + 1), "staging fits the operand LDS");
+  const int64_t i0 = (int64_t)(blockIdx.x / nbn) * BM, j0 = (int64_t)(blockIdx.x % nbn) * BN;
+  Tile t;
+  t.init(nullptr, 0, 0, 0, make_float2(0.f, 0.f), false);
+  t.mac(sm, A + i0 * lda, lda, B + j0, ldb, M - i0, N - j0, K);
+  __syncthreads();
+  float2* stage = reinterpret_cast<float2*>(&sm);   // [BM][BN + 1]
+#pragma unroll
+  for (int a = 0; a < Tile::TM; ++a)
+#pragma unroll
+    for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        stage[Tile::row_of(a, r) * (BN + 1) + Tile::col_of(b)] = make_float2(t.cr[a][b][r], t.ci[a][b][r]);
+  __syncthreads();
+  for (int c = threadIdx.x; c < BM * (BN / 8); c += 256) {
+    const int row = c / (BN / 8), g = c % (BN / 8);
+    const int64_t grow = i0 + row, gcol = j0 + 8 * g;
+    if (grow >= Mp || gcol >= Np) continue;
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = stage[row * (BN + 1) + 8 * g + e];
+    u32x4 u[2 * NS];
+    mg_ps::split8<NS>(v, u);
+    u32x4* dst = out + (grow * ldg + gcol / 8) * 2 * NS;
+#pragma unroll
+    for (int q = 0; q < 2 * NS; ++q) dst[q] = u[q];
+  }
+}
+
+template <int NS>
+int panel_split(int64_t M, int64_t N, int64_t K, const float2* A, int64_t lda, const float2* B, int64_t ldb,
+                int64_t Mp, int64_t Np, u32x4* out, int64_t ldg, hipStream_t s) {
+  if (Mp <= 0 || Np <= 0) return MG_OK;
+  const int nbm = (int)((Mp + 63) / 64), nbn = (int)((Np + 63) / 64);
+  hipLaunchKernelGGL(k_panel_split<NS>, dim3(nbm * nbn), dim3(256), 0, s, M, N, K, A, lda, B, ldb, Mp, Np, out, ldg,
+                     nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
 // Trailing update of the split factorisation, C -= A_0 B_0 (+ A_1 B_1) with
 // every A (M rows) and B (N columns) pre-split over one 256-block of k (the
 // panels of one or two steps); C complex float row-major.  The accumulators
@@ -605,8 +660,8 @@ int64_t split_elems(int ns, int64_t n) {
   const SplitLayout lay{padded(n), (n + kIB - 1) / kIB, 2 * ns};
   return 2 * lay.total() * 2;   // SPL and SPUr, two float2 per 16-byte unit
 }
-// the double-buffered pre-split copy of A12 per step (widest split, 6 units)
-int64_t a12_split_elems(int64_t n) { return 2 * padded(n) * (kIB / 8) * 6 * 2; }
+// the pre-split copies of A12, one per step for four steps in flight (widest split, 6 units)
+int64_t a12_split_elems(int64_t n) { return 4 * padded(n) * (kIB / 8) * 6 * 2; }
 }  // namespace
 
 int64_t ldu_factor_elems(int ns, int64_t n) { return ldu_dinv_elems(n) + split_elems(ns, n); }
@@ -648,17 +703,24 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   u32x4* SPL = split_ns ? reinterpret_cast<u32x4*>(D + ldu_dinv_elems(n)) : nullptr;
   u32x4* SPUr = split_ns ? SPL + lay.total() : nullptr;
   u32x4* A12s = reinterpret_cast<u32x4*>(R + nb * nb);
-  auto spa12 = [&](int64_t k) { return A12s + (k & 1) * lay.np * (kIB / 8) * U; };
-  auto pack_panels = [&](int64_t k, const float2* S12, hipStream_t st) -> int {
-    const int64_t r1 = (k + 1) * nb, r = n - r1, rk = lay.rows(k);
+  auto spa12 = [&](int64_t k) { return A12s + (k & 3) * lay.np * (kIB / 8) * U; };
+  // step k's panels of a split precision, straight from A and D_k: A12
+  // pre-split as the trailing update's B operand, Uh = D_k A12 and
+  // Lh = A21 D_k as GEMMs whose epilogues write the pre-split panels (A is
+  // not written, so no copies are needed)
+  auto split_panels = [&](int64_t k, hipStream_t st) -> int {
+    const int64_t r0 = k * nb, r1 = r0 + nb, r = n - r1, rk = lay.rows(k);
+    const float2* Dk = D + k * nb * nb;
+    const float2* A12 = Ap + r0 * lda + r1;
+    const float2* A21 = Ap + r1 * lda + r0;
     if (split_ns == 2) {
-      MG_TRY(pack_rows<2>(Ap + r1 * lda + k * nb, lda, r, nb, rk, kIB / 8, SPL + lay.off(k), st));
-      MG_TRY(pack_rows<2>(Ap + k * nb * lda + r1, lda, nb, r, nb, rk / 8, SPUr + lay.off(k), st));
-      return pack_cols<2>(S12, r, nb, r, rk, kIB / 8, spa12(k), st);
+      MG_TRY(pack_cols<2>(A12, lda, nb, r, rk, kIB / 8, spa12(k), st));
+      MG_TRY(panel_split<2>(nb, r, nb, Dk, nb, A12, lda, nb, rk, SPUr + lay.off(k), rk / 8, st));
+      return panel_split<2>(r, nb, nb, A21, lda, Dk, nb, rk, nb, SPL + lay.off(k), kIB / 8, st);
     }
-    MG_TRY(pack_rows<3>(Ap + r1 * lda + k * nb, lda, r, nb, rk, kIB / 8, SPL + lay.off(k), st));
-    MG_TRY(pack_rows<3>(Ap + k * nb * lda + r1, lda, nb, r, nb, rk / 8, SPUr + lay.off(k), st));
-    return pack_cols<3>(S12, r, nb, r, rk, kIB / 8, spa12(k), st);
+    MG_TRY(pack_cols<3>(A12, lda, nb, r, rk, kIB / 8, spa12(k), st));
+    MG_TRY(panel_split<3>(nb, r, nb, Dk, nb, A12, lda, nb, rk, SPUr + lay.off(k), rk / 8, st));
+    return panel_split<3>(r, nb, nb, A21, lda, Dk, nb, rk, nb, SPL + lay.off(k), kIB / 8, st);
   };
   auto inverse = [&](int64_t k, hipStream_t st) -> int {
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb;
@@ -680,6 +742,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   auto panel_rest = [&](int64_t k, hipStream_t st) -> int {
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
     if (r == 0) return MG_OK;
+    if (split_ns) return split_panels(k, st);
     const float2* Dk = D + k * nb * nb;
     float2 *S12 = s12(k), *S21 = s21(k);
     MG_HIP(hipMemcpy2DAsync(S12, r * sizeof(float2), Ap + r0 * lda + r1, lda * sizeof(float2), r * sizeof(float2),
@@ -688,7 +751,6 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
                             r, hipMemcpyDeviceToDevice, st));
     MG_TRY(g(h, r, h, cd{1, 0}, Dk, nb, S12, r, cd{0, 0}, Ap + r0 * lda + r1, lda, st));   // Uh = S^-1 A12
     MG_TRY(g(r, h, h, cd{1, 0}, S21, h, Dk, nb, cd{0, 0}, Ap + r1 * lda + r0, lda, st));   // Lh = A21 S^-1
-    if (split_ns) MG_TRY(pack_panels(k, S12, st));
     return MG_OK;
   };
   // A22[rows i0.., cols j0..j0+nc] -= A21 S^-1 A12 for step k (A21 from the scratch copy, Uh in place)
@@ -718,9 +780,9 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
                                            nsrc, Ap + i0 * lda + j0, lda, st);
     });
   };
-  MG_TRY(inverse(0, s));
-  MG_TRY(panel_rest(0, s));
   if (!split_ns || !ws.s2) {
+    MG_TRY(inverse(0, s));
+    MG_TRY(panel_rest(0, s));
     // Look-ahead of depth one (with a second stream): block row and column
     // k+1 are updated first, step k+1's inverse and panels run on the second
     // stream while the rest of step k's trailing update runs here.
@@ -749,37 +811,38 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     }
     return MG_OK;
   }
-  // Split precisions: the second stream carries the factorisation's
-  // critical path.  Step k updates the next diagonal block S_{k+1} there and
-  // inverts it at once, while the rest of block row and column k+1 is
-  // updated on the caller's stream; step k+1's panels follow on the second
-  // stream once those are final, and the rest of step k's trailing update
-  // runs beside them.  That rest leaves out the block S_{k+2}, which the next
-  // step's diagonal update takes with both steps' panels (so that no two
-  // launches update one block at the same time).  evA: step k's panels done;
-  // evB: block row and column k+1 done.
-  MG_HIP(hipEventRecord(ws.evA, s));
-  for (int64_t k = 0; k < nblk; ++k) {
-    const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb, r1 = r0 + h, r = n - r1;
-    if (r == 0) break;
-    const int64_t h2 = (r < nb) ? r : nb;   // next block's order
-    const int64_t r2 = r1 + h2, h3 = (n - r2 < nb) ? n - r2 : nb;   // block k+2
-    MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
-    if (k == 0) MG_TRY(trail_ps(0, false, r1, h2, r1, h2, ws.s2));
-    else MG_TRY(trail_ps(k - 1, true, r1, h2, r1, h2, ws.s2));   // S_{k+1}: steps k-1 and k
-    MG_TRY(inverse(k + 1, ws.s2));
-    MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));
-    MG_TRY(trail_ps(k, false, r1, h2, r2, n - r2, s));
-    MG_TRY(trail_ps(k, false, r2, n - r2, r1, h2, s));
+  // Split precisions, in super-steps of two blocks (k, k+1).  The second
+  // stream factors a super-step's panels: inverse and panels of block k, the
+  // update of block row and column k+1 by step k, inverse and panels of block
+  // k+1.  The caller's stream then updates the next super-step's block rows
+  // and columns (k+2, k+3) by both steps at once (k = 512 products), after
+  // which the next super-step's panels start on the second stream while the
+  // rest of the trailing matrix is updated, again by both steps at once: one
+  // read and write of the trailing matrix per two steps.  The two streams
+  // touch disjoint blocks.  evA: a super-step's panels done; evB: the next
+  // super-step's block rows and columns done.
+  auto chain = [&](int64_t k, hipStream_t st) -> int {
+    MG_TRY(inverse(k, st));
+    MG_TRY(panel_rest(k, st));
+    if (k + 1 >= nblk) return MG_OK;
+    const int64_t r1 = (k + 1) * nb, h2 = (n - r1 < nb) ? n - r1 : nb, r2 = r1 + h2;
+    MG_TRY(trail_ps(k, false, r1, h2, r1, n - r1, st));   // block row k+1 with its diagonal block
+    MG_TRY(trail_ps(k, false, r2, n - r2, r1, h2, st));   // block column k+1 below it
+    MG_TRY(inverse(k + 1, st));
+    return panel_rest(k + 1, st);
+  };
+  MG_TRY(chain(0, s));
+  for (int64_t k = 0; k + 2 < nblk; k += 2) {
+    const int64_t b2 = (k + 2) * nb, w = (n - b2 < 2 * nb) ? n - b2 : 2 * nb, b4 = b2 + w;
+    MG_TRY(trail_ps(k, true, b2, w, b2, n - b2, s));      // block rows k+2, k+3
+    MG_TRY(trail_ps(k, true, b4, n - b4, b2, w, s));      // block columns k+2, k+3 below them
     MG_HIP(hipEventRecord(ws.evB, s));
     MG_HIP(hipStreamWaitEvent(ws.s2, ws.evB, 0));
-    MG_TRY(panel_rest(k + 1, ws.s2));
+    MG_TRY(chain(k + 2, ws.s2));
     MG_HIP(hipEventRecord(ws.evA, ws.s2));
-    // the rest of step k's update, without block S_{k+2}
-    MG_TRY(trail_ps(k, false, r2, h3, r2 + h3, n - r2 - h3, s));
-    MG_TRY(trail_ps(k, false, r2 + h3, n - r2 - h3, r2, n - r2, s));
+    MG_TRY(trail_ps(k, true, b4, n - b4, b4, n - b4, s));   // the rest
+    MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));
   }
-  MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));   // the last inverse
   return MG_OK;
 }
 

@@ -354,6 +354,24 @@ def maxpy(V, h, w, alpha=1.0):
     return w
 
 
+def givens(H, H0, cs, sn, g, hn, bnorm, j, est):
+    """Column j of the batched GMRES Hessenberg matrices (H, H0: (m+1, m, T);
+    cs, sn: (m, T); g: (m+1, T), complex128; hn, bnorm: (T,) float64): sets
+    h_{j+1,j} = hn, copies the column to H0, applies the earlier Givens
+    rotations and a new one, rotates g; est (1,) float64 receives
+    max_t |g_{j+1,t}| / bnorm[t]."""
+    lib = L.load()
+    dev = H.device
+    m, T = H.shape[1], H.shape[2]
+    for t in (H, H0, cs, sn, g, hn, bnorm, est):
+        if not t.is_contiguous():
+            raise ValueError("givens: contiguous tensors expected")
+    with torch.cuda.device(dev):
+        st = lib.mg_givens(_p(H), _p(H0), _p(cs), _p(sn), _p(g), _p(hn), _p(bnorm), m, j, T, _p(est), _stream(dev))
+    L.check(st, "mg_givens")
+    return est
+
+
 def hankel1(nu, x, exact=False):
     """H_nu^(1)(x); exact=False gives the reference's torch values (see bessel_ref.h)."""
     lib = L.load()

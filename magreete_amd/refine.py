@@ -50,6 +50,8 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None):
         E.convert(lowp, out)
         return out
 
+    bnorm64 = bnorm.to(torch.float64).contiguous()
+    est_buf = torch.empty(1, dtype=torch.float64, device=dev)
     X.zero_()
     R = B.clone()
     beta = torch.linalg.norm(R, dim=0)
@@ -85,25 +87,13 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None):
                 E.maxpy(V[: j + 1], h, w, alpha=-1.0)
                 H[: j + 1, j] += h
             hn = torch.linalg.norm(w, dim=0)
-            H[j + 1, j] = hn.to(torch.complex128)
-            H0[: j + 2, j] = H[: j + 2, j]
             w /= torch.where(hn == 0, torch.ones_like(hn), hn)
-            for i in range(j):                                # apply earlier Givens rotations
-                a, b = H[i, j].clone(), H[i + 1, j].clone()
-                H[i, j] = cs[i].conj() * a + sn[i].conj() * b
-                H[i + 1, j] = -sn[i] * a + cs[i] * b
-            a, b = H[j, j], H[j + 1, j]
-            den = torch.sqrt(a.abs() ** 2 + b.abs() ** 2)
-            den = torch.where(den == 0, torch.ones_like(den), den)
-            cs[j] = a / den
-            sn[j] = b / den
-            H[j, j] = den.to(torch.complex128)
-            H[j + 1, j] = 0
-            g[j + 1] = -sn[j] * g[j]
-            g[j] = cs[j].conj() * g[j]
+            # h_{j+1,j} = hn, the unrotated column into H0, the earlier Givens
+            # rotations and a new one, g rotated (one HIP kernel)
+            E.givens(H, H0, cs, sn, g, hn, bnorm64, j, est_buf)
             k = j + 1
             # |g_{j+1}| is the norm of the Arnoldi residual formed below
-            est = (g[j + 1].abs() / bnorm).max().item()
+            est = est_buf.item()
             if history is not None:
                 history.append(est)
             if est <= tol:

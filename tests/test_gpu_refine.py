@@ -148,3 +148,43 @@ def test_stall_falls_back_to_fp64():
     assert st["tier"] == "fp64" and st["fallback"]
     ref = np.linalg.solve(A, B)
     assert colerr(X.cpu(), torch.from_numpy(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("j", [0, 1, 4])
+def test_givens_kernel_matches_torch(j):
+    """The fused Arnoldi-column kernel against the same rotations written as
+    torch operations (earlier rotations, the new one, g and the estimate)."""
+    say("givens", j)
+    rng = np.random.default_rng(7 + j)
+    m, T = 6, 37
+    H = torch.from_numpy(crand((m + 1, m, T), rng)).to(DEV)
+    H[j + 1:] = 0
+    cs = torch.from_numpy(crand((m, T), rng)).to(DEV)
+    sn = torch.from_numpy(crand((m, T), rng)).to(DEV)
+    g = torch.from_numpy(crand((m + 1, T), rng)).to(DEV)
+    hn = torch.from_numpy(rng.uniform(0.5, 2.0, T)).to(DEV)
+    hn[3] = 0.0
+    bnorm = torch.from_numpy(rng.uniform(0.5, 2.0, T)).to(DEV)
+    # reference
+    Hr, csr, snr, gr = H.clone(), cs.clone(), sn.clone(), g.clone()
+    Hr[j + 1, j] = hn.to(torch.complex128)
+    H0r = torch.zeros_like(Hr)
+    H0r[: j + 2, j] = Hr[: j + 2, j]
+    for i in range(j):
+        a, b = Hr[i, j].clone(), Hr[i + 1, j].clone()
+        Hr[i, j] = csr[i].conj() * a + snr[i].conj() * b
+        Hr[i + 1, j] = -snr[i] * a + csr[i] * b
+    a, b = Hr[j, j].clone(), Hr[j + 1, j].clone()
+    den = torch.sqrt(a.abs() ** 2 + b.abs() ** 2)
+    den = torch.where(den == 0, torch.ones_like(den), den)
+    csr[j], snr[j] = a / den, b / den
+    Hr[j, j], Hr[j + 1, j] = den.to(torch.complex128), 0
+    gr[j + 1] = -snr[j] * gr[j]
+    gr[j] = csr[j].conj() * gr[j]
+    est_r = (gr[j + 1].abs() / bnorm).max().item()
+    H0 = torch.zeros_like(H)
+    est = torch.empty(1, dtype=torch.float64, device=DEV)
+    E.givens(H, H0, cs, sn, g, hn, bnorm, j, est)
+    for x, y in ((H, Hr), (H0, H0r), (cs, csr), (sn, snr), (g, gr)):
+        assert torch.allclose(x, y, rtol=1e-13, atol=1e-13)
+    assert abs(est.item() - est_r) <= 1e-13 * est_r
