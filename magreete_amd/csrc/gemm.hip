@@ -107,14 +107,20 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(int64_t M, int64_t N, int
 }
 
 // Number of K-splits for a tall-skinny product whose 128 x 64 tile grid
-// leaves CUs idle: enough splits for two workgroups per CU, each split at
-// least 512 deep.  1 = no split.
+// leaves CUs idle: at least enough splits for two workgroups per CU, and
+// preferably a number that makes the workgroup count a multiple of the CU
+// count (the 128 x 64 FP64 tile holds one workgroup per CU, so a partial
+// last round leaves CUs idle: 12288 x 128 x 12288 took 3.1 ms with three
+// splits, 2.25 rounds); each split at least 512 deep.  1 = no split.
 int splitk_parts(int64_t M, int64_t N, int64_t K) {
   const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64);
-  const int64_t want = 2 * (int64_t)mg::cu_count();
+  const int64_t cus = (int64_t)mg::cu_count(), want = 2 * cus;
   if (tiles >= want || K < 1024) return 1;
+  const int64_t top = std::min<int64_t>(K / 512, 16);
   int64_t S = (want + tiles - 1) / tiles;
-  S = std::min<int64_t>(S, std::min<int64_t>(K / 512, 16));
+  for (int64_t t = S; t <= top; ++t)
+    if ((tiles * t) % cus == 0) return (int)t;
+  S = std::min<int64_t>(S, top);
   return (int)std::max<int64_t>(S, 1);
 }
 

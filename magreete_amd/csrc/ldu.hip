@@ -780,7 +780,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
                                            nsrc, Ap + i0 * lda + j0, lda, st);
     });
   };
-  if (!split_ns || !ws.s2) {
+  if (!split_ns || !ws.s2 || !ws.s3) {
     MG_TRY(inverse(0, s));
     MG_TRY(panel_rest(0, s));
     // Look-ahead of depth one (with a second stream): block row and column
@@ -819,16 +819,25 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   // which the next super-step's panels start on the second stream while the
   // rest of the trailing matrix is updated, again by both steps at once: one
   // read and write of the trailing matrix per two steps.  The two streams
-  // touch disjoint blocks.  evA: a super-step's panels done; evB: the next
-  // super-step's block rows and columns done.
+  // touch disjoint blocks.  Inside a super-step, only the diagonal block
+  // S_{k+1} is updated by step k before its inversion; the rest of block row
+  // and column k+1 is updated on the third stream during that inversion.
+  // evA: a super-step's panels done; evB: the next super-step's block rows
+  // and columns done; evC / evD: step k's panels / block row and column k+1
+  // done (third stream).
   auto chain = [&](int64_t k, hipStream_t st) -> int {
     MG_TRY(inverse(k, st));
     MG_TRY(panel_rest(k, st));
     if (k + 1 >= nblk) return MG_OK;
     const int64_t r1 = (k + 1) * nb, h2 = (n - r1 < nb) ? n - r1 : nb, r2 = r1 + h2;
-    MG_TRY(trail_ps(k, false, r1, h2, r1, n - r1, st));   // block row k+1 with its diagonal block
-    MG_TRY(trail_ps(k, false, r2, n - r2, r1, h2, st));   // block column k+1 below it
+    MG_TRY(trail_ps(k, false, r1, h2, r1, h2, st));       // S_{k+1}
+    MG_HIP(hipEventRecord(ws.evC, st));
+    MG_HIP(hipStreamWaitEvent(ws.s3, ws.evC, 0));
+    MG_TRY(trail_ps(k, false, r1, h2, r2, n - r2, ws.s3));   // block row k+1 right of S_{k+1}
+    MG_TRY(trail_ps(k, false, r2, n - r2, r1, h2, ws.s3));   // block column k+1 below it
+    MG_HIP(hipEventRecord(ws.evD, ws.s3));
     MG_TRY(inverse(k + 1, st));
+    MG_HIP(hipStreamWaitEvent(st, ws.evD, 0));
     return panel_rest(k + 1, st);
   };
   MG_TRY(chain(0, s));

@@ -38,7 +38,8 @@ struct Workspace {
   Getf2Work getf2;
   UpdateProfile prof;
   hipStream_t s2 = nullptr;        // look-ahead panel stream (default priority)
-  hipEvent_t evA = nullptr, evB = nullptr;
+  hipStream_t s3 = nullptr;        // a third stream beside the look-ahead (default priority)
+  hipEvent_t evA = nullptr, evB = nullptr, evC = nullptr, evD = nullptr;
   float2* inv_exch = nullptr;      // LDU block inverse: two 256 x 32 published slabs
   int* inv_ctl = nullptr;          // LDU block inverse: barrier counter [0] and failure flag [32]
   int* sweep = nullptr;            // ticket + per-block-row counters of the LDU solve sweeps

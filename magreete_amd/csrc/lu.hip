@@ -565,8 +565,11 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipMemset(ws.getf2.err, 0, sizeof(int) * 64));
   if (ws.coop_blocks > G) ws.coop_blocks = G;
   MG_HIP(hipStreamCreateWithFlags(&ws.s2, hipStreamNonBlocking));
+  MG_HIP(hipStreamCreateWithFlags(&ws.s3, hipStreamNonBlocking));
   MG_HIP(hipEventCreateWithFlags(&ws.evA, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evB, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evC, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evD, hipEventDisableTiming));
   ws.ready = true;
   return MG_OK;
 }
@@ -601,8 +604,11 @@ void workspace_free(Workspace& ws) {
   if (ws.inv_exch) (void)hipFree(ws.inv_exch);
   if (ws.inv_ctl) (void)hipFree(ws.inv_ctl);
   if (ws.s2) (void)hipStreamDestroy(ws.s2);
+  if (ws.s3) (void)hipStreamDestroy(ws.s3);
   if (ws.evA) (void)hipEventDestroy(ws.evA);
   if (ws.evB) (void)hipEventDestroy(ws.evB);
+  if (ws.evC) (void)hipEventDestroy(ws.evC);
+  if (ws.evD) (void)hipEventDestroy(ws.evD);
   ws = Workspace();
 }
 
