@@ -24,12 +24,14 @@ PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0, "bf16x6": 2500.0 / 6
 # the factorisation's trailing-update launches (timed with HIP events on their stream)
 KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
           "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
-          "bf16x3": "k_cgemm_split<2, .., 1> (block-LDU trailing update; BF16x3 MFMA, 128x128 tiles)",
-          "bf16x6": "k_cgemm_split<3, .., 1> (block-LDU trailing update; BF16x6 MFMA, 128x128 tiles)"}
+          "bf16x3": "k_ldu_update<2, 64, 128, 2, 2> (block-LDU trailing update over pre-split panels, one or two "
+                    "steps per launch; BF16x3 MFMA, 64x128 tiles)",
+          "bf16x6": "k_ldu_update<3, 64, 64, 2, 2> (block-LDU trailing update over pre-split panels, one or two "
+                    "steps per launch; BF16x6 MFMA, 64x64 tiles)"}
 # HBM bytes per trailing-update launch (average over one factorisation of this workload), measured
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the
 # counters' reading for that 2 GiB copy (fetch x2.000, write x1.000)
-TRAFFIC = {"fp64": 2.503e9, "c64": 4.410e8, "bf16x3": 4.624e8, "bf16x6": 4.633e8}
+TRAFFIC = {"fp64": 2.497e9, "c64": 4.410e8, "bf16x3": 2.866e8, "bf16x6": 4.392e8}
 
 
 def colerr(x, ref):

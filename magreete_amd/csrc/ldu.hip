@@ -453,19 +453,14 @@ int launch_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0, const 
 
 // C -= A_0 B_0 (+ A_1 B_1: nsrc = 2).  BF16x3: 64 x 128 tiles (two
 // workgroups per CU; 12032^2 x 256 measured 1.07 ms against 1.21 ms with
-// 128 x 128); BF16x6: 128 x 128 when the grid gives every CU a tile, else
-// 64 x 64 (its 64 x 128 tile's LDS would leave one workgroup per CU).
+// 128 x 128); BF16x6: 64 x 64 (its 64 x 128 tile's LDS would leave one
+// workgroup per CU, and its 128 x 128 tile spills registers).
 template <int NS>
 int ldu_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0, const u32x4* A1, const u32x4* B1, int nsrc,
                float2* C, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return MG_OK;
-  if constexpr (NS == 2) {
-    return launch_update<NS, 64, 128, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
-  } else {
-    if (((M + 127) / 128) * ((N + 127) / 128) >= (int64_t)mg::cu_count())
-      return launch_update<NS, 128, 128, 4, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
-    return launch_update<NS, 64, 64, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
-  }
+  if constexpr (NS == 2) return launch_update<NS, 64, 128, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
+  else return launch_update<NS, 64, 64, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
 }
 
 constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)

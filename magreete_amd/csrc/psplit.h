@@ -68,51 +68,61 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
   static constexpr int BU = BN * (BK / 8) * U / NT;      // the same for B
   static_assert(AU * NT == BM * (BK / 8) * U && BU * NT == BN * (BK / 8) * U, "staging layout");
 
-  u32x4 pa[AU], pb[BU];
+  // One k-tile of both operands in flight (registers).
+  struct Stage {
+    u32x4 a[AU], b[BU];
+  };
+  Stage cur;
 
   // The k-tile of groups [g0, g0 + 4) of A (rows at the tile origin, ldga
   // groups per row) and B (columns at the tile origin, ldgb groups per
-  // column); rows >= M and columns >= N read as 0.  A row's k-tile is 4 U
-  // consecutive units, and consecutive threads take consecutive units, so
+  // column) into st; rows >= M and columns >= N read as 0.  A row's k-tile is
+  // 4 U consecutive units, and consecutive threads take consecutive units, so
   // every load instruction reads contiguous memory across the wave.
   // COHERENT_B: B with agent-scope loads (B written by other workgroups of
   // this launch).
   template <bool COHERENT_B = false>
-  __device__ void pfetch(const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
-                         int64_t N, int64_t g0) {
+  __device__ static void pfetch_to(Stage& st, const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb,
+                                   int64_t M, int64_t N, int64_t g0) {
     const int tid = Base::ltid();
 #pragma unroll
     for (int c = 0; c < AU; ++c) {
       const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U);
-      pa[c] = r < M ? A[((int64_t)r * ldga + g0) * U + e] : u32x4{0u, 0u, 0u, 0u};
+      st.a[c] = r < M ? A[((int64_t)r * ldga + g0) * U + e] : u32x4{0u, 0u, 0u, 0u};
     }
 #pragma unroll
     for (int c = 0; c < BU; ++c) {
       const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U);
       if (r < N) {
         const u32x4* p = B + ((int64_t)r * ldgb + g0) * U + e;
-        if constexpr (COHERENT_B) pb[c] = coherent_load16(p);
-        else pb[c] = *p;
+        if constexpr (COHERENT_B) st.b[c] = coherent_load16(p);
+        else st.b[c] = *p;
       } else {
-        pb[c] = u32x4{0u, 0u, 0u, 0u};
+        st.b[c] = u32x4{0u, 0u, 0u, 0u};
       }
     }
   }
+  template <bool COHERENT_B = false>
+  __device__ void pfetch(const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
+                         int64_t N, int64_t g0) {
+    pfetch_to<COHERENT_B>(cur, A, ldga, B, ldgb, M, N, g0);
+  }
 
-  // The fetched k-tile into the LDS planes (no conversion).
-  __device__ void pstash(S& s) const {
+  // A fetched k-tile into the LDS planes (no conversion).
+  __device__ static void pstash_from(const Stage& st, S& s) {
     const int tid = Base::ltid();
 #pragma unroll
     for (int c = 0; c < AU; ++c) {
       const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U), g = e / U, q = e - g * U;
-      *reinterpret_cast<u32x4*>(&s.a[q][r][g * 8]) = pa[c];
+      *reinterpret_cast<u32x4*>(&s.a[q][r][g * 8]) = st.a[c];
     }
 #pragma unroll
     for (int c = 0; c < BU; ++c) {
       const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U), g = e / U, q = e - g * U;
-      *reinterpret_cast<u32x4*>(&s.b[q][r][g * 8]) = pb[c];
+      *reinterpret_cast<u32x4*>(&s.b[q][r][g * 8]) = st.b[c];
     }
   }
+  __device__ void pstash(S& s) const { pstash_from(cur, s); }
 
   // acc += A B over KG groups (a multiple of 4) from group 0 of A's rows and
   // B's columns.

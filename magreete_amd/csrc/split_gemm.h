@@ -73,8 +73,9 @@ struct Tile {
   static constexpr int APASS = BM / AROWS;
   static constexpr int BGRP = NT / BN;          // k-groups of 8 staged per pass of B
   static constexpr int BPASS = BK / (8 * BGRP);
-  static_assert(BM % AROWS == 0 && BK % (8 * BGRP) == 0 && NT % BN == 0, "staging layout");
   typedef Smem<NS, BM, BN> S;
+  // the complex-float staging of fetch / stash (pre-split operands stage their own way)
+  static constexpr bool kFloatStaging = BM % AROWS == 0 && BK % (8 * BGRP) == 0 && NT % BN == 0;
 
   v4f cr[TM][TN], ci[TM][TN];
 
@@ -135,6 +136,7 @@ struct Tile {
   template <bool COHERENT_B = false>
   __device__ void fetch(const float2* __restrict__ A, int64_t lda, const float2* B, int64_t ldb, int64_t M, int64_t N,
                         int64_t K, int64_t k0) {
+    static_assert(kFloatStaging, "staging layout");
     const int tid = ltid();
     const int ar = tid / (BK / 8), ak = (tid % (BK / 8)) * 8;
     const int bc = tid % BN, bk = (tid / BN) * 8;
@@ -168,6 +170,7 @@ struct Tile {
 
   // The fetched k-tile, split into bf16 planes, into LDS.
   __device__ void stash(S& s) const {
+    static_assert(kFloatStaging, "staging layout");
     const int tid = ltid();
     const int ar = tid / (BK / 8), ak = (tid % (BK / 8)) * 8;
     const int bc = tid % BN, bk = (tid / BN) * 8;
