@@ -419,13 +419,20 @@ int panel_split(int64_t M, int64_t N, int64_t K, const float2* A, int64_t lda, c
   return MG_OK;
 }
 
-// Trailing update of the split factorisation, C -= A_0 B_0 (+ A_1 B_1) with
-// every A (M rows) and B (N columns) pre-split over one 256-block of k (the
-// panels of one or two steps); C complex float row-major.  The accumulators
-// start from -C (loaded before the k-loop).
+// Up to four (A, B) operand pairs of the split trailing update.
+constexpr int kMaxSrc = 4;
+struct UpdSrc {
+  const u32x4* A[kMaxSrc];
+  const u32x4* B[kMaxSrc];
+  int n;
+};
+
+// Trailing update of the split factorisation, C -= sum_s A_s B_s with every A
+// (M rows) and B (N columns) pre-split over one 256-block of k (the panels of
+// one to four steps); C complex float row-major.  The accumulators start from
+// -C (loaded before the k-loop).
 template <int NS, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void k_ldu_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0,
-                                                             const u32x4* A1, const u32x4* B1, int nsrc,
+__global__ __launch_bounds__(64 * WM * WN) void k_ldu_update(int64_t M, int64_t N, UpdSrc src,
                                                              float2* __restrict__ C, int64_t ldc, int nbm, int nbn) {
   typedef mg_ps::PTile<NS, BM, BN, WM, WN> Tile;
   constexpr int U = 2 * NS, KG = kIB / 8;
@@ -433,34 +440,38 @@ __global__ __launch_bounds__(64 * WM * WN) void k_ldu_update(int64_t M, int64_t 
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
   const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
-  const u32x4* A[2] = {A0 + i0 * KG * U, A1 + i0 * KG * U};
-  const u32x4* B[2] = {B0 + j0 * KG * U, B1 + j0 * KG * U};
+  const u32x4* A[kMaxSrc];
+  const u32x4* B[kMaxSrc];
+#pragma unroll
+  for (int q = 0; q < kMaxSrc; ++q) {
+    A[q] = src.A[q] + i0 * KG * U;
+    B[q] = src.B[q] + j0 * KG * U;
+  }
   Tile t;
   t.init(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), true);
-  t.pmac_n(sm, A, B, nsrc, KG, M - i0, N - j0, KG);
+  t.pmac_n(sm, A, B, src.n, KG, M - i0, N - j0, KG);
   t.store(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
 
 template <int NS, int BM, int BN, int WM, int WN>
-int launch_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0, const u32x4* A1, const u32x4* B1, int nsrc,
-                  float2* C, int64_t ldc, hipStream_t s) {
+int launch_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
   const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
-  hipLaunchKernelGGL((k_ldu_update<NS, BM, BN, WM, WN>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N, A0, B0,
-                     A1 ? A1 : A0, B1 ? B1 : B0, nsrc, C, ldc, nbm, nbn);
+  hipLaunchKernelGGL((k_ldu_update<NS, BM, BN, WM, WN>), dim3(nbm * nbn), dim3(64 * WM * WN), 0, s, M, N, src, C, ldc,
+                     nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
 
-// C -= A_0 B_0 (+ A_1 B_1: nsrc = 2).  BF16x3: 64 x 128 tiles (two
-// workgroups per CU; 12032^2 x 256 measured 1.07 ms against 1.21 ms with
-// 128 x 128); BF16x6: 64 x 64 (its 64 x 128 tile's LDS would leave one
-// workgroup per CU, and its 128 x 128 tile spills registers).
+// C -= sum_s A_s B_s.  BF16x3: 64 x 128 tiles (two workgroups per CU;
+// 12032^2 x 256 measured 1.07 ms against 1.21 ms with 128 x 128); BF16x6:
+// 64 x 64 (its 64 x 128 tile's LDS would leave one workgroup per CU, and its
+// 128 x 128 tile spills registers).
 template <int NS>
-int ldu_update(int64_t M, int64_t N, const u32x4* A0, const u32x4* B0, const u32x4* A1, const u32x4* B1, int nsrc,
-               float2* C, int64_t ldc, hipStream_t s) {
+int ldu_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return MG_OK;
-  if constexpr (NS == 2) return launch_update<NS, 64, 128, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
-  else return launch_update<NS, 64, 64, 2, 2>(M, N, A0, B0, A1, B1, nsrc, C, ldc, s);
+  if (src.n < 1 || src.n > kMaxSrc) return MG_ERR_ARG;
+  if constexpr (NS == 2) return launch_update<NS, 64, 128, 2, 2>(M, N, src, C, ldc, s);
+  else return launch_update<NS, 64, 64, 2, 2>(M, N, src, C, ldc, s);
 }
 
 constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
@@ -755,24 +766,21 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     return timed_update(ws, gt, mr, nc, h, cd{-1, 0}, s21(k) + (i0 - r1) * h, h, Ap + r0 * lda + j0, lda, cd{1, 0},
                         Ap + i0 * lda + j0, lda, st);
   };
-  // split precisions: A22[rows i0.., cols j0..] -= Lh_k A12_k (+ Lh_{k+1} A12_{k+1} when both), pre-split
-  auto trail_ps = [&](int64_t k, bool both, int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) -> int {
+  // split precisions: A22[rows i0.., cols j0..] -= sum over steps k .. k+cnt-1 of Lh A12, pre-split
+  auto trail_ps = [&](int64_t k, int cnt, int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) -> int {
     if (mr <= 0 || nc <= 0) return MG_OK;
     constexpr int64_t KG = kIB / 8;
-    const u32x4* A[2];
-    const u32x4* B[2];
-    const int nsrc = both ? 2 : 1;
-    for (int q = 0; q < nsrc; ++q) {
-      const int64_t r1 = (k + q + 1) * nb;
-      A[q] = SPL + lay.off(k + q) + (i0 - r1) * KG * U;
-      B[q] = spa12(k + q) + (j0 - r1) * KG * U;
+    UpdSrc src;
+    src.n = cnt;
+    for (int q = 0; q < kMaxSrc; ++q) {
+      const int64_t kq = k + (q < cnt ? q : 0), r1 = (kq + 1) * nb;
+      src.A[q] = SPL + lay.off(kq) + (i0 - r1) * KG * U;
+      src.B[q] = spa12(kq) + (j0 - r1) * KG * U;
     }
-    const double flops = 8.0 * (double)mr * (double)nc * (double)(nsrc * nb);
+    const double flops = 8.0 * (double)mr * (double)nc * (double)(cnt * nb);
     return timed_launch(ws, flops, st, [&] {
-      return split_ns == 2 ? ldu_update<2>(mr, nc, A[0], B[0], nsrc > 1 ? A[1] : nullptr, nsrc > 1 ? B[1] : nullptr,
-                                           nsrc, Ap + i0 * lda + j0, lda, st)
-                           : ldu_update<3>(mr, nc, A[0], B[0], nsrc > 1 ? A[1] : nullptr, nsrc > 1 ? B[1] : nullptr,
-                                           nsrc, Ap + i0 * lda + j0, lda, st);
+      return split_ns == 2 ? ldu_update<2>(mr, nc, src, Ap + i0 * lda + j0, lda, st)
+                           : ldu_update<3>(mr, nc, src, Ap + i0 * lda + j0, lda, st);
     });
   };
   if (!split_ns || !ws.s2 || !ws.s3) {
@@ -786,7 +794,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
       if (r == 0) break;
       const int64_t h2 = (r < nb) ? r : nb;   // next block's order
       auto upd = [&](int64_t i0, int64_t mr, int64_t j0, int64_t nc, hipStream_t st) {
-        return split_ns ? trail_ps(k, false, i0, mr, j0, nc, st) : trail(k, i0, mr, j0, nc, st);
+        return split_ns ? trail_ps(k, 1, i0, mr, j0, nc, st) : trail(k, i0, mr, j0, nc, st);
       };
       MG_TRY(upd(r1, h2, r1, r, s));
       MG_TRY(upd(r1 + h2, r - h2, r1, h2, s));
@@ -807,45 +815,63 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     return MG_OK;
   }
   // Split precisions, in super-steps of two blocks (k, k+1).  The second
-  // stream factors a super-step's panels: inverse and panels of block k, the
-  // update of block row and column k+1 by step k, inverse and panels of block
-  // k+1.  The caller's stream then updates the next super-step's block rows
-  // and columns (k+2, k+3) by both steps at once (k = 512 products), after
-  // which the next super-step's panels start on the second stream while the
-  // rest of the trailing matrix is updated, again by both steps at once: one
-  // read and write of the trailing matrix per two steps.  The two streams
-  // touch disjoint blocks.  Inside a super-step, only the diagonal block
-  // S_{k+1} is updated by step k before its inversion; the rest of block row
-  // and column k+1 is updated on the third stream during that inversion.
-  // evA: a super-step's panels done; evB: the next super-step's block rows
-  // and columns done; evC / evD: step k's panels / block row and column k+1
-  // done (third stream).
-  auto chain = [&](int64_t k, hipStream_t st) -> int {
-    MG_TRY(inverse(k, st));
+  // stream carries the critical path: the inverse and panels of block k, the
+  // update of the next diagonal block S_{k+1} by step k, the inverse and
+  // panels of block k+1; the rest of block row and column k+1 is updated on
+  // the third stream during that inversion.  The caller's stream updates the
+  // next super-step's block rows and columns (k+2, k+3) by both steps at once
+  // (k = 512 products) while the second stream updates S_{k+2} alone and
+  // inverts it; the next super-step's panels follow, and the rest of the
+  // trailing matrix is updated beside them, again by both steps at once: one
+  // read and write of the trailing matrix per two steps.  Concurrent launches
+  // touch disjoint blocks.  evA: a super-step's panels done; evB: the next
+  // super-step's block rows and columns done; evC / evD: step k's panels /
+  // block row and column k+1 done (third stream); evE: the next diagonal
+  // block S_{k+4} updated by the super-step (third stream).
+  auto chain_tail = [&](int64_t k, hipStream_t st) -> int {   // after inverse(k)
     MG_TRY(panel_rest(k, st));
     if (k + 1 >= nblk) return MG_OK;
     const int64_t r1 = (k + 1) * nb, h2 = (n - r1 < nb) ? n - r1 : nb, r2 = r1 + h2;
-    MG_TRY(trail_ps(k, false, r1, h2, r1, h2, st));       // S_{k+1}
+    MG_TRY(trail_ps(k, 1, r1, h2, r1, h2, st));           // S_{k+1}
     MG_HIP(hipEventRecord(ws.evC, st));
     MG_HIP(hipStreamWaitEvent(ws.s3, ws.evC, 0));
-    MG_TRY(trail_ps(k, false, r1, h2, r2, n - r2, ws.s3));   // block row k+1 right of S_{k+1}
-    MG_TRY(trail_ps(k, false, r2, n - r2, r1, h2, ws.s3));   // block column k+1 below it
+    MG_TRY(trail_ps(k, 1, r1, h2, r2, n - r2, ws.s3));   // block row k+1 right of S_{k+1}
+    MG_TRY(trail_ps(k, 1, r2, n - r2, r1, h2, ws.s3));   // block column k+1 below it
     MG_HIP(hipEventRecord(ws.evD, ws.s3));
     MG_TRY(inverse(k + 1, st));
     MG_HIP(hipStreamWaitEvent(st, ws.evD, 0));
     return panel_rest(k + 1, st);
   };
-  MG_TRY(chain(0, s));
+  MG_TRY(inverse(0, s));
+  MG_TRY(chain_tail(0, s));
+  MG_HIP(hipEventRecord(ws.evA, s));
   for (int64_t k = 0; k + 2 < nblk; k += 2) {
     const int64_t b2 = (k + 2) * nb, w = (n - b2 < 2 * nb) ? n - b2 : 2 * nb, b4 = b2 + w;
-    MG_TRY(trail_ps(k, true, b2, w, b2, n - b2, s));      // block rows k+2, k+3
-    MG_TRY(trail_ps(k, true, b4, n - b4, b2, w, s));      // block columns k+2, k+3 below them
+    const int64_t h3 = (n - b2 < nb) ? n - b2 : nb, b3 = b2 + h3;   // block k+2
+    const int64_t h5 = (n - b4 < nb) ? n - b4 : nb, b5 = b4 + h5;   // block k+4
+    // S_{k+2}: steps k and k+1 (steps k-2 and k-1, whose rest left it out,
+    // were applied on the third stream during the previous rest)
+    MG_HIP(hipStreamWaitEvent(ws.s2, ws.evA, 0));
+    if (k >= 2) MG_HIP(hipStreamWaitEvent(ws.s2, ws.evE, 0));
+    MG_TRY(trail_ps(k, 2, b2, h3, b2, h3, ws.s2));
+    MG_TRY(inverse(k + 2, ws.s2));
+    MG_TRY(trail_ps(k, 2, b2, h3, b3, n - b3, s));       // block row k+2 right of S_{k+2}
+    MG_TRY(trail_ps(k, 2, b3, b4 - b3, b2, n - b2, s));   // block row k+3
+    MG_TRY(trail_ps(k, 2, b4, n - b4, b2, w, s));         // block columns k+2, k+3 below them
     MG_HIP(hipEventRecord(ws.evB, s));
     MG_HIP(hipStreamWaitEvent(ws.s2, ws.evB, 0));
-    MG_TRY(chain(k + 2, ws.s2));
+    MG_TRY(chain_tail(k + 2, ws.s2));
     MG_HIP(hipEventRecord(ws.evA, ws.s2));
-    MG_TRY(trail_ps(k, true, b4, n - b4, b4, n - b4, s));   // the rest
+    // the rest, without S_{k+4}, whose update by these two steps runs beside
+    // it on the third stream (evE) so that the next super-step's chain only
+    // adds its own two steps there
+    MG_TRY(trail_ps(k, 2, b4, h5, b5, n - b5, s));
+    MG_TRY(trail_ps(k, 2, b5, n - b5, b4, n - b4, s));
+    MG_HIP(hipStreamWaitEvent(ws.s3, ws.evB, 0));
+    MG_TRY(trail_ps(k, 2, b4, h5, b4, h5, ws.s3));
+    MG_HIP(hipEventRecord(ws.evE, ws.s3));
     MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));
+    MG_HIP(hipStreamWaitEvent(s, ws.evE, 0));
   }
   return MG_OK;
 }

@@ -39,7 +39,7 @@ struct Workspace {
   UpdateProfile prof;
   hipStream_t s2 = nullptr;        // look-ahead panel stream (default priority)
   hipStream_t s3 = nullptr;        // a third stream beside the look-ahead (default priority)
-  hipEvent_t evA = nullptr, evB = nullptr, evC = nullptr, evD = nullptr;
+  hipEvent_t evA = nullptr, evB = nullptr, evC = nullptr, evD = nullptr, evE = nullptr;
   float2* inv_exch = nullptr;      // LDU block inverse: two 256 x 32 published slabs
   int* inv_ctl = nullptr;          // LDU block inverse: barrier counter [0] and failure flag [32]
   int* sweep = nullptr;            // ticket + per-block-row counters of the LDU solve sweeps

@@ -570,6 +570,7 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipEventCreateWithFlags(&ws.evB, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evC, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evD, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evE, hipEventDisableTiming));
   ws.ready = true;
   return MG_OK;
 }
@@ -609,6 +610,7 @@ void workspace_free(Workspace& ws) {
   if (ws.evB) (void)hipEventDestroy(ws.evB);
   if (ws.evC) (void)hipEventDestroy(ws.evC);
   if (ws.evD) (void)hipEventDestroy(ws.evD);
+  if (ws.evE) (void)hipEventDestroy(ws.evE);
   ws = Workspace();
 }
 
