@@ -101,7 +101,9 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
  * info: first exactly-zero pivot or 0.  The preconditioner of the
  * mixed-precision refinement (no counterpart in the reference).
  * mg_getrs_ldu solves with it by GEMMs only (work:
- * mg_ldu_solve_work_elems(prec, n, nrhs) elements, 16-byte aligned). */
+ * mg_ldu_solve_work_elems(prec, n, nrhs) elements, 16-byte aligned); the
+ * split precisions read dinv alone, so their LDU may be null (the factorised
+ * matrix need not be kept). */
 int64_t mg_ldu_dinv_elems(int64_t n);
 int64_t mg_ldu_factor_elems(int prec, int64_t n);
 int64_t mg_ldu_work_elems(int64_t n);

@@ -130,7 +130,7 @@ class _Base:
     # -- solve / propagate -----------------------------------------------------
     def _solve(self, k0, alpha, radius, E0, self_interaction, si_type, flat):
         F = self._factor(k0, alpha, radius, self_interaction, si_type)
-        dev = F.A.device
+        dev = F.device
         vec = E0.dim() == 1 and flat
         B = E0.reshape(self.d * self.N, -1).to(device=dev, dtype=torch.complex128).contiguous().clone()
         F.solve(B)
@@ -175,7 +175,7 @@ class _Base:
     def _ldos_dots(self, F, pts, k0, radius, self_interaction, si_type, regularize, chunk=None):
         """Row sums diag(G_p W G_p^T) for the measurement points, by solving
         M X = G_p^T with the factor F and contracting X with G_p^T column by column."""
-        dev = F.A.device
+        dev = F.device
         pts = pts.to(device=dev, dtype=torch.float64).contiguous()
         fill = self._self_int(k0, radius, si_type) / self._volume(radius) if self_interaction else 0.0
         flags = L.MG_F_NANFILL | (L.MG_F_REGULARIZE if regularize else 0) | self._bf()
@@ -206,7 +206,7 @@ class _Base:
               pref):
         F = self._factor(k0, alpha, radius, self_interaction, si_type)
         dots = self._ldos_dots(F, measure_points, k0, radius, self_interaction, si_type, regularize)
-        E.check_barriers(F.A.device)
+        E.check_barriers(F.device)
         self._raise_singular(F.info)
         return _like(self._ldos_finish(dots, measure_points.shape[0], pref, alpha, discard_absorption),
                      measure_points)
@@ -219,7 +219,7 @@ class _Base:
             pts = batches[b]
             dots = self._ldos_dots(F, pts, k0, radius, self_interaction, si_type, regularize)
             outs.append(self._ldos_finish(dots, pts.shape[0], pref, alpha, discard_absorption))
-        E.check_barriers(F.A.device)
+        E.check_barriers(F.device)
         self._raise_singular(F.info)
         return _like(torch.cat(outs), batches[0] if len(batches) else None)
 
@@ -227,7 +227,7 @@ class _Base:
                   discard_absorption, pref):
         F = self._factor(k0, alpha, radius, self_interaction, si_type)
         dots = self._ldos_dots(F, measure_points, k0, radius, self_interaction, si_type, regularize)
-        E.check_barriers(F.A.device)
+        E.check_barriers(F.device)
         self._raise_singular(F.info)
         alpha_ = onp.real(alpha) if discard_absorption else alpha
         dos = dots.sum() / measure_points.shape[0]

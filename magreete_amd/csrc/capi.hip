@@ -187,7 +187,8 @@ int mg_getrf_ldu(mg_handle* h, int prec, void* A, int64_t n, int64_t lda, void* 
 
 int mg_getrs_ldu(mg_handle* h, int prec, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B,
                  int64_t ldb, int64_t nrhs, void* work, void* stream) {
-  if (!h || !LDU || !dinv || !B || !work) return MG_ERR_ARG;
+  // the split precisions solve from dinv's D inverses and pre-split panels alone (LDU may be null)
+  if (!h || !dinv || !B || !work || (!LDU && prec == MG_C64)) return MG_ERR_ARG;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   switch (prec) {
     case MG_C64: return mg::getrs_ldu(&h->ws, LDU, n, lda, dinv, B, ldb, nrhs, work, s, nullptr, 0);

@@ -180,6 +180,7 @@ class Factor:
     def __init__(self, A, prec="c128", pivot=True):
         lib = L.load()
         self.A, self.prec, self.n, self.pivot = A, prec, A.shape[0], pivot
+        self.dtype, self.device = A.dtype, A.device
         dev = A.device
         if pivot:
             self.ipiv, self.info = getrf(A, prec)
@@ -201,6 +202,11 @@ class Factor:
                                       _p(self.info), _stream(dev))
             L.check(st, "mg_getrf_ldu")
             del work          # stream-ordered: the factorisation's last step waits on this stream
+            if prec in ("bf16x3", "bf16x6"):
+                # the solves read the pre-split panels and the D inverses only:
+                # the factorised matrix's storage may go (the caller's copy is
+                # the caller's to keep or free)
+                self.A = None
         self._work = None
 
     def work_elems(self, nrhs):
@@ -215,21 +221,22 @@ class Factor:
         caller (solves that run at the same time on one factor each bring
         their own); default: a buffer kept by the factor."""
         lib = L.load()
-        dev = self.A.device
+        dev = self.device
         need = self.work_elems(B.shape[1])
         if work is None:
             if self._work is None or self._work.numel() < need:
-                self._work = torch.empty(need, dtype=self.A.dtype, device=dev)
+                self._work = torch.empty(need, dtype=self.dtype, device=dev)
             work = self._work
-        elif work.numel() < need or work.dtype != self.A.dtype:
-            raise ValueError("work needs %d elements of %s" % (need, self.A.dtype))
+        elif work.numel() < need or work.dtype != self.dtype:
+            raise ValueError("work needs %d elements of %s" % (need, self.dtype))
         with torch.cuda.device(dev):
             if self.pivot:
                 st = lib.mg_getrs_dinv(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A),
                                        _p(self.ipiv), _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(work),
                                        _stream(dev))
             else:
-                st = lib.mg_getrs_ldu(handle(dev), PRECS[self.prec], _p(self.A), self.n, _ld(self.A),
+                lda = _ld(self.A) if self.A is not None else self.n
+                st = lib.mg_getrs_ldu(handle(dev), PRECS[self.prec], _p(self.A), self.n, lda,
                                       _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(work), _stream(dev))
         L.check(st, "mg_getrs")
         return B

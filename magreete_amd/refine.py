@@ -48,7 +48,7 @@ class _Krylov:
         with self._ctx():
             self.bnorm = bnorm.contiguous()
             self.lowp = torch.empty((n, T), dtype=torch.complex64, device=dev)
-            self.fwork = torch.empty(F.work_elems(T), dtype=F.A.dtype, device=dev)
+            self.fwork = torch.empty(F.work_elems(T), dtype=F.dtype, device=dev)
             m_max = max(1, min(restart, max_iters))
             self.work = torch.empty(L.load().mg_mdot_work_elems(m_max + 1, T), dtype=torch.complex128, device=dev)
             self.est_buf = torch.empty(1, dtype=torch.float64, device=dev)
@@ -195,13 +195,19 @@ class MixedSolver:
 
     def _factor(self, fresh=False):
         if self.tier == "fp64":
+            self.M32 = None
             self.F = E.Factor(self.M.clone(), "c128")
             return
+        if self.M32 is None:
+            self.M32 = torch.empty(self.M.shape, dtype=torch.complex64, device=self.M.device)
+            fresh = False
         if not fresh:
             E.convert(self.M, self.M32)                    # the previous tier overwrote M32
         self.F = E.Factor(self.M32, self.prec, pivot=(self.tier == "pivoted"))
         if self.tier == "nopiv" and int(self.F.info.item()) != 0:
             self._escalate()                               # exact zero pivot: pivot this time
+        elif self.F.A is None:
+            self.M32 = None                                # the split factor does not read it again
 
     def _escalate(self):
         self.tier = self.TIERS[self.TIERS.index(self.tier) + 1]

@@ -123,16 +123,28 @@ def workload(req):
 
 def resident_bytes(n, method):
     """Bytes that stay resident for a factorised system of order n: M
-    (complex128) and the factors (complex128 LU with its diagonal-block
-    inverses, or the complex64 block LDU, its D inverses and, for the split
-    precisions, its panels stored split into bf16 parts: 4 bytes per part and
-    complex element, Lh and Uh)."""
+    (complex128) and the factors: the complex128 LU with its diagonal-block
+    inverses; the complex64 block LDU with its D inverses; or, for the split
+    precisions, the D inverses and the panels stored split into bf16 parts
+    (4 bytes per part and complex element, Lh and Uh), the complex64 working
+    copy being released after the factorisation."""
     nblk = (n + 255) // 256
     if method == "fp64":
         return 16 * n * n + 16 * 256 * 256 * 2 * nblk
     parts = {"bf16x3": 2, "bf16x6": 3}.get(method, 0)
-    split = 2 * 4 * parts * 256 * 256 * nblk * (nblk - 1) // 2
-    return 16 * n * n + 8 * n * n + 8 * 256 * 256 * nblk + split
+    if parts:
+        return 16 * n * n + 8 * 256 * 256 * nblk + 2 * 4 * parts * 256 * 256 * nblk * (nblk - 1) // 2
+    return 16 * n * n + 8 * n * n + 8 * 256 * 256 * nblk
+
+
+def factor_bytes(n, method):
+    """Bytes a factorisation needs while it runs, beyond its resident result:
+    the split precisions' complex64 working copy, and the LDU workspace."""
+    if method == "fp64":
+        return 8 * 256 * n * 4
+    np_ = 256 * ((n + 255) // 256)
+    work = 8 * (4 * 256 * n + 2 * 256 * 256 + 4 * np_ * 32 * 6 * 2)
+    return work + (8 * n * n if method in ("bf16x3", "bf16x6") else 0)
 
 
 def transient_bytes(n, T, n_meas, method, restart=12):
@@ -220,8 +232,9 @@ class Server:
                     continue                                   # its factor is in use or on its way
                 need = transient_bytes(r.n, r.T, r.d * r.n_measure, self.method)
                 if ent is None:
-                    need += resident_bytes(r.n, self.method)
+                    need += resident_bytes(r.n, self.method) + factor_bytes(r.n, self.method)
                 if state["reserved"] + need > self.budget:
+                    evicted = False
                     for k in list(entries.keys()):           # evict idle resident factors, oldest first
                         if state["reserved"] + need <= self.budget:
                             break
@@ -230,6 +243,11 @@ class Server:
                             e.done.synchronize()
                             state["reserved"] -= e.bytes
                             del entries[k]
+                            evicted = True
+                    if evicted:
+                        # hand the evicted factors' memory back to the device, so that
+                        # allocations outside torch's cache (the runtime's) find room
+                        torch.cuda.empty_cache()
                 if state["reserved"] + need > self.budget:
                     if not state["reserved"] and not entries:
                         raise MemoryError("request %d needs %.1f GB, over the %.1f GB budget" % (

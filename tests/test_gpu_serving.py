@@ -54,8 +54,8 @@ def test_mix_matches_fp64(trace, mix):
 
 def test_admission_within_budget(trace):
     say("serving budget")
-    big = max(S.resident_bytes(r.n, "bf16x3") + S.transient_bytes(r.n, r.T, r.d * r.n_measure, "bf16x3")
-              for r in trace)
+    big = max(S.resident_bytes(r.n, "bf16x3") + S.factor_bytes(r.n, "bf16x3")
+              + S.transient_bytes(r.n, r.T, r.d * r.n_measure, "bf16x3") for r in trace)
     srv = S.Server(DEV, workers=4, method="bf16x3", hbm_budget=1.5 * big)
     res = srv.run(trace)
     assert all(r is not None for r in res)
