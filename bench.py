@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--batch-tol", type=float, default=1e-10,
                     help="refinement tolerance of the ensemble and serving measurements (their smaller, "
                          "differently conditioned systems reach a 1e-9 error only at a tighter residual)")
+    ap.add_argument("--no-relax", action="store_true",
+                    help="every refinement product in FP64 (default: the late products on split-bf16 MFMA once "
+                         "the residual allows their error, refine.RELAX_LEVELS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ensemble", type=int, default=256,
@@ -101,7 +104,7 @@ def main():
     outputs = {}
     for method in [m for m in args.methods.split(",") if m]:
         try:
-            s = Solver(prob, method=method, tol=args.tol)
+            s = Solver(prob, method=method, tol=args.tol, relax=not args.no_relax)
             for _ in range(args.warmup):
                 s.run()
             torch.cuda.synchronize()
@@ -197,7 +200,7 @@ def main():
         "data": "synthetic (seeded uniform ball, Fibonacci directions)",
         "config": {"workload": "3d_vector N=%d (n=%d) T=%d (%d dirs x 2 pol) measure=%d" % (
             args.n_points, n, w.T, args.n_dirs, args.n_measure), "method": best, "k0": w.k0,
-            "seed": args.seed, "tol": args.tol},
+            "seed": args.seed, "tol": args.tol, "relaxed_products": not args.no_relax},
         "roofline": roof, "roofline_fp64": roofline("fp64") if best != "fp64" else None,
         "cpu_baseline": cpu, "methods": results, "errors": errors,
         "ensemble": ens, "serving": srv, "ldos_lattice": ldos,

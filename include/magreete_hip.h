@@ -175,6 +175,22 @@ int mg_givens(void* H, void* H0, void* cs, void* sn, void* g, const double* hn, 
               int64_t T, double* est, void* stream);
 int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
              int64_t ldw, int64_t n, int64_t T, void* stream);
+/* Relaxed-accuracy Krylov products of the refinement (no counterpart in the
+ * reference): once the GMRES residual has dropped to r, a product with a
+ * relative error of about tol / r moves the final residual by about tol, so
+ * the last products of a solve may run on split-bf16 MFMA.
+ * mg_split_rows: A (rows x cols, lda; prec MG_C128 or MG_C64) -> AS
+ * (mg_split_rows_bytes bytes, 16-byte aligned), each element rounded to
+ * complex float and held as three bf16 parts of its real and imaginary part.  mg_relaxed_product: C (m x n complex128, ldc) = A B with
+ * A given as AS (m x k) and B (k x n complex128, ldb); prec MG_BF16X3 (three
+ * bf16 products per real product, ~16 bits) or MG_BF16X6 (six, ~24 bits);
+ * work: mg_relaxed_product_work_bytes(prec, m, n, k) bytes, 16-byte aligned.
+ * k = 0 is MG_ERR_ARG. */
+int64_t mg_split_rows_bytes(int64_t rows, int64_t cols);
+int mg_split_rows(int prec, const void* A, int64_t lda, int64_t rows, int64_t cols, void* AS, void* stream);
+int64_t mg_relaxed_product_work_bytes(int prec, int64_t m, int64_t n, int64_t k);
+int mg_relaxed_product(int prec, int64_t m, int64_t n, int64_t k, const void* AS, const void* B, int64_t ldb,
+                       void* C, int64_t ldc, void* work, int64_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

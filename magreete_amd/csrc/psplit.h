@@ -80,15 +80,25 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
   // 4 U consecutive units, and consecutive threads take consecutive units, so
   // every load instruction reads contiguous memory across the wave.
   // COHERENT_B: B with agent-scope loads (B written by other workgroups of
-  // this launch).
-  template <bool COHERENT_B = false>
+  // this launch).  SNA: parts per real / imaginary part stored in A (at
+  // least NS); its first NS parts of each are read (A split once into more
+  // parts than a product uses).
+  template <bool COHERENT_B = false, int SNA = NS>
   __device__ static void pfetch_to(Stage& st, const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb,
                                    int64_t M, int64_t N, int64_t g0) {
+    static_assert(SNA >= NS, "A holds fewer parts than the product uses");
     const int tid = Base::ltid();
 #pragma unroll
     for (int c = 0; c < AU; ++c) {
       const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U);
-      st.a[c] = r < M ? A[((int64_t)r * ldga + g0) * U + e] : u32x4{0u, 0u, 0u, 0u};
+      int64_t src;
+      if constexpr (SNA == NS) {
+        src = ((int64_t)r * ldga + g0) * U + e;
+      } else {
+        const int g = e / U, q = e - g * U;
+        src = ((int64_t)r * ldga + g0 + g) * (2 * SNA) + (q / NS) * SNA + q % NS;
+      }
+      st.a[c] = r < M ? A[src] : u32x4{0u, 0u, 0u, 0u};
     }
 #pragma unroll
     for (int c = 0; c < BU; ++c) {
@@ -102,10 +112,10 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
       }
     }
   }
-  template <bool COHERENT_B = false>
+  template <bool COHERENT_B = false, int SNA = NS>
   __device__ void pfetch(const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
                          int64_t N, int64_t g0) {
-    pfetch_to<COHERENT_B>(cur, A, ldga, B, ldgb, M, N, g0);
+    pfetch_to<COHERENT_B, SNA>(cur, A, ldga, B, ldgb, M, N, g0);
   }
 
   // A fetched k-tile into the LDS planes (no conversion).
@@ -125,17 +135,17 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
   __device__ void pstash(S& s) const { pstash_from(cur, s); }
 
   // acc += A B over KG groups (a multiple of 4) from group 0 of A's rows and
-  // B's columns.
-  template <bool COHERENT_B = false>
+  // B's columns (A holding SNA parts).
+  template <bool COHERENT_B = false, int SNA = NS>
   __device__ void pmac(S& sm, const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
                        int64_t N, int64_t KG) {
     const int64_t KT = KG / 4;
-    pfetch<COHERENT_B>(A, ldga, B, ldgb, M, N, 0);
+    pfetch<COHERENT_B, SNA>(A, ldga, B, ldgb, M, N, 0);
     for (int64_t kt = 0; kt < KT; ++kt) {
       __syncthreads();   // previous tile's reads are done
       pstash(sm);
       __syncthreads();
-      if (kt + 1 < KT) pfetch<COHERENT_B>(A, ldga, B, ldgb, M, N, (kt + 1) * 4);
+      if (kt + 1 < KT) pfetch<COHERENT_B, SNA>(A, ldga, B, ldgb, M, N, (kt + 1) * 4);
       this->compute(sm);
     }
   }

@@ -285,6 +285,39 @@ def cgemm(A, B, C, alpha=1.0, beta=0.0, prec="c128", split=False):
     return C
 
 
+def split_rows(A):
+    """A (complex128 or complex64, row-major) split once for relaxed_product:
+    every element rounded to complex float and held as three bf16 parts of
+    its real and imaginary part (a uint8 buffer, 12 bytes per element)."""
+    lib = L.load()
+    dev = A.device
+    rows, cols = A.shape
+    prec = L.MG_C128 if A.dtype == torch.complex128 else L.MG_C64
+    AS = torch.empty(int(lib.mg_split_rows_bytes(rows, cols)), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_split_rows(prec, _p(A), _ld(A), rows, cols, _p(AS), _stream(dev))
+    L.check(st, "mg_split_rows")
+    return AS
+
+
+def relaxed_product(AS, k, B, C, prec):
+    """C <- A B (complex128) with A (C.shape[0] x k) given by split_rows and
+    the products on split-bf16 MFMA (prec 'bf16x3': relative error ~1e-5,
+    'bf16x6': ~1e-7); the Krylov products that may be inexact."""
+    lib = L.load()
+    dev = C.device
+    m, n = C.shape
+    nb = int(lib.mg_relaxed_product_work_bytes(PRECS[prec], m, n, k))
+    if nb < 0:
+        raise ValueError("relaxed products are bf16x3 or bf16x6, not %r" % prec)
+    work = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_relaxed_product(PRECS[prec], m, n, k, _p(AS), _p(B), _ld(B), _p(C), _ld(C), _p(work), nb,
+                                    _stream(dev))
+    L.check(st, "mg_relaxed_product")
+    return C
+
+
 def convert(src, dst):
     lib = L.load()
     dev = src.device

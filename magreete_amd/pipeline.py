@@ -42,10 +42,12 @@ class Problem:
 class Solver:
     """Buffers for repeated solves of one problem shape on one device.
 
-    method: 'fp64' (complex128 LU), 'c64' or 'bf16x3' (complex64 LU with
-    FP64 GMRES-IR, falling back to the FP64 LU when refinement stalls)."""
+    method: 'fp64' (complex128 LU), 'c64', 'bf16x3' or 'bf16x6' (complex64
+    LU with FP64 GMRES-IR, falling back to the FP64 LU when refinement
+    stalls).  relax: the refinement's late products on split-bf16 MFMA
+    (refine.MixedSolver)."""
 
-    def __init__(self, prob: Problem, method="fp64", tol=1e-12, max_iters=30):
+    def __init__(self, prob: Problem, method="fp64", tol=1e-12, max_iters=30, relax=False):
         self.p = prob
         self.method = method
         dev = prob.points.device
@@ -58,6 +60,7 @@ class Solver:
             self.out = torch.empty((prob.d * prob.measure.shape[0], T), dtype=torch.complex128, device=dev)
         self.tol = tol
         self.max_iters = max_iters
+        self.relax = relax
         self.stats = {}
 
     def assemble(self):
@@ -77,7 +80,7 @@ class Solver:
             self.stats = {"iters": 0, "fallback": False}
             return self.X
         st = refine.solve_mixed(self.M, self.M32, p.E0s, self.X, prec=self.method, tol=self.tol,
-                                max_iters=self.max_iters)
+                                max_iters=self.max_iters, relax=self.relax)
         self.info = st.pop("info")
         self.stats = st
         return self.X

@@ -25,6 +25,19 @@ import torch
 from . import _lib as L
 from . import engine as E
 
+# Relaxed Krylov products (relax=True): once the residual estimate has
+# dropped to r, the next product M z may carry a relative error eps and the
+# final residual moves by about eps r (the error enters multiplied by that
+# direction's coefficient, of the size of r; inexact Krylov methods,
+# Simoncini and Szyld 2003, Bouras and Fraysse 2005).  A product takes the
+# cheapest split-bf16 level whose error bound times r stays within
+# RELAX_SAFETY tol, else FP64.  The bounds are twice the relative errors
+# measured against the FP64 product on the 3D vector workload with N = 4096
+# (BF16x3 1.4e-5, BF16x6 4.4e-6, for directions z = P^{-1} v whose product
+# cancels about 95x: || |M| |z| || / || M z ||).
+RELAX_LEVELS = (("bf16x3", 3e-5), ("bf16x6", 1e-5))
+RELAX_SAFETY = 0.2
+
 
 class _Krylov:
     """One flexible-GMRES refinement process over a block of columns (with
@@ -39,8 +52,10 @@ class _Krylov:
     rounding of the FP64 products already taken, at the cost of one pass over
     the basis instead of another product."""
 
-    def __init__(self, M, F, B, X, bnorm, tol, max_iters, restart, stream, handle, history):
+    def __init__(self, M, F, B, X, bnorm, tol, max_iters, restart, stream, handle, history, relax=None):
         self.M, self.F, self.B, self.X = M, F, B, X
+        self.relax = relax          # M split by E.split_rows (relaxed products), or None
+        self.products = []          # precision of every product taken
         self.tol, self.max_iters, self.restart = tol, max_iters, restart
         self.stream, self.handle, self.history = stream, handle, history
         dev = M.device
@@ -83,6 +98,7 @@ class _Krylov:
                 self.ok = False
                 return False
             self.prev = rel
+            self.res = rel
             self.cycles += 1
             n, T = self.B.shape
             dev = self.M.device
@@ -107,13 +123,28 @@ class _Krylov:
             self.F.solve(self.lowp, work=self.fwork)
             E.convert(self.lowp, self.Z[self.j])
 
+    def product_prec(self):
+        """Precision of the next product: FP64, or with relaxed products the
+        cheapest level whose error bound times the current residual estimate
+        is within RELAX_SAFETY tol."""
+        if self.relax is not None:
+            for prec, eps in RELAX_LEVELS:
+                if eps * self.res <= RELAX_SAFETY * self.tol:
+                    return prec
+        return "fp64"
+
     def product(self):
-        """w = M z_j (FP64), classical Gram-Schmidt twice, the Arnoldi column
-        and its Givens rotations (enqueued)."""
+        """w = M z_j (FP64 or relaxed), classical Gram-Schmidt twice, the
+        Arnoldi column and its Givens rotations (enqueued)."""
         j = self.j
+        prec = self.product_prec()
+        self.products.append(prec)
         with self._ctx():
             w = self.V[j + 1]
-            E.cgemm(self.M, self.Z[j], w, alpha=1.0, beta=0.0, split=True)
+            if prec == "fp64":
+                E.cgemm(self.M, self.Z[j], w, alpha=1.0, beta=0.0, split=True)
+            else:
+                E.relaxed_product(self.relax, self.M.shape[1], self.Z[j], w, prec)
             for _ in range(2):
                 h = E.mdot(self.V[: j + 1], w, work=self.work)
                 E.maxpy(self.V[: j + 1], h, w, alpha=-1.0)
@@ -129,6 +160,7 @@ class _Krylov:
         tolerance or the basis is full.  True when the cycle ended."""
         with self._ctx():
             est = self.est_buf.item()   # |g_{j+1}| / ||b||: the Arnoldi residual formed below
+            self.res = est
             if self.history is not None:
                 self.history.append(est)
             self.j += 1
@@ -149,12 +181,14 @@ class _Krylov:
             return True
 
 
-def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None):
+def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=None, products=None):
     """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F,
     all columns as one batched process on the current stream.
     history: optional list that receives the largest per-column relative
-    residual estimate after every FP64 product.
-    Returns (converged, FP64 products, cycles, residual).
+    residual estimate after every product.  relax: M split by E.split_rows
+    (relaxed products), or None.  products: optional list that receives the
+    precision of every product.
+    Returns (converged, products, cycles, residual).
 
     Measured and rejected: two half-width processes on two streams whose
     preconditioner sweeps alternate beside the other half's FP64 product.
@@ -163,13 +197,16 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None):
     for the full-width pair in sequence, i.e. 6.8 ms per iteration: the
     sweeps' waiting workgroups hold CUs that the product's one-per-CU
     workgroups then cannot use."""
-    K = _Krylov(M, F, B, X, bnorm, tol, max_iters, restart, torch.cuda.current_stream(M.device), None, history)
+    K = _Krylov(M, F, B, X, bnorm, tol, max_iters, restart, torch.cuda.current_stream(M.device), None, history,
+                relax)
     while K.start_cycle():
         while True:
             K.precond()
             K.product()
             if K.after_product():
                 break
+    if products is not None:
+        products.extend(K.products)
     return K.ok, K.iters, K.cycles, K.rel
 
 
@@ -181,21 +218,29 @@ class MixedSolver:
     at the unpivoted block LDU ('nopiv', or 'pivoted' with pivot=True); a
     solve whose refinement breaks down or stalls moves the solver to the
     next tier (complex64 LU with partial pivoting, then complex128 LU) for
-    this and every later solve."""
+    this and every later solve.
+
+    relax=True: the refinement's late products run on split-bf16 MFMA when
+    the residual allows (RELAX_LEVELS); M is split once, from M32 before the
+    factorisation (12 bytes per element, resident beside M).  The residual
+    then holds to (1 + RELAX_SAFETY) tol while the levels' error bounds
+    hold."""
 
     TIERS = ("nopiv", "pivoted", "fp64")
 
-    def __init__(self, M, M32, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False):
+    def __init__(self, M, M32, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False, relax=False):
         self.M, self.M32, self.prec = M, M32, prec
         self.tol, self.max_iters, self.restart = tol, max_iters, restart
         self.tier = "pivoted" if pivot else "nopiv"
         self.fell = False
         self.F = None
+        self.relax = relax
+        self.AS = None           # M split for the relaxed products
         self._factor(fresh=True)
 
     def _factor(self, fresh=False):
         if self.tier == "fp64":
-            self.M32 = None
+            self.M32 = self.AS = None
             self.F = E.Factor(self.M.clone(), "c128")
             return
         if self.M32 is None:
@@ -203,6 +248,8 @@ class MixedSolver:
             fresh = False
         if not fresh:
             E.convert(self.M, self.M32)                    # the previous tier overwrote M32
+        if self.relax and self.AS is None:
+            self.AS = E.split_rows(self.M32)               # before the factors overwrite it
         self.F = E.Factor(self.M32, self.prec, pivot=(self.tier == "pivoted"))
         if self.tier == "nopiv" and int(self.F.info.item()) != 0:
             self._escalate()                               # exact zero pivot: pivot this time
@@ -220,19 +267,21 @@ class MixedSolver:
 
     def solve(self, B, X):
         """X <- M^{-1} B to a relative FP64 residual <= tol per column.
-        Returns stats: tier, fallback, iters (FP64 products), cycles,
-        residual."""
+        Returns stats: tier, fallback, iters (products), cycles, residual,
+        history (residual estimate after every product), products (the
+        precision of every product)."""
         bnorm = torch.linalg.norm(B, dim=0)
         bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
         total = 0
         hist = []
+        prods = []
         while self.tier != "fp64":
             ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart,
-                                               hist)
+                                               hist, self.AS, prods)
             total += iters
             if ok:
                 return {"tier": self.tier, "fallback": self.fell, "iters": total, "cycles": cycles,
-                        "residual": rel, "history": hist}
+                        "residual": rel, "history": hist, "products": prods}
             self._escalate()
         # refinement stalled in low precision: complex128 LU
         X.copy_(B)
@@ -243,11 +292,11 @@ class MixedSolver:
         return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel}
 
 
-def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False):
+def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False, relax=False):
     """Solve M X = B to a relative FP64 residual <= tol (one MixedSolver,
     one solve).  Returns stats: tier ('nopiv', 'pivoted' or 'fp64'), iters
-    (FP64 products), cycles (GMRES restarts + 1), residual, fallback, info."""
-    ms = MixedSolver(M, M32, prec, tol=tol, max_iters=max_iters, restart=restart, pivot=pivot)
+    (products), cycles (GMRES restarts + 1), residual, fallback, info."""
+    ms = MixedSolver(M, M32, prec, tol=tol, max_iters=max_iters, restart=restart, pivot=pivot, relax=relax)
     st = ms.solve(B, X)
     st["info"] = ms.info
     return st

@@ -150,6 +150,52 @@ def test_stall_falls_back_to_fp64():
     assert colerr(X.cpu(), torch.from_numpy(ref)) < 1e-12
 
 
+@pytest.mark.parametrize("prec,bound", [("bf16x3", 3e-5), ("bf16x6", 1e-5)])
+@pytest.mark.parametrize("m,k,T", [(1000, 1000, 37), (777, 1301, 256), (64, 8, 1), (300, 5000, 130)])
+def test_relaxed_product(m, k, T, prec, bound):
+    """Split-bf16 Krylov product against the FP64 product: the relative error
+    of every column within the level's bound (refine.RELAX_LEVELS), ragged
+    m, k and T, a strided right-hand side; A split from complex128 and from
+    its complex64 rounding gives the same bytes."""
+    say("relaxed product", m, k, T, prec)
+    assert dict(refine.RELAX_LEVELS)[prec] == bound
+    rng = np.random.default_rng(m + k + T)
+    A = crand((m, k), rng)
+    Zb = crand((k, T + 3), rng)
+    ref = A @ Zb[:, :T]
+    Ad = torch.from_numpy(A).to(DEV)
+    AS = E.split_rows(Ad)
+    assert torch.equal(AS, E.split_rows(Ad.to(torch.complex64)))
+    Z = torch.from_numpy(Zb).to(DEV)[:, :T]
+    C = torch.full((m, T), float("nan"), dtype=torch.complex128, device=DEV)
+    E.relaxed_product(AS, k, Z, C, prec)
+    x = C.cpu().numpy()
+    err = (np.linalg.norm(x - ref, axis=0) / np.linalg.norm(ref, axis=0)).max()
+    assert err < bound / 2, err
+
+
+@pytest.mark.parametrize("method", ["bf16x3", "bf16x6"])
+def test_relaxed_refinement(method):
+    """Refinement with relaxed late products: the last products leave FP64,
+    the true FP64 residual stays within (1 + RELAX_SAFETY) tol and the
+    solution within 1e-9 of the FP64 path."""
+    say("relaxed refinement", method)
+    w, prob = _problem(npts=800, ndirs=16)
+    s64 = Solver(prob, "fp64")
+    s64.run()
+    X64 = s64.X.clone()
+    tol = 1e-9
+    sm = Solver(prob, method, tol=tol, relax=True)
+    sm.run()
+    st = sm.stats
+    assert st["tier"] == "nopiv" and not st["fallback"], st
+    assert st["products"][0] == "fp64" and st["products"][-1] != "fp64", st
+    R = prob.E0s - sm.M @ sm.X
+    rel = (torch.linalg.norm(R, dim=0) / torch.linalg.norm(prob.E0s, dim=0)).max().item()
+    assert rel <= (1 + refine.RELAX_SAFETY) * tol, (rel, st)
+    assert colerr(sm.X, X64) < 1e-9
+
+
 @pytest.mark.parametrize("j", [0, 1, 4])
 def test_givens_kernel_matches_torch(j):
     """The fused Arnoldi-column kernel against the same rotations written as
