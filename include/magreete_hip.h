@@ -136,6 +136,15 @@ int mg_convert(int dir, const void* src, int64_t lds, void* dst, int64_t ldd, in
 int mg_propagate(int kind, const double* meas, int64_t n_meas, const double* scat, int64_t n_scat, double k0,
                  double alpha_re, double alpha_im, int flags, double radius, const void* Ek, int64_t ld_ek,
                  int64_t nrhs, double beta_re, double beta_im, void* E, int64_t ld_e, void* stream);
+/* mg_propagate with the scatterers split over a workspace of
+ * mg_propagate_work_bytes(kind, n_meas, n_scat, nrhs) bytes (16-byte
+ * aligned; 0: no split), so that a grid of few output tiles still gives every
+ * CU two workgroups; the parts are added in a fixed order. */
+int64_t mg_propagate_work_bytes(int kind, int64_t n_meas, int64_t n_scat, int64_t nrhs);
+int mg_propagate_ws(int kind, const double* meas, int64_t n_meas, const double* scat, int64_t n_scat, double k0,
+                    double alpha_re, double alpha_im, int flags, double radius, const void* Ek, int64_t ld_ek,
+                    int64_t nrhs, double beta_re, double beta_im, void* E, int64_t ld_e, void* work,
+                    int64_t work_bytes, void* stream);
 
 /* H_nu^(1)(x) for nu in {0, 1} (torch_hankel1, Transmission2D.py:46-61, 578-593);
  * exact = 0: the reference's torch values, 1: accurate to ~1e-15. */
@@ -175,6 +184,11 @@ int mg_givens(void* H, void* H0, void* cs, void* sn, void* g, const double* hn, 
               int64_t T, double* est, void* stream);
 int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
              int64_t ldw, int64_t n, int64_t T, void* stream);
+/* mg_maxpy followed by the column norms of the updated w (no reference
+ * counterpart): norms[t] = ||w[:, t]||_2 (T doubles, summed in a fixed
+ * order); work: mg_mdot_work_elems(1, T) complex128 elements. */
+int mg_maxpy_norm(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
+                  int64_t ldw, int64_t n, int64_t T, void* work, double* norms, void* stream);
 /* Relaxed-accuracy Krylov products of the refinement (no counterpart in the
  * reference): once the GMRES residual has dropped to r, a product with a
  * relative error of about tol / r moves the final residual by about tol, so

@@ -47,6 +47,9 @@ SIGNATURES = {
     "mg_convert": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _vp],
     "mg_propagate": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
                      _vp, _i64, _vp],
+    "mg_propagate_work_bytes": [_int, _i64, _i64, _i64],
+    "mg_propagate_ws": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
+                        _vp, _i64, _vp, _i64, _vp],
     "mg_hankel1": [_int, _int, _vp, _i64, _vp, _vp],
     "mg_coldot": [_vp, _i64, _vp, _i64, _i64, _i64, _vp, _vp],
     "mg_sources": [_int, _int, _vp, _i64, _vp, _i64, _vp, _i64, _dbl, _dbl, _vp, _vp],
@@ -60,6 +63,7 @@ SIGNATURES = {
     "mg_mdot_work_elems": [_i64, _i64],
     "mg_mdot": [_vp, _i64, _i64, _int, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _vp],
     "mg_maxpy": [_vp, _i64, _i64, _int, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _vp],
+    "mg_maxpy_norm": [_vp, _i64, _i64, _int, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
     "mg_split_rows_bytes": [_i64, _i64],
     "mg_split_rows": [_int, _vp, _i64, _i64, _i64, _vp, _vp],
     "mg_relaxed_product_work_bytes": [_int, _i64, _i64, _i64],
@@ -68,7 +72,8 @@ SIGNATURES = {
 _RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64, "mg_cgemm_work_elems": _c.c_int64,
              "mg_mdot_work_elems": _c.c_int64, "mg_ldu_dinv_elems": _c.c_int64, "mg_ldu_work_elems": _c.c_int64,
              "mg_ldu_factor_elems": _c.c_int64, "mg_ldu_solve_work_elems": _c.c_int64,
-             "mg_split_rows_bytes": _c.c_int64, "mg_relaxed_product_work_bytes": _c.c_int64}
+             "mg_split_rows_bytes": _c.c_int64, "mg_relaxed_product_work_bytes": _c.c_int64,
+             "mg_propagate_work_bytes": _c.c_int64}
 
 _lib = None
 

@@ -59,18 +59,79 @@ __global__ __launch_bounds__(256) void k_mdot_part(const double2* __restrict__ V
   }
 }
 
-__global__ void k_mdot_reduce(const double2* __restrict__ part, int k, int64_t T, int nchunks,
-                              double2* __restrict__ out, int64_t ldo) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)k * T) return;
-  const int64_t i = e / T, t = e - i * T;
+// out[i ldo + t] = sum over chunks of part: one workgroup per vector i and 64
+// columns, its 4 waves summing interleaved chunks, combined in a fixed order.
+__global__ __launch_bounds__(256) void k_mdot_reduce(const double2* __restrict__ part, int k, int64_t T, int nchunks,
+                                                     double2* __restrict__ out, int64_t ldo) {
+  __shared__ double2 red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int i = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 64 + tx;
   double2 s = make_double2(0.0, 0.0);
-  for (int c = 0; c < nchunks; ++c) {
-    const double2 p = part[((int64_t)c * k + i) * T + t];
-    s.x += p.x;
-    s.y += p.y;
+  if (t < T) {
+    for (int c = ty; c < nchunks; c += 4) {
+      const double2 p = part[((int64_t)c * k + i) * T + t];
+      s.x += p.x;
+      s.y += p.y;
+    }
   }
-  out[i * ldo + t] = s;
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && t < T) {
+    double2 o = red[0][tx];
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+      o.x += red[q][tx].x;
+      o.y += red[q][tx].y;
+    }
+    out[i * ldo + t] = o;
+  }
+}
+
+// w += alpha sum_i V_i h_i (the arithmetic of k_maxpy) over row chunk
+// blockIdx.x and 64 columns, with part[c T + t] = that chunk's sum of
+// |w[r, t]|^2 after the update.
+__global__ __launch_bounds__(256) void k_maxpy_norm(const double2* __restrict__ V, int64_t sV, int64_t ldv, int k,
+                                                    const double2* __restrict__ h, int64_t ldh, double alpha,
+                                                    double2* __restrict__ w, int64_t ldw, int64_t n, int64_t T,
+                                                    int64_t rows, double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t c = blockIdx.x, t = (int64_t)blockIdx.y * 64 + tx;
+  const int64_t r0 = c * rows, r1 = (r0 + rows < n) ? r0 + rows : n;
+  double acc = 0.0;
+  if (t < T) {
+    for (int64_t r = r0 + ty; r < r1; r += 4) {
+      double2 s = make_double2(0.0, 0.0);
+      for (int i = 0; i < k; ++i) {
+        const double2 v = V[(int64_t)i * sV + r * ldv + t], q = h[(int64_t)i * ldh + t];
+        s.x += v.x * q.x - v.y * q.y;
+        s.y += v.x * q.y + v.y * q.x;
+      }
+      double2 o = w[r * ldw + t];
+      o.x += alpha * s.x;
+      o.y += alpha * s.y;
+      w[r * ldw + t] = o;
+      acc += o.x * o.x + o.y * o.y;
+    }
+  }
+  red[ty][tx] = acc;
+  __syncthreads();
+  if (ty == 0 && t < T) part[c * T + t] = ((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx];
+}
+
+// out[t] = sqrt(sum over chunks of part[c T + t]) (fixed order, as k_mdot_reduce)
+__global__ __launch_bounds__(256) void k_norm_reduce(const double* __restrict__ part, int64_t T, int nchunks,
+                                                     double* __restrict__ out) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t t = (int64_t)blockIdx.x * 64 + tx;
+  double s = 0.0;
+  if (t < T)
+    for (int c = ty; c < nchunks; c += 4) s += part[(int64_t)c * T + t];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && t < T) out[t] = sqrt(((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx]);
 }
 
 __global__ __launch_bounds__(256) void k_maxpy(const double2* __restrict__ V, int64_t sV, int64_t ldv, int k,
@@ -174,7 +235,7 @@ int mg_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* w, int64_
                      reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(w), ldw, n, T,
                      rows, reinterpret_cast<double2*>(work));
   MG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_mdot_reduce, dim3((unsigned)((k * T + 255) / 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_mdot_reduce, dim3((unsigned)((T + 63) / 64), (unsigned)k), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(work), k, T, kChunks, reinterpret_cast<double2*>(out), ldo);
   MG_HIP(hipGetLastError());
   return MG_OK;
@@ -188,6 +249,23 @@ int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64
   hipLaunchKernelGGL(k_maxpy, dim3((unsigned)((n * T + 255) / 256)), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(h), ldh, alpha,
                      reinterpret_cast<double2*>(w), ldw, n, T);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+int mg_maxpy_norm(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
+                  int64_t ldw, int64_t n, int64_t T, void* work, double* norms, void* stream) {
+  if (!w || !work || !norms || (k > 0 && (!V || !h)) || k < 0 || n < 0 || T < 0 || ldv < T || ldw < T || ldh < T)
+    return MG_ERR_ARG;
+  if (T == 0) return MG_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int64_t rows = (n + kChunks - 1) / kChunks;
+  double* part = reinterpret_cast<double*>(work);
+  hipLaunchKernelGGL(k_maxpy_norm, dim3(kChunks, (unsigned)((T + 63) / 64)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(h), ldh, alpha,
+                     reinterpret_cast<double2*>(w), ldw, n, T, rows, part);
+  MG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)((T + 63) / 64)), dim3(256), 0, s, part, T, kChunks, norms);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }

@@ -94,13 +94,13 @@ __device__ inline void inv_barrier(unsigned* bar, unsigned target, int* err) {
 // in its MFMA accumulators for the whole inversion.  Panel p:
 //   slab p, published by its owner to exch (coherent stores), gives the
 //   pivot block P (rows 32p..) and the column panel C;
-//   wave 0 of every workgroup inverts P in registers (FP32, lane = column,
-//   rows through cross-lane shuffles);
+//   every workgroup inverts P in LDS (FP32 Gauss-Jordan, all threads);
 //   R = P^{-1} X_p (the slab's pivot rows; R = P^{-1} on slab p) becomes the
 //   slab's pivot rows, and every other row gets X_i <- X_i - C_i R (slab p:
 //   X_i <- -C_i P^{-1}) as BF16x3 MFMA products.
 // One barrier per panel; zero pivots are reported in info (first, 1-based).
-__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
+// (kSlabs workgroups only: one wave per SIMD, the registers of all four.)
+__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
                                                               float2* __restrict__ X, int* info, int64_t row0,
                                                               float2* exch, unsigned* bar, int* err) {
   __shared__ InvSmem sm;
@@ -137,37 +137,40 @@ __global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(2)))
 
   for (int p = 0; p < kSlabs; ++p) {
     const float2* C = exch + (p & 1) * (kIB * kPW);   // slab p: 256 x 32, row stride 32
-    // (1) P^{-1}: lane (j = lane % 32, half hh) holds P[16 hh + t][j], t = 0..15
-    if (wave == 0) {
-      const int j = lane & 31, hh = lane >> 5;
-      float2 v[16];
+    // (1) P^{-1} by Gauss-Jordan in LDS, all threads: thread (row pi, columns
+    // pj .. pj + 3); two barriers per pivot step (reads, then writes)
+    {
+      const int pi = tid >> 3, pj = (tid & 7) * 4;
 #pragma unroll
-      for (int t = 0; t < 16; ++t) v[t] = mg_sg::coherent_load(&C[(kPW * p + 16 * hh + t) * kPW + j]);
-#pragma unroll
+      for (int e = 0; e < 4; ++e) sm.pf[pi][pj + e] = mg_sg::coherent_load(&C[(kPW * p + pi) * kPW + pj + e]);
+      __syncthreads();
       for (int st = 0; st < kPW; ++st) {
-        const int hs = st >> 4, ts = st & 15;
-        const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[ts].x), hs * 32 + st));
-        const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[ts].y), hs * 32 + st));
-        if (q == 0 && lane == 0 && px == 0.f && py == 0.f && kPW * p + st < h)
-          atomicCAS(info, 0, (int)(row0 + kPW * p + st + 1));
-        const float rd = __builtin_amdgcn_rcpf(px * px + py * py);
-        const float2 inv = make_float2(px * rd, -py * rd);
-        const float2 rowv = make_float2(__shfl(v[ts].x, hs * 32 + j), __shfl(v[ts].y, hs * 32 + j));
-        const float2 u = (j == st) ? inv : cmulf(rowv, inv);   // row st of the result
+        const float2 piv = sm.pf[st][st], ci = sm.pf[pi][st];
+        float2 rj[4], xv[4];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const float2 cv = make_float2(__shfl(v[t].x, hh * 32 + st), __shfl(v[t].y, hh * 32 + st));
-          const float2 tq = cmulf(cv, (j == st) ? inv : u);
-          const int i = 16 * hh + t;
-          v[t] = (i == st) ? u : (j == st) ? make_float2(-tq.x, -tq.y) : make_float2(v[t].x - tq.x, v[t].y - tq.y);
+        for (int e = 0; e < 4; ++e) {
+          rj[e] = sm.pf[st][pj + e];
+          xv[e] = sm.pf[pi][pj + e];
         }
+        if (q == 0 && tid == 0 && piv.x == 0.f && piv.y == 0.f && kPW * p + st < h)
+          atomicCAS(info, 0, (int)(row0 + kPW * p + st + 1));
+        const float rd = __builtin_amdgcn_rcpf(piv.x * piv.x + piv.y * piv.y);
+        const float2 inv = make_float2(piv.x * rd, -piv.y * rd);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int j = pj + e;
+          const float2 u = (j == st) ? inv : cmulf(rj[e], inv);   // row st of the result
+          const float2 tq = cmulf(ci, (j == st) ? inv : u);
+          sm.pf[pi][j] = (pi == st) ? u : (j == st) ? make_float2(-tq.x, -tq.y)
+                                                   : make_float2(xv[e].x - tq.x, xv[e].y - tq.y);
+        }
+        __syncthreads();
       }
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int i = 16 * hh + t;
-        sm.pf[i][j] = v[t];
-        split_planes(v[t], &sm.pa[0][i][j], &sm.pa[1][i][j], &sm.pa[2][i][j], &sm.pa[3][i][j]);
-      }
+      for (int e = 0; e < 4; ++e)
+        split_planes(sm.pf[pi][pj + e], &sm.pa[0][pi][pj + e], &sm.pa[1][pi][pj + e], &sm.pa[2][pi][pj + e],
+                     &sm.pa[3][pi][pj + e]);
     }
     // the slab's pivot rows (wave pw, row tiles pa0 and pa0 + 1) as B operand
     const int pw = p >> 1, pa0 = 2 * (p & 1);

@@ -330,16 +330,20 @@ def convert(src, dst):
 
 
 def propagate(kind, meas, scat, k0, alpha, Ek, out, flags=0, radius=0.0, beta=1.0):
-    """out <- alpha * G(meas, scat) @ Ek + beta * out, G evaluated in-kernel."""
+    """out <- alpha * G(meas, scat) @ Ek + beta * out, G evaluated in-kernel
+    (the scatterers split over a workspace from torch's allocator when the
+    output tiles alone would leave CUs idle)."""
     lib = L.load()
     dev = out.device
     meas = _pts(meas, dev)
     scat = _pts(scat, dev)
     alpha, beta = complex(alpha), complex(beta)
+    nb = int(lib.mg_propagate_work_bytes(KINDS[kind], meas.shape[0], scat.shape[0], Ek.shape[1]))
+    work = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
     with torch.cuda.device(dev):
-        st = lib.mg_propagate(KINDS[kind], _p(meas), meas.shape[0], _p(scat), scat.shape[0], float(k0),
-                              alpha.real, alpha.imag, int(flags), float(radius), _p(Ek), _ld(Ek), Ek.shape[1],
-                              beta.real, beta.imag, _p(out), _ld(out), _stream(dev))
+        st = lib.mg_propagate_ws(KINDS[kind], _p(meas), meas.shape[0], _p(scat), scat.shape[0], float(k0),
+                                 alpha.real, alpha.imag, int(flags), float(radius), _p(Ek), _ld(Ek), Ek.shape[1],
+                                 beta.real, beta.imag, _p(out), _ld(out), _p(work), nb, _stream(dev))
     L.check(st, "mg_propagate")
     return out
 
@@ -392,6 +396,23 @@ def maxpy(V, h, w, alpha=1.0):
         st = lib.mg_maxpy(pv, sv, ldv, k, _p(h), _ld(h), float(alpha), _p(w), _ld(w), n, T, _stream(dev))
     L.check(st, "mg_maxpy")
     return w
+
+
+def maxpy_norm(V, h, w, norms, alpha=1.0, work=None):
+    """maxpy, then norms[t] = ||w[:, t]|| of the updated w (fixed summation
+    order); work: optional complex128 buffer of mg_mdot_work_elems(1, T)."""
+    lib = L.load()
+    dev = w.device
+    k, n, T = V.shape
+    need = lib.mg_mdot_work_elems(1, T)
+    if work is None or work.numel() < need:
+        work = torch.empty(max(need, 1), dtype=torch.complex128, device=dev)
+    pv, sv, ldv = _vecs(V)
+    with torch.cuda.device(dev):
+        st = lib.mg_maxpy_norm(pv, sv, ldv, k, _p(h), _ld(h), float(alpha), _p(w), _ld(w), n, T, _p(work),
+                               _p(norms), _stream(dev))
+    L.check(st, "mg_maxpy_norm")
+    return norms
 
 
 def givens(H, H0, cs, sn, g, hn, bnorm, j, est):

@@ -26,17 +26,41 @@ from . import _lib as L
 from . import engine as E
 
 # Relaxed Krylov products (relax=True): once the residual estimate has
-# dropped to r, the next product M z may carry a relative error eps and the
-# final residual moves by about eps r (the error enters multiplied by that
-# direction's coefficient, of the size of r; inexact Krylov methods,
-# Simoncini and Szyld 2003, Bouras and Fraysse 2005).  A product takes the
-# cheapest split-bf16 level whose error bound times r stays within
-# RELAX_SAFETY tol, else FP64.  The bounds are twice the relative errors
-# measured against the FP64 product on the 3D vector workload with N = 4096
-# (BF16x3 1.4e-5, BF16x6 4.4e-6, for directions z = P^{-1} v whose product
-# cancels about 95x: || |M| |z| || / || M z ||).
-RELAX_LEVELS = (("bf16x3", 3e-5), ("bf16x6", 1e-5))
-RELAX_SAFETY = 0.2
+# dropped to r, the next product M z may carry an error and still leave the
+# solution within about eps r of the FP64-product one, where eps is what that
+# product level adds to the solution per unit of residual: the error enters
+# the residual multiplied by that direction's coefficient, which is of the
+# size of r (inexact Krylov methods: Simoncini and Szyld 2003, Bouras and
+# Fraysse 2005), and the solution through M^{-1}.  A product takes the
+# cheapest split-bf16 level with eps r <= RELAX_SAFETY tol, else FP64.  eps
+# is twice what each level added on the 3D vector workload with N = 4096
+# (BF16x3 1.8e-4, BF16x6 1.3e-5, measured as the solution's deviation from
+# the all-FP64 refinement divided by r); it exceeds the products' own
+# relative errors there (BF16x3 1.4e-5, BF16x6 4.4e-6 against the FP64
+# product), so the true residual also stays within (1 + RELAX_SAFETY) tol.
+RELAX_LEVELS = (("bf16x3", 4e-4), ("bf16x6", 3e-5))
+RELAX_SAFETY = 0.5
+
+
+class _SplitA:
+    """M split for the relaxed products (E.split_rows), made on a side stream
+    so that it runs beside the first iterations (the preconditioner sweeps
+    leave CUs idle); get() makes the caller's stream wait for it."""
+
+    def __init__(self, M):
+        dev = M.device
+        main = torch.cuda.current_stream(dev)
+        self.side = torch.cuda.Stream(dev)
+        self.side.wait_stream(main)            # M is assembled
+        with torch.cuda.stream(self.side):
+            self.AS = E.split_rows(M)
+            self.ready = torch.cuda.Event()
+            self.ready.record(self.side)
+        self.AS.record_stream(main)
+
+    def get(self):
+        torch.cuda.current_stream(self.AS.device).wait_event(self.ready)
+        return self.AS
 
 
 class _Krylov:
@@ -54,7 +78,7 @@ class _Krylov:
 
     def __init__(self, M, F, B, X, bnorm, tol, max_iters, restart, stream, handle, history, relax=None):
         self.M, self.F, self.B, self.X = M, F, B, X
-        self.relax = relax          # M split by E.split_rows (relaxed products), or None
+        self.relax = relax          # _SplitA (relaxed products), or None
         self.products = []          # precision of every product taken
         self.tol, self.max_iters, self.restart = tol, max_iters, restart
         self.stream, self.handle, self.history = stream, handle, history
@@ -67,6 +91,7 @@ class _Krylov:
             m_max = max(1, min(restart, max_iters))
             self.work = torch.empty(L.load().mg_mdot_work_elems(m_max + 1, T), dtype=torch.complex128, device=dev)
             self.est_buf = torch.empty(1, dtype=torch.float64, device=dev)
+            self.hn = torch.empty(T, dtype=torch.float64, device=dev)
             X.zero_()
             self.R = B.clone(memory_format=torch.contiguous_format)
             self.beta = torch.linalg.norm(self.R, dim=0)
@@ -144,12 +169,15 @@ class _Krylov:
             if prec == "fp64":
                 E.cgemm(self.M, self.Z[j], w, alpha=1.0, beta=0.0, split=True)
             else:
-                E.relaxed_product(self.relax, self.M.shape[1], self.Z[j], w, prec)
-            for _ in range(2):
+                E.relaxed_product(self.relax.get(), self.M.shape[1], self.Z[j], w, prec)
+            hn = self.hn
+            for it in range(2):
                 h = E.mdot(self.V[: j + 1], w, work=self.work)
-                E.maxpy(self.V[: j + 1], h, w, alpha=-1.0)
+                if it == 0:
+                    E.maxpy(self.V[: j + 1], h, w, alpha=-1.0)
+                else:
+                    E.maxpy_norm(self.V[: j + 1], h, w, hn, alpha=-1.0, work=self.work)
                 self.H[: j + 1, j] += h
-            hn = torch.linalg.norm(w, dim=0)
             w /= torch.where(hn == 0, torch.ones_like(hn), hn)
             E.givens(self.H, self.H0, self.cs, self.sn, self.g, hn, self.bnorm, j, self.est_buf)
         self.iters += 1
@@ -185,8 +213,8 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=No
     """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F,
     all columns as one batched process on the current stream.
     history: optional list that receives the largest per-column relative
-    residual estimate after every product.  relax: M split by E.split_rows
-    (relaxed products), or None.  products: optional list that receives the
+    residual estimate after every product.  relax: a _SplitA of M (relaxed
+    products), or None.  products: optional list that receives the
     precision of every product.
     Returns (converged, products, cycles, residual).
 
@@ -221,9 +249,9 @@ class MixedSolver:
     this and every later solve.
 
     relax=True: the refinement's late products run on split-bf16 MFMA when
-    the residual allows (RELAX_LEVELS); M is split once, from M32 before the
-    factorisation (12 bytes per element, resident beside M).  The residual
-    then holds to (1 + RELAX_SAFETY) tol while the levels' error bounds
+    the residual allows (RELAX_LEVELS); M is split once, at the first solve,
+    on a side stream (12 bytes per element, resident beside M).  The
+    residual then holds to (1 + RELAX_SAFETY) tol while the levels' bounds
     hold."""
 
     TIERS = ("nopiv", "pivoted", "fp64")
@@ -235,7 +263,7 @@ class MixedSolver:
         self.fell = False
         self.F = None
         self.relax = relax
-        self.AS = None           # M split for the relaxed products
+        self.AS = None           # _SplitA of M for the relaxed products
         self._factor(fresh=True)
 
     def _factor(self, fresh=False):
@@ -248,8 +276,6 @@ class MixedSolver:
             fresh = False
         if not fresh:
             E.convert(self.M, self.M32)                    # the previous tier overwrote M32
-        if self.relax and self.AS is None:
-            self.AS = E.split_rows(self.M32)               # before the factors overwrite it
         self.F = E.Factor(self.M32, self.prec, pivot=(self.tier == "pivoted"))
         if self.tier == "nopiv" and int(self.F.info.item()) != 0:
             self._escalate()                               # exact zero pivot: pivot this time
@@ -276,6 +302,8 @@ class MixedSolver:
         hist = []
         prods = []
         while self.tier != "fp64":
+            if self.relax and self.AS is None:
+                self.AS = _SplitA(self.M)
             ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart,
                                                hist, self.AS, prods)
             total += iters

@@ -224,20 +224,34 @@ def test_getrf_singular_info():
 
 
 @pytest.mark.parametrize("kind", ["2d_tm", "2d_te", "3d_vector", "3d_scalar"])
-def test_propagate_kernel_vs_oracle(kind):
-    say("propagate", kind)
+@pytest.mark.parametrize("nscat", [45, 1601])
+def test_propagate_kernel_vs_oracle(kind, nscat):
+    """Fused propagation against the oracle's G @ Ek; with 1601 scatterers the
+    scatterers are split over a workspace, whose parts must add up to the
+    unsplit launch."""
+    say("propagate", kind, nscat)
     rng = np.random.default_rng(4)
     sp = 2 if kind.startswith("2d") else 3
     d = {"2d_tm": 1, "2d_te": 2, "3d_vector": 3, "3d_scalar": 1}[kind]
-    scat = torch.from_numpy(rng.uniform(-1, 1, size=(45, sp)))
+    scat = torch.from_numpy(rng.uniform(-1, 1, size=(nscat, sp)))
     meas = torch.from_numpy(rng.uniform(-2, 2, size=(29, sp)))
     k0 = 7.5
     T = 70
-    Ek = torch.from_numpy(crand((d * 45, T), rng))
+    Ek = torch.from_numpy(crand((d * nscat, T), rng))
     E0 = torch.from_numpy(crand((d * 29, T), rng))
     alpha = 0.3 - 0.1j
     out = E0.clone().to(DEV)
     E.propagate(kind, meas.to(DEV), scat.to(DEV), k0, alpha, Ek.to(DEV), out)
+    lib = L.load()
+    nb = lib.mg_propagate_work_bytes(E.KINDS[kind], 29, nscat, T)
+    assert (nb > 0) == (nscat > 1000), nb
+    whole = E0.clone().to(DEV)
+    md, sd, ekd = meas.to(DEV).contiguous(), scat.to(DEV).contiguous(), Ek.to(DEV)
+    st = lib.mg_propagate(E.KINDS[kind], md.data_ptr(), 29, sd.data_ptr(), nscat, k0, alpha.real, alpha.imag, 0, 0.0,
+                          ekd.data_ptr(), T, T, 1.0, 0.0, whole.data_ptr(), T,
+                          torch.cuda.current_stream().cuda_stream)
+    assert st == 0
+    assert ((out - whole).abs().max() / whole.abs().max()).item() < 1e-14
     cls = O.CLASSES[kind](scat, source='plane')
     if kind == "2d_te":
         G = cls.G0_vector(meas, k0)
@@ -261,6 +275,13 @@ def test_krylov_mdot_maxpy(k, n, T):
     ref = np.einsum("int,nt->it", V[:k].conj(), w)
     assert np.abs(h.cpu().numpy() - ref).max() < 1e-12 * np.abs(ref).max() + 1e-13
     assert torch.equal(h, E.mdot(Vd, wd))
+    w2 = wd.clone()
     E.maxpy(Vd, h, wd, alpha=-0.5)
     ref_w = w - 0.5 * np.einsum("int,it->nt", V[:k], ref)
     assert np.abs(wd.cpu().numpy() - ref_w).max() < 1e-12 * np.abs(ref_w).max()
+    # the fused form: the same update bit for bit, and the column norms
+    norms = torch.empty(T, dtype=torch.float64, device=DEV)
+    E.maxpy_norm(Vd, h, w2, norms, alpha=-0.5)
+    assert torch.equal(w2, wd)
+    ref_n = np.linalg.norm(ref_w, axis=0)
+    assert np.abs(norms.cpu().numpy() - ref_n).max() < 1e-13 * ref_n.max()

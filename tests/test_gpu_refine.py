@@ -150,15 +150,15 @@ def test_stall_falls_back_to_fp64():
     assert colerr(X.cpu(), torch.from_numpy(ref)) < 1e-12
 
 
-@pytest.mark.parametrize("prec,bound", [("bf16x3", 3e-5), ("bf16x6", 1e-5)])
+@pytest.mark.parametrize("prec,bound", [("bf16x3", 1.5e-5), ("bf16x6", 5e-6)])
 @pytest.mark.parametrize("m,k,T", [(1000, 1000, 37), (777, 1301, 256), (64, 8, 1), (300, 5000, 130)])
 def test_relaxed_product(m, k, T, prec, bound):
     """Split-bf16 Krylov product against the FP64 product: the relative error
-    of every column within the level's bound (refine.RELAX_LEVELS), ragged
-    m, k and T, a strided right-hand side; A split from complex128 and from
-    its complex64 rounding gives the same bytes."""
+    of every column within the level's accuracy (about 2^-16 and 2^-22 of
+    the products' magnitudes for these unstructured operands), ragged m, k
+    and T, a strided right-hand side; A split from complex128 and from its
+    complex64 rounding gives the same bytes."""
     say("relaxed product", m, k, T, prec)
-    assert dict(refine.RELAX_LEVELS)[prec] == bound
     rng = np.random.default_rng(m + k + T)
     A = crand((m, k), rng)
     Zb = crand((k, T + 3), rng)
@@ -171,7 +171,7 @@ def test_relaxed_product(m, k, T, prec, bound):
     E.relaxed_product(AS, k, Z, C, prec)
     x = C.cpu().numpy()
     err = (np.linalg.norm(x - ref, axis=0) / np.linalg.norm(ref, axis=0)).max()
-    assert err < bound / 2, err
+    assert err < bound, err
 
 
 @pytest.mark.parametrize("method", ["bf16x3", "bf16x6"])
