@@ -189,6 +189,9 @@ int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64
  * order); work: mg_mdot_work_elems(1, T) complex128 elements. */
 int mg_maxpy_norm(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
                   int64_t ldw, int64_t n, int64_t T, void* work, double* norms, void* stream);
+/* norms[t] = ||X[:, t]||_2 of an n x T complex128 matrix (row stride ldx),
+ * summed in the fixed order of mg_maxpy_norm; work as there. */
+int mg_colnorm(const void* X, int64_t ldx, int64_t n, int64_t T, void* work, double* norms, void* stream);
 /* Relaxed-accuracy Krylov products of the refinement (no counterpart in the
  * reference): once the GMRES residual has dropped to r, a product with a
  * relative error of about tol / r moves the final residual by about tol, so

@@ -120,6 +120,24 @@ __global__ __launch_bounds__(256) void k_maxpy_norm(const double2* __restrict__ 
   if (ty == 0 && t < T) part[c * T + t] = ((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx];
 }
 
+// part[c T + t] = sum over row chunk c of |X[r, t]|^2 (the chunks of k_maxpy_norm)
+__global__ __launch_bounds__(256) void k_colnorm_part(const double2* __restrict__ X, int64_t ldx, int64_t n, int64_t T,
+                                                      int64_t rows, double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t c = blockIdx.x, t = (int64_t)blockIdx.y * 64 + tx;
+  const int64_t r0 = c * rows, r1 = (r0 + rows < n) ? r0 + rows : n;
+  double acc = 0.0;
+  if (t < T)
+    for (int64_t r = r0 + ty; r < r1; r += 4) {
+      const double2 o = X[r * ldx + t];
+      acc += o.x * o.x + o.y * o.y;
+    }
+  red[ty][tx] = acc;
+  __syncthreads();
+  if (ty == 0 && t < T) part[c * T + t] = ((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx];
+}
+
 // out[t] = sqrt(sum over chunks of part[c T + t]) (fixed order, as k_mdot_reduce)
 __global__ __launch_bounds__(256) void k_norm_reduce(const double* __restrict__ part, int64_t T, int nchunks,
                                                      double* __restrict__ out) {
@@ -264,6 +282,20 @@ int mg_maxpy_norm(const void* V, int64_t sV, int64_t ldv, int k, const void* h, 
   hipLaunchKernelGGL(k_maxpy_norm, dim3(kChunks, (unsigned)((T + 63) / 64)), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(h), ldh, alpha,
                      reinterpret_cast<double2*>(w), ldw, n, T, rows, part);
+  MG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)((T + 63) / 64)), dim3(256), 0, s, part, T, kChunks, norms);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+int mg_colnorm(const void* X, int64_t ldx, int64_t n, int64_t T, void* work, double* norms, void* stream) {
+  if (!X || !work || !norms || n < 0 || T < 0 || ldx < T) return MG_ERR_ARG;
+  if (T == 0) return MG_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int64_t rows = (n + kChunks - 1) / kChunks;
+  double* part = reinterpret_cast<double*>(work);
+  hipLaunchKernelGGL(k_colnorm_part, dim3(kChunks, (unsigned)((T + 63) / 64)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(X), ldx, n, T, rows, part);
   MG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)((T + 63) / 64)), dim3(256), 0, s, part, T, kChunks, norms);
   MG_HIP(hipGetLastError());

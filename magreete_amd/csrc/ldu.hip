@@ -99,8 +99,7 @@ __device__ inline void inv_barrier(unsigned* bar, unsigned target, int* err) {
 //   slab's pivot rows, and every other row gets X_i <- X_i - C_i R (slab p:
 //   X_i <- -C_i P^{-1}) as BF16x3 MFMA products.
 // One barrier per panel; zero pivots are reported in info (first, 1-based).
-// (kSlabs workgroups only: one wave per SIMD, the registers of all four.)
-__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
+__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
                                                               float2* __restrict__ X, int* info, int64_t row0,
                                                               float2* exch, unsigned* bar, int* err) {
   __shared__ InvSmem sm;

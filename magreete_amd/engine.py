@@ -415,6 +415,22 @@ def maxpy_norm(V, h, w, norms, alpha=1.0, work=None):
     return norms
 
 
+def colnorm(X, out=None, work=None):
+    """out[t] = ||X[:, t]|| (complex128 X, float64 out; fixed summation order)."""
+    lib = L.load()
+    dev = X.device
+    n, T = X.shape
+    if out is None:
+        out = torch.empty(T, dtype=torch.float64, device=dev)
+    need = lib.mg_mdot_work_elems(1, T)
+    if work is None or work.numel() < need:
+        work = torch.empty(max(need, 1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_colnorm(_p(X), _ld(X), n, T, _p(work), _p(out), _stream(dev))
+    L.check(st, "mg_colnorm")
+    return out
+
+
 def givens(H, H0, cs, sn, g, hn, bnorm, j, est):
     """Column j of the batched GMRES Hessenberg matrices (H, H0: (m+1, m, T);
     cs, sn: (m, T); g: (m+1, T), complex128; hn, bnorm: (T,) float64): sets

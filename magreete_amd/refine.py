@@ -94,7 +94,7 @@ class _Krylov:
             self.hn = torch.empty(T, dtype=torch.float64, device=dev)
             X.zero_()
             self.R = B.clone(memory_format=torch.contiguous_format)
-            self.beta = torch.linalg.norm(self.R, dim=0)
+            self.beta = E.colnorm(self.R, work=self.work)
         self.iters = self.cycles = 0
         self.prev = None
         self.ok = None
@@ -205,7 +205,7 @@ class _Krylov:
             self.R.zero_()
             E.maxpy(self.V[: k + 1], c.contiguous(), self.R, alpha=1.0)   # r = V_{k+1} c
             del self.V, self.Z, self.H, self.H0
-            self.beta = torch.linalg.norm(self.R, dim=0)
+            self.beta = E.colnorm(self.R, work=self.work)
             return True
 
 
@@ -296,7 +296,7 @@ class MixedSolver:
         Returns stats: tier, fallback, iters (products), cycles, residual,
         history (residual estimate after every product), products (the
         precision of every product)."""
-        bnorm = torch.linalg.norm(B, dim=0)
+        bnorm = E.colnorm(B.contiguous())
         bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
         total = 0
         hist = []

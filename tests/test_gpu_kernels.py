@@ -285,3 +285,5 @@ def test_krylov_mdot_maxpy(k, n, T):
     assert torch.equal(w2, wd)
     ref_n = np.linalg.norm(ref_w, axis=0)
     assert np.abs(norms.cpu().numpy() - ref_n).max() < 1e-13 * ref_n.max()
+    # read-only column norms, the same fixed summation order
+    assert torch.equal(E.colnorm(w2), norms)
