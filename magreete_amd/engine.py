@@ -285,15 +285,25 @@ def cgemm(A, B, C, alpha=1.0, beta=0.0, prec="c128", split=False):
     return C
 
 
-def split_rows(A):
+def split_rows_bytes(rows, cols):
+    return int(L.load().mg_split_rows_bytes(rows, cols))
+
+
+def split_rows(A, out=None):
     """A (complex128 or complex64, row-major) split once for relaxed_product:
     every element rounded to complex float and held as three bf16 parts of
-    its real and imaginary part (a uint8 buffer, 12 bytes per element)."""
+    its real and imaginary part (a uint8 buffer of split_rows_bytes, 12 bytes
+    per element; out: the caller's)."""
     lib = L.load()
     dev = A.device
     rows, cols = A.shape
     prec = L.MG_C128 if A.dtype == torch.complex128 else L.MG_C64
-    AS = torch.empty(int(lib.mg_split_rows_bytes(rows, cols)), dtype=torch.uint8, device=dev)
+    nb = split_rows_bytes(rows, cols)
+    if out is None:
+        out = torch.empty(nb, dtype=torch.uint8, device=dev)
+    elif out.numel() < nb or out.dtype != torch.uint8:
+        raise ValueError("split_rows needs %d bytes" % nb)
+    AS = out
     with torch.cuda.device(dev):
         st = lib.mg_split_rows(prec, _p(A), _ld(A), rows, cols, _p(AS), _stream(dev))
     L.check(st, "mg_split_rows")

@@ -47,16 +47,24 @@ class _SplitA:
     so that it runs beside the first iterations (the preconditioner sweeps
     leave CUs idle); get() makes the caller's stream wait for it."""
 
+    _streams = {}   # one side stream per device, kept for the process
+
     def __init__(self, M):
         dev = M.device
         main = torch.cuda.current_stream(dev)
-        self.side = torch.cuda.Stream(dev)
+        idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        if idx not in _SplitA._streams:
+            _SplitA._streams[idx] = torch.cuda.Stream(dev)
+        self.side = _SplitA._streams[idx]
+        # allocated from the main stream's pool (the blocks the previous
+        # factor and split released), written on the side stream
+        self.AS = torch.empty(E.split_rows_bytes(*M.shape), dtype=torch.uint8, device=dev)
+        self.AS.record_stream(self.side)
         self.side.wait_stream(main)            # M is assembled
         with torch.cuda.stream(self.side):
-            self.AS = E.split_rows(M)
+            E.split_rows(M, out=self.AS)
             self.ready = torch.cuda.Event()
             self.ready.record(self.side)
-        self.AS.record_stream(main)
 
     def get(self):
         torch.cuda.current_stream(self.AS.device).wait_event(self.ready)
@@ -76,8 +84,9 @@ class _Krylov:
     rounding of the FP64 products already taken, at the cost of one pass over
     the basis instead of another product."""
 
-    def __init__(self, M, F, B, X, bnorm, tol, max_iters, restart, stream, handle, history, relax=None):
+    def __init__(self, M, F, B, X, bnorm, tol, max_iters, restart, stream, handle, history, relax=None, gate=None):
         self.M, self.F, self.B, self.X = M, F, B, X
+        self.gate = gate            # called once after the first host sync; False stops the process
         self.relax = relax          # _SplitA (relaxed products), or None
         self.products = []          # precision of every product taken
         self.tol, self.max_iters, self.restart = tol, max_iters, restart
@@ -113,6 +122,8 @@ class _Krylov:
     def start_cycle(self):
         """Decides (host sync) whether another cycle runs; if so sets it up.
         False once the process has converged (ok) or failed."""
+        if self.ok is False:
+            return False
         with self._ctx():
             self.rel = (self.beta / self.bnorm).max().item()
             rel, tol = self.rel, self.tol
@@ -188,6 +199,11 @@ class _Krylov:
         tolerance or the basis is full.  True when the cycle ended."""
         with self._ctx():
             est = self.est_buf.item()   # |g_{j+1}| / ||b||: the Arnoldi residual formed below
+            if self.gate is not None:
+                gate, self.gate = self.gate, None
+                if not gate():
+                    self.ok = False
+                    return True
             self.res = est
             if self.history is not None:
                 self.history.append(est)
@@ -209,13 +225,15 @@ class _Krylov:
             return True
 
 
-def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=None, products=None):
+def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=None, products=None, gate=None):
     """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F,
     all columns as one batched process on the current stream.
     history: optional list that receives the largest per-column relative
     residual estimate after every product.  relax: a _SplitA of M (relaxed
     products), or None.  products: optional list that receives the
-    precision of every product.
+    precision of every product.  gate: optional callable run once after the
+    first host sync (the factorisation has completed by then); False stops
+    the refinement unconverged.
     Returns (converged, products, cycles, residual).
 
     Measured and rejected: two half-width processes on two streams whose
@@ -226,7 +244,7 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=No
     sweeps' waiting workgroups hold CUs that the product's one-per-CU
     workgroups then cannot use."""
     K = _Krylov(M, F, B, X, bnorm, tol, max_iters, restart, torch.cuda.current_stream(M.device), None, history,
-                relax)
+                relax, gate)
     while K.start_cycle():
         while True:
             K.precond()
@@ -264,6 +282,7 @@ class MixedSolver:
         self.F = None
         self.relax = relax
         self.AS = None           # _SplitA of M for the relaxed products
+        self._gate = None        # deferred zero-pivot check of the unpivoted factorisation
         self._factor(fresh=True)
 
     def _factor(self, fresh=False):
@@ -277,9 +296,20 @@ class MixedSolver:
         if not fresh:
             E.convert(self.M, self.M32)                    # the previous tier overwrote M32
         self.F = E.Factor(self.M32, self.prec, pivot=(self.tier == "pivoted"))
-        if self.tier == "nopiv" and int(self.F.info.item()) != 0:
-            self._escalate()                               # exact zero pivot: pivot this time
-        elif self.F.A is None:
+        if self.tier == "nopiv":
+            # an exact zero pivot moves the solve to the pivoted tier; the
+            # check waits for the refinement's first host sync instead of
+            # stalling the stream here
+            info = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            info.copy_(self.F.info, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(torch.cuda.current_stream(self.M.device))
+
+            def gate():
+                done.synchronize()
+                return int(info[0]) == 0
+            self._gate = gate
+        if self.F.A is None:
             self.M32 = None                                # the split factor does not read it again
 
     def _escalate(self):
@@ -304,8 +334,9 @@ class MixedSolver:
         while self.tier != "fp64":
             if self.relax and self.AS is None:
                 self.AS = _SplitA(self.M)
+            gate, self._gate = self._gate, None
             ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart,
-                                               hist, self.AS, prods)
+                                               hist, self.AS, prods, gate)
             total += iters
             if ok:
                 return {"tier": self.tier, "fallback": self.fell, "iters": total, "cycles": cycles,
