@@ -182,6 +182,11 @@ int mg_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* w, int64_
  * receives max_t |g_{j+1,t}| / bnorm[t] (device pointers, stream-ordered). */
 int mg_givens(void* H, void* H0, void* cs, void* sn, void* g, const double* hn, const double* bnorm, int m, int j,
               int64_t T, double* est, void* stream);
+/* The GMRES coefficients of all T columns after k Arnoldi steps (no
+ * reference counterpart): y (k x T complex128, y[i T + t]) solves the upper-
+ * triangular R_t y_t = g_t with R_t the first k x k of the rotated H of
+ * mg_givens (layout as there) and g_t its first k entries. */
+int mg_hess_solve(const void* H, const void* g, int m, int k, int64_t T, void* y, void* stream);
 int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
              int64_t ldw, int64_t n, int64_t T, void* stream);
 /* mg_maxpy followed by the column norms of the updated w (no reference

@@ -225,9 +225,41 @@ __global__ __launch_bounds__(256) void k_givens(double2* __restrict__ H, double2
   if (threadIdx.x == 0) est[0] = red[0];
 }
 
+// y (k x T) from the Givens-rotated Hessenberg matrices: column t solves the
+// upper-triangular R_t y_t = g_t (R_t = H[:k, :k] of column t) by back
+// substitution, one thread per column.
+__global__ __launch_bounds__(256) void k_hess_solve(const double2* __restrict__ H, const double2* __restrict__ g,
+                                                    int m, int k, int64_t T, double2* __restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  for (int i = k - 1; i >= 0; --i) {
+    double2 s = g[(int64_t)i * T + t];
+    for (int j = i + 1; j < k; ++j) {
+      const double2 h = H[((int64_t)i * m + j) * T + t], yj = y[(int64_t)j * T + t];
+      s.x -= h.x * yj.x - h.y * yj.y;
+      s.y -= h.x * yj.y + h.y * yj.x;
+    }
+    const double2 d = H[((int64_t)i * m + i) * T + t];
+    const double den = d.x * d.x + d.y * d.y;
+    y[(int64_t)i * T + t] = make_double2((s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int mg_hess_solve(const void* H, const void* g, int m, int k, int64_t T, void* y, void* stream) {
+  if (!H || !g || !y || m < 1 || k < 0 || k > m || T < 0) return MG_ERR_ARG;
+  if (k == 0 || T == 0) return MG_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_hess_solve, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(H), reinterpret_cast<const double2*>(g), m, k, T,
+                     reinterpret_cast<double2*>(y));
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
 
 int mg_givens(void* H, void* H0, void* cs, void* sn, void* g, const double* hn, const double* bnorm, int m, int j,
               int64_t T, double* est, void* stream) {

@@ -441,6 +441,20 @@ def colnorm(X, out=None, work=None):
     return out
 
 
+def hess_solve(H, g, k, out=None):
+    """y (k x T): back substitution in the first k x k of every column's
+    rotated Hessenberg matrix H ((m+1) x m x T) against g ((m+1) x T)."""
+    lib = L.load()
+    dev = H.device
+    m, T = H.shape[1], H.shape[2]
+    if out is None:
+        out = torch.empty((k, T), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_hess_solve(_p(H), _p(g), m, k, T, _p(out), _stream(dev))
+    L.check(st, "mg_hess_solve")
+    return out
+
+
 def givens(H, H0, cs, sn, g, hn, bnorm, j, est):
     """Column j of the batched GMRES Hessenberg matrices (H, H0: (m+1, m, T);
     cs, sn: (m, T); g: (m+1, T), complex128; hn, bnorm: (T,) float64): sets

@@ -8,11 +8,12 @@ complex128, kept resident), starting from x = 0 so that the first direction
 is P^{-1} b at no FP64 cost.  All T right-hand sides run together: the
 matrix products are one (n x n) @ (n x T) FP64 GEMM and one complex64
 multi-RHS solve (GEMM-only, through stored diagonal-block inverses) per
-iteration; the per-column Gram-Schmidt runs as HIP multi-vector dot/axpy
-passes and the Givens updates are small (T x j) torch operations.
-Convergence is declared on the FP64 residual ||b - A x|| / ||b|| <= tol per
-column, formed at the end of every cycle from the Arnoldi relation (equal to
-the product b - A x up to the rounding of the FP64 products already taken).
+iteration; the per-column Gram-Schmidt (classical, twice), the Givens
+updates and the triangular solve for the coefficients run as HIP kernels.
+Convergence is declared on the least-squares residual |g_{k+1}| / ||b|| <=
+tol of every column (the norm of V_{k+1} (beta e1 - H_k y), which equals
+b - A x up to the rounding of the products already taken); a cycle that
+ends unconverged forms that residual explicitly and restarts from it.
 
 The first factorisation is unpivoted (no pivot search, no grid barrier, no
 row swap); the refinement restores the accuracy.  If it breaks down (an exact
@@ -78,11 +79,12 @@ class _Krylov:
 
     Solves M X = B from X = 0, right-preconditioned by F; the first Krylov
     direction is P^{-1} b, so the first cycle starts with no FP64 product.
-    Every cycle ends with the residual b - M x formed from the Arnoldi
-    relation M Z_k = V_{k+1} H_k, i.e. r = V_{k+1} (beta e1 - H_k y) with the
-    unrotated Hessenberg matrix: it equals the FP64 product b - M x up to the
-    rounding of the FP64 products already taken, at the cost of one pass over
-    the basis instead of another product."""
+    A cycle ends converged when every column's least-squares residual
+    |g_{k+1}| / ||b|| meets the tolerance; otherwise (basis full) the residual
+    b - M x is formed from the Arnoldi relation M Z_k = V_{k+1} H_k, i.e.
+    r = V_{k+1} (beta e1 - H_k y) with the unrotated Hessenberg matrix, at
+    the cost of one pass over the basis instead of another product, and the
+    next cycle restarts from it."""
 
     def __init__(self, M, F, B, X, bnorm, tol, max_iters, restart, stream, handle, history, relax=None, gate=None):
         self.M, self.F, self.B, self.X = M, F, B, X
@@ -122,7 +124,7 @@ class _Krylov:
     def start_cycle(self):
         """Decides (host sync) whether another cycle runs; if so sets it up.
         False once the process has converged (ok) or failed."""
-        if self.ok is False:
+        if self.ok is not None:
             return False
         with self._ctx():
             self.rel = (self.beta / self.bnorm).max().item()
@@ -195,10 +197,11 @@ class _Krylov:
 
     def after_product(self):
         """Reads the least-squares residual estimate (host sync); ends the
-        cycle (x += Z y, residual from the Arnoldi relation) when it meets the
-        tolerance or the basis is full.  True when the cycle ended."""
+        cycle (x += Z y) when it meets the tolerance (converged) or the basis
+        is full (residual from the Arnoldi relation, for a restart).  True
+        when the cycle ended."""
         with self._ctx():
-            est = self.est_buf.item()   # |g_{j+1}| / ||b||: the Arnoldi residual formed below
+            est = self.est_buf.item()   # max over columns of |g_{j+1}| / ||b||
             if self.gate is not None:
                 gate, self.gate = self.gate, None
                 if not gate():
@@ -211,11 +214,13 @@ class _Krylov:
             if est > self.tol and self.j < self.m:
                 return False
             k = self.j
-            H, g = self.H, self.g
-            Hk = H[:k, :k].permute(2, 0, 1).contiguous()
-            gk = g[:k].T.contiguous().unsqueeze(-1)
-            y = torch.linalg.solve_triangular(Hk, gk, upper=True).squeeze(-1).T.contiguous()   # (k, T)
+            y = E.hess_solve(self.H, self.g, k)                           # (k, T)
             E.maxpy(self.Z[:k], y, self.X, alpha=1.0)                  # x += Z y
+            if est <= self.tol:
+                # converged on the least-squares residual |g_{k+1}| / ||b||
+                self.ok, self.rel = True, est
+                del self.V, self.Z, self.H, self.H0
+                return True
             c = -(self.H0[: k + 1, :k] * y.unsqueeze(0)).sum(1)          # beta e1 - H_k y
             c[0] += self.beta.to(torch.complex128)
             self.R.zero_()
