@@ -99,11 +99,17 @@ __device__ inline void inv_barrier(unsigned* bar, unsigned target, int* err) {
 //   slab's pivot rows, and every other row gets X_i <- X_i - C_i R (slab p:
 //   X_i <- -C_i P^{-1}) as BF16x3 MFMA products.
 // One barrier per panel; zero pivots are reported in info (first, 1-based).
+// bar / err: this launch's barrier counter and failure flag (zero on entry);
+// next: the other set, cleared here for the next launch.
 __global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
                                                               float2* __restrict__ X, int* info, int64_t row0,
-                                                              float2* exch, unsigned* bar, int* err) {
+                                                              float2* exch, unsigned* bar, int* err, int* next) {
   __shared__ InvSmem sm;
   const int q = blockIdx.x;
+  if (q == 0 && threadIdx.x == 0) {   // the next launch's barrier counter and flag
+    __hip_atomic_store(next, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(next + 32, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
   // slab item r of tile (a, b): row 64 wave + 16 a + 4 lg + r, column 32 q + 16 b + li
@@ -734,10 +740,14 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb;
     float2* Dk = D + k * nb * nb;
     const float2* S = Ap + r0 * lda + r0;
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, ws.inv_ctl, 64);
+    // the two control sets alternate: this launch's was cleared by the
+    // previous one, which this one clears in turn (launches are ordered)
+    int* ctl = ws.inv_ctl + 64 * ws.inv_parity;
+    int* other = ws.inv_ctl + 64 * (1 - ws.inv_parity);
     hipLaunchKernelGGL(k_block_inverse, dim3(kSlabs), dim3(kMwThreads), 0, st, S, lda, (int)h, newton ? X0 : Dk,
-                       info, r0, ws.inv_exch, reinterpret_cast<unsigned*>(ws.inv_ctl), ws.inv_ctl + 32);
+                       info, r0, ws.inv_exch, reinterpret_cast<unsigned*>(ctl), ctl + 32, other);
     MG_HIP(hipGetLastError());
+    ws.inv_parity ^= 1;
     if (newton) {
       hipLaunchKernelGGL(k_two_eye, dim3((unsigned)((h * nb + 255) / 256)), dim3(256), 0, st, R, (int)h);
       MG_HIP(hipGetLastError());

@@ -41,7 +41,8 @@ struct Workspace {
   hipStream_t s3 = nullptr;        // a third stream beside the look-ahead (default priority)
   hipEvent_t evA = nullptr, evB = nullptr, evC = nullptr, evD = nullptr, evE = nullptr;
   float2* inv_exch = nullptr;      // LDU block inverse: two 256 x 32 published slabs
-  int* inv_ctl = nullptr;          // LDU block inverse: barrier counter [0] and failure flag [32]
+  int* inv_ctl = nullptr;          // LDU block inverse: two sets of barrier counter [0] and failure flag [32]
+  int inv_parity = 0;              // the set the next block inverse uses (it clears the other)
   int* sweep = nullptr;            // ticket + per-block-row counters of the LDU solve sweeps
   int64_t sweep_cap = 0;           // ints allocated at sweep
 };

@@ -560,7 +560,8 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kLeaf));
   MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * kBarWords));
   MG_HIP(hipMalloc(&ws.inv_exch, sizeof(float2) * 2 * 256 * 32));
-  MG_HIP(hipMalloc(&ws.inv_ctl, sizeof(int) * 64));
+  MG_HIP(hipMalloc(&ws.inv_ctl, sizeof(int) * 128));
+  MG_HIP(hipMemset(ws.inv_ctl, 0, sizeof(int) * 128));
   MG_HIP(hipMalloc(&ws.getf2.err, sizeof(int) * 64));
   MG_HIP(hipMemset(ws.getf2.err, 0, sizeof(int) * 64));
   if (ws.coop_blocks > G) ws.coop_blocks = G;
