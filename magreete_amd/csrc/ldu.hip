@@ -495,7 +495,8 @@ constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
 // (split-bf16 MFMA).  Finished tiles are published pre-split in XS (B-operand
 // units, one column per right-hand side, np / 8 groups per column; rows past
 // n as zeros) for the later block rows, and as complex float in X.
-// ctr[0] hands out tickets in start order and ctr[1 + i] counts the final
+// ctr (zero on entry; next: the counters of the following sweep, cleared
+// here) -- ctr[0] hands out tickets in start order and ctr[1 + i] counts the final
 // tiles of block row i; tickets map to block rows in dependency order, so a
 // workgroup only waits on workgroups that started before it and the sweep
 // needs no co-residency.  XS is written with agent-scope stores and read with
@@ -506,11 +507,14 @@ constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
 template <int NS, int BM, int BN, int WM, int WN, bool FWD>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(
     const u32x4* __restrict__ SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx, u32x4* XS,
-    int64_t n, int64_t nrhs, int nblk, int ct, int* ctr) {
+    int64_t n, int64_t nrhs, int nblk, int ct, int* ctr, int* next) {
   typedef mg_ps::PTile<NS, BM, BN, WM, WN> Tile;
   constexpr int U = 2 * NS, KG = kIB / 8, KT = KG / 4;
   __shared__ typename Tile::S sm;
   __shared__ int s_ticket, s_ok;
+  if (blockIdx.x == 0)   // the next sweep's counters (that launch follows this one on the stream)
+    for (int e = threadIdx.x; e <= nblk; e += blockDim.x)
+      __hip_atomic_store(next + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int tpr = (kIB / BM) * ct;   // tiles per block row
@@ -614,18 +618,31 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
     tile.store(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
 
-// Sweep launch: counters zeroed by a kernel (no memset node), grid = block
-// rows x tiles per block row.
+// Counter sets of the sweeps of one handle: two, used alternately; each
+// sweep clears the other set for the next one (no zeroing launch).
+struct SweepCtr {
+  int* base;
+  int64_t cap;   // ints per set
+  int* parity;
+  int* take() {
+    int* c = base + cap * *parity;
+    *parity ^= 1;
+    return c;
+  }
+  int* other() const { return base + cap * *parity; }   // after take(): the next sweep's set
+};
+
+// Sweep launch: grid = block rows x tiles per block row.
 template <int NS, int BM, int BN, int WM, int WN, bool FWD>
-int launch_sweep(int* ctr, const u32x4* SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx,
-                 u32x4* XS, int64_t n, int64_t nrhs, hipStream_t s) {
+int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X,
+                 int64_t ldx, u32x4* XS, int64_t n, int64_t nrhs, hipStream_t s) {
   const int nblk = (int)((n + kIB - 1) / kIB);
   const int ct = (int)((nrhs + BN - 1) / BN);
   const int64_t grid = (int64_t)nblk * (kIB / BM) * ct;
   if (grid > 0x7fffffffLL) return MG_ERR_ARG;
-  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((nblk + 1 + 255) / 256)), dim3(256), 0, s, ctr, nblk + 1);
+  int* ctr = sc.take();
   hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, SP, np,
-                     Bin, ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr);
+                     Bin, ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr, sc.other());
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
@@ -636,8 +653,8 @@ int launch_sweep(int* ctr, const u32x4* SP, int64_t np, const float2* Bin, int64
 // for 64 x 128 and 1.2-1.5 ms for 32 x 64, 64 x 32 and 32 x 32 (the sweep's
 // time is its chain of 48 dependent block rows, not its MFMA work).
 template <int NS>
-int ldu_sweeps(int* ctr, const u32x4* SPL, const u32x4* SPUr, int64_t np, const float2* D, float2* B, int64_t ldb,
-               int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
+int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, int64_t np, const float2* D, float2* B,
+               int64_t ldb, int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
   constexpr int BN = 64;
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
   MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
@@ -907,15 +924,20 @@ int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void
       ws->sweep = nullptr;
       ws->sweep_cap = 0;
       const int64_t cap = ((nblk + 1 + 255) / 256) * 256;
-      MG_HIP(hipMalloc(&ws->sweep, sizeof(int) * cap));
+      MG_HIP(hipMalloc(&ws->sweep, sizeof(int) * 2 * cap));
+      hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((2 * cap + 255) / 256)), dim3(256), 0, s, ws->sweep,
+                         (int)(2 * cap));
+      MG_HIP(hipGetLastError());
       ws->sweep_cap = cap;
+      ws->sweep_parity = 0;
     }
+    SweepCtr ctr{ws->sweep, ws->sweep_cap, &ws->sweep_parity};
     const SplitLayout lay{padded(n), nblk, 2 * sweep_ns};
     const u32x4* SPL = reinterpret_cast<const u32x4*>(D + ldu_dinv_elems(n));
     const u32x4* SPUr = SPL + lay.total();
     u32x4* XS = reinterpret_cast<u32x4*>(W + ((n * nrhs + 1) & ~(int64_t)1));
-    if (sweep_ns == 2) return ldu_sweeps<2>(ws->sweep, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
-    return ldu_sweeps<3>(ws->sweep, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
+    if (sweep_ns == 2) return ldu_sweeps<2>(ctr, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
+    return ldu_sweeps<3>(ctr, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
   }
   // forward: B[r1:] -= Lh[r1:, block k] B[block k]
   for (int64_t k = 0; k + 1 < nblk; ++k) {

@@ -43,8 +43,9 @@ struct Workspace {
   float2* inv_exch = nullptr;      // LDU block inverse: two 256 x 32 published slabs
   int* inv_ctl = nullptr;          // LDU block inverse: two sets of barrier counter [0] and failure flag [32]
   int inv_parity = 0;              // the set the next block inverse uses (it clears the other)
-  int* sweep = nullptr;            // ticket + per-block-row counters of the LDU solve sweeps
-  int64_t sweep_cap = 0;           // ints allocated at sweep
+  int* sweep = nullptr;            // two sets of ticket + per-block-row counters of the LDU solve sweeps
+  int64_t sweep_cap = 0;           // ints per set
+  int sweep_parity = 0;            // the set the next sweep uses (it clears the other)
 };
 
 typedef int (*GemmFn)(int64_t, int64_t, int64_t, cd, const void*, int64_t, const void*, int64_t, cd, void*, int64_t,
