@@ -618,6 +618,28 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
     tile.store(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
 
+// W = D^{-1} X block by block from the pre-split D inverses (DS: A-operand
+// rows, 32 groups each) and the forward sweep's pre-split result (XS:
+// B-operand columns, ldgx groups each); one workgroup per 64 x 64 tile of a
+// block row, W complex float (n x nrhs, ldw).
+template <int NS>
+__global__ __launch_bounds__(256) void k_d_product(const u32x4* __restrict__ DS, const u32x4* __restrict__ XS,
+                                                   int64_t ldgx, float2* __restrict__ W, int64_t ldw, int64_t n,
+                                                   int64_t nrhs, int ct) {
+  typedef mg_ps::PTile<NS, 64, 64, 2, 2> Tile;
+  constexpr int U = 2 * NS, KG = kIB / 8;
+  __shared__ typename Tile::S sm;
+  const int tpb = (kIB / 64) * ct;
+  const int blk = blockIdx.x / tpb, w = blockIdx.x - blk * tpb;
+  const int64_t i0 = (int64_t)blk * kIB + (int64_t)(w / ct) * 64, j0 = (int64_t)(w % ct) * 64;
+  const int64_t M = n - i0, N = nrhs - j0;
+  if (M <= 0) return;
+  Tile t;
+  t.init(nullptr, 0, 0, 0, make_float2(0.f, 0.f), false);
+  t.pmac(sm, DS + i0 * KG * U, KG, XS + (j0 * ldgx + (int64_t)blk * KG) * U, ldgx, M, N, KG);
+  t.store(W + i0 * ldw + j0, ldw, M, N, make_float2(1.f, 0.f), make_float2(0.f, 0.f), false);
+}
+
 // Counter sets of the sweeps of one handle: two, used alternately; each
 // sweep clears the other set for the next one (no zeroing launch).
 struct SweepCtr {
@@ -647,25 +669,22 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
   return MG_OK;
 }
 
-// The LDU solve with pre-split panels: forward sweep, the batched product
-// with the D inverses (FP32 MFMA), backward sweep.  64 x 64 tiles: on the
+// The LDU solve with pre-split panels: forward sweep, the block-diagonal
+// product with the D inverses (split-bf16 MFMA from the pre-split inverses
+// and the forward sweep's pre-split result), backward sweep.  64 x 64 tiles: on the
 // 12288-row, 256-column solve they measured 1.15 ms per sweep against 1.4 ms
 // for 64 x 128 and 1.2-1.5 ms for 32 x 64, 64 x 32 and 32 x 32 (the sweep's
 // time is its chain of 48 dependent block rows, not its MFMA work).
 template <int NS>
-int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, int64_t np, const float2* D, float2* B,
+int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, const u32x4* DS, int64_t np, float2* B,
                int64_t ldb, int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
   constexpr int BN = 64;
-  const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
+  const int64_t nblk = (n + kIB - 1) / kIB;
+  const int ct = (int)((nrhs + 63) / 64);
   MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
-  if (full > 0)
-    MG_TRY(mg::cgemm_batched<float>(nb, nrhs, nb, mg::cd{1, 0}, D, nb, nb * nb, B, ldb, nb * ldb, mg::cd{0, 0}, W,
-                                    nrhs, nb * nrhs, full, s));
-  if (full < nblk) {
-    const int64_t r0 = full * nb, h = n - r0;
-    MG_TRY(mg::cgemm<float>(h, nrhs, h, mg::cd{1, 0}, D + full * nb * nb, nb, B + r0 * ldb, ldb, mg::cd{0, 0},
-                            W + r0 * nrhs, nrhs, s));
-  }
+  hipLaunchKernelGGL(k_d_product<NS>, dim3((unsigned)(nblk * (kIB / 64) * ct)), dim3(256), 0, s, DS, XS, np / 8, W,
+                     nrhs, n, nrhs, ct);
+  MG_HIP(hipGetLastError());
   return launch_sweep<NS, 64, BN, 2, 2, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
 }
 
@@ -689,7 +708,8 @@ int64_t padded(int64_t n) { return ((n + kIB - 1) / kIB) * kIB; }
 int64_t split_elems(int ns, int64_t n) {
   if (ns == 0) return 0;
   const SplitLayout lay{padded(n), (n + kIB - 1) / kIB, 2 * ns};
-  return 2 * lay.total() * 2;   // SPL and SPUr, two float2 per 16-byte unit
+  // SPL and SPUr, then DS (the D inverses' rows, 32 groups each), two float2 per 16-byte unit
+  return (2 * lay.total() + padded(n) * (kIB / 8) * 2 * ns) * 2;
 }
 // the pre-split copies of A12, one per step for four steps in flight (widest split, 6 units)
 int64_t a12_split_elems(int64_t n) { return 4 * padded(n) * (kIB / 8) * 6 * 2; }
@@ -812,6 +832,18 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
                            : ldu_update<3>(mr, nc, src, Ap + i0 * lda + j0, lda, st);
     });
   };
+  // split precisions: the D inverses pre-split as A operands (DS, after
+  // SPUr) for the solves' block-diagonal products, once all are final on s
+  auto pack_d = [&]() -> int {
+    if (!split_ns) return MG_OK;
+    u32x4* DS = SPUr + lay.total();
+    const int64_t rows = nblk * nb, total = rows * (kIB / 8);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (split_ns == 2) hipLaunchKernelGGL(k_pack_rows<2>, grid, dim3(256), 0, s, D, nb, rows, nb, rows, kIB / 8, DS);
+    else hipLaunchKernelGGL(k_pack_rows<3>, grid, dim3(256), 0, s, D, nb, rows, nb, rows, kIB / 8, DS);
+    MG_HIP(hipGetLastError());
+    return MG_OK;
+  };
   if (!split_ns || !ws.s2 || !ws.s3) {
     MG_TRY(inverse(0, s));
     MG_TRY(panel_rest(0, s));
@@ -841,7 +873,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
         MG_TRY(panel_rest(k + 1, s));
       }
     }
-    return MG_OK;
+    return pack_d();
   }
   // Split precisions, in super-steps of two blocks (k, k+1).  The second
   // stream carries the critical path: the inverse and panels of block k, the
@@ -902,7 +934,7 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
     MG_HIP(hipStreamWaitEvent(s, ws.evA, 0));
     MG_HIP(hipStreamWaitEvent(s, ws.evE, 0));
   }
-  return MG_OK;
+  return pack_d();
 }
 
 int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb,
@@ -935,9 +967,10 @@ int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void
     const SplitLayout lay{padded(n), nblk, 2 * sweep_ns};
     const u32x4* SPL = reinterpret_cast<const u32x4*>(D + ldu_dinv_elems(n));
     const u32x4* SPUr = SPL + lay.total();
+    const u32x4* DS = SPUr + lay.total();
     u32x4* XS = reinterpret_cast<u32x4*>(W + ((n * nrhs + 1) & ~(int64_t)1));
-    if (sweep_ns == 2) return ldu_sweeps<2>(ctr, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
-    return ldu_sweeps<3>(ctr, SPL, SPUr, lay.np, D, Bp, ldb, nrhs, W, XS, n, s);
+    if (sweep_ns == 2) return ldu_sweeps<2>(ctr, SPL, SPUr, DS, lay.np, Bp, ldb, nrhs, W, XS, n, s);
+    return ldu_sweeps<3>(ctr, SPL, SPUr, DS, lay.np, Bp, ldb, nrhs, W, XS, n, s);
   }
   // forward: B[r1:] -= Lh[r1:, block k] B[block k]
   for (int64_t k = 0; k + 1 < nblk; ++k) {
