@@ -45,29 +45,40 @@ RELAX_SAFETY = 0.5
 
 class _SplitA:
     """M split for the relaxed products (E.split_rows), made on a side stream
-    so that it runs beside the first iterations (the preconditioner sweeps
-    leave CUs idle); get() makes the caller's stream wait for it."""
+    beside the first preconditioner sweep (latency-bound, it leaves HBM
+    bandwidth idle; beside the memory-bound start of the refinement the split
+    slowed it); get() makes the caller's stream wait for it."""
 
     _streams = {}   # one side stream per device, kept for the process
 
     def __init__(self, M):
+        self.M = M
+        self.AS = None
+
+    def launch(self):
+        """Enqueues the split on the side stream after the work already on
+        the caller's stream (once)."""
+        if self.AS is not None:
+            return
+        M = self.M
         dev = M.device
         main = torch.cuda.current_stream(dev)
         idx = dev.index if dev.index is not None else torch.cuda.current_device()
         if idx not in _SplitA._streams:
             _SplitA._streams[idx] = torch.cuda.Stream(dev)
-        self.side = _SplitA._streams[idx]
+        side = _SplitA._streams[idx]
         # allocated from the main stream's pool (the blocks the previous
         # factor and split released), written on the side stream
         self.AS = torch.empty(E.split_rows_bytes(*M.shape), dtype=torch.uint8, device=dev)
-        self.AS.record_stream(self.side)
-        self.side.wait_stream(main)            # M is assembled
-        with torch.cuda.stream(self.side):
+        self.AS.record_stream(side)
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
             E.split_rows(M, out=self.AS)
             self.ready = torch.cuda.Event()
-            self.ready.record(self.side)
+            self.ready.record(side)
 
     def get(self):
+        self.launch()
         torch.cuda.current_stream(self.AS.device).wait_event(self.ready)
         return self.AS
 
@@ -158,6 +169,8 @@ class _Krylov:
         """z_j = P^{-1} v_j (enqueued on the process's stream)."""
         with self._ctx():
             E.convert(self.V[self.j], self.lowp)
+            if self.relax is not None:
+                self.relax.launch()
             self.F.solve(self.lowp, work=self.fwork)
             E.convert(self.lowp, self.Z[self.j])
 
