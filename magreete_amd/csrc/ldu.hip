@@ -834,13 +834,21 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   };
   // split precisions: the D inverses pre-split as A operands (DS, after
   // SPUr) for the solves' block-diagonal products, once all are final on s
+  // (full blocks in one launch; a short last block with its h x h inverse
+  // only, zeros around it: the Newton step writes no padding)
   auto pack_d = [&]() -> int {
     if (!split_ns) return MG_OK;
     u32x4* DS = SPUr + lay.total();
-    const int64_t rows = nblk * nb, total = rows * (kIB / 8);
-    const dim3 grid((unsigned)((total + 255) / 256));
-    if (split_ns == 2) hipLaunchKernelGGL(k_pack_rows<2>, grid, dim3(256), 0, s, D, nb, rows, nb, rows, kIB / 8, DS);
-    else hipLaunchKernelGGL(k_pack_rows<3>, grid, dim3(256), 0, s, D, nb, rows, nb, rows, kIB / 8, DS);
+    const int64_t full = n / nb, h = n - full * nb;
+    auto pack = [&](const float2* src, int64_t rows, int64_t K, int64_t rows_pad, u32x4* out) {
+      const dim3 grid((unsigned)((rows_pad * (kIB / 8) + 255) / 256));
+      if (split_ns == 2)
+        hipLaunchKernelGGL(k_pack_rows<2>, grid, dim3(256), 0, s, src, nb, rows, K, rows_pad, kIB / 8, out);
+      else
+        hipLaunchKernelGGL(k_pack_rows<3>, grid, dim3(256), 0, s, src, nb, rows, K, rows_pad, kIB / 8, out);
+    };
+    if (full > 0) pack(D, full * nb, nb, full * nb, DS);
+    if (h > 0) pack(D + full * nb * nb, h, h, nb, DS + full * nb * (kIB / 8) * U);
     MG_HIP(hipGetLastError());
     return MG_OK;
   };
