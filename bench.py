@@ -196,7 +196,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": rb["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": {"fp64": "complex128", "c64": "complex64 LU + complex128 GMRES-IR",
                                        "bf16x3": "complex64 (bf16x3 products) LU + complex128 GMRES-IR",
-                                       "bf16x6": "complex64 (bf16x6 products) LU + complex128 GMRES-IR"}[best],
+                                       "bf16x6": "complex64 (bf16x6 products) LU + complex128 GMRES-IR"}[best]
+        + ("" if best == "fp64" or args.no_relax else " (late Krylov products split-bf16 where the residual allows)"),
         "data": "synthetic (seeded uniform ball, Fibonacci directions)",
         "config": {"workload": "3d_vector N=%d (n=%d) T=%d (%d dirs x 2 pol) measure=%d" % (
             args.n_points, n, w.T, args.n_dirs, args.n_measure), "method": best, "k0": w.k0,

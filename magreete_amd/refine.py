@@ -71,6 +71,7 @@ class _SplitA:
         # factor and split released), written on the side stream
         self.AS = torch.empty(E.split_rows_bytes(*M.shape), dtype=torch.uint8, device=dev)
         self.AS.record_stream(side)
+        M.record_stream(side)                  # M's block outlives the split even if no product waits for it
         side.wait_stream(main)
         with torch.cuda.stream(side):
             E.split_rows(M, out=self.AS)
