@@ -567,11 +567,14 @@ int workspace_init(Workspace& ws) {
   if (ws.coop_blocks > G) ws.coop_blocks = G;
   MG_HIP(hipStreamCreateWithFlags(&ws.s2, hipStreamNonBlocking));
   MG_HIP(hipStreamCreateWithFlags(&ws.s3, hipStreamNonBlocking));
+  MG_HIP(hipStreamCreateWithFlags(&ws.s4, hipStreamNonBlocking));
   MG_HIP(hipEventCreateWithFlags(&ws.evA, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evB, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evC, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evD, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evE, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evF, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evG, hipEventDisableTiming));
   ws.ready = true;
   return MG_OK;
 }
@@ -607,11 +610,14 @@ void workspace_free(Workspace& ws) {
   if (ws.inv_ctl) (void)hipFree(ws.inv_ctl);
   if (ws.s2) (void)hipStreamDestroy(ws.s2);
   if (ws.s3) (void)hipStreamDestroy(ws.s3);
+  if (ws.s4) (void)hipStreamDestroy(ws.s4);
   if (ws.evA) (void)hipEventDestroy(ws.evA);
   if (ws.evB) (void)hipEventDestroy(ws.evB);
   if (ws.evC) (void)hipEventDestroy(ws.evC);
   if (ws.evD) (void)hipEventDestroy(ws.evD);
   if (ws.evE) (void)hipEventDestroy(ws.evE);
+  if (ws.evF) (void)hipEventDestroy(ws.evF);
+  if (ws.evG) (void)hipEventDestroy(ws.evG);
   ws = Workspace();
 }
 
