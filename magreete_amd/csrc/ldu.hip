@@ -758,10 +758,10 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
   // step k's panels of a split precision, straight from A and D_k: A12
   // pre-split as the trailing update's B operand, Uh = D_k A12 and
   // Lh = A21 D_k as GEMMs whose epilogues write the pre-split panels (A is
-  // not written, so no copies are needed).  Uh is read by the solves only:
-  // it is formed on the fourth stream, off the chain of inverses and panels
-  // (A12 and D_k are final here and never written again; the factorisation's
-  // end joins that stream).
+  // not written, so no copies are needed).  A12's split and then Uh, which
+  // the solves alone read, are formed on the fourth stream beside Lh (A12
+  // and D_k are final here and never written again); the chain waits for
+  // the split only (evH), the factorisation's end joins that stream.
   auto split_panels = [&](int64_t k, hipStream_t st) -> int {
     const int64_t r0 = k * nb, r1 = r0 + nb, r = n - r1, rk = lay.rows(k);
     const float2* Dk = D + k * nb * nb;
@@ -772,14 +772,18 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
       MG_HIP(hipEventRecord(ws.evF, st));
       MG_HIP(hipStreamWaitEvent(su, ws.evF, 0));
     }
+    if (split_ns == 2) MG_TRY(pack_cols<2>(A12, lda, nb, r, rk, kIB / 8, spa12(k), su));
+    else MG_TRY(pack_cols<3>(A12, lda, nb, r, rk, kIB / 8, spa12(k), su));
+    if (su != st) MG_HIP(hipEventRecord(ws.evH, su));
     if (split_ns == 2) {
       MG_TRY(panel_split<2>(nb, r, nb, Dk, nb, A12, lda, nb, rk, SPUr + lay.off(k), rk / 8, su));
-      MG_TRY(pack_cols<2>(A12, lda, nb, r, rk, kIB / 8, spa12(k), st));
-      return panel_split<2>(r, nb, nb, A21, lda, Dk, nb, rk, nb, SPL + lay.off(k), kIB / 8, st);
+      MG_TRY(panel_split<2>(r, nb, nb, A21, lda, Dk, nb, rk, nb, SPL + lay.off(k), kIB / 8, st));
+    } else {
+      MG_TRY(panel_split<3>(nb, r, nb, Dk, nb, A12, lda, nb, rk, SPUr + lay.off(k), rk / 8, su));
+      MG_TRY(panel_split<3>(r, nb, nb, A21, lda, Dk, nb, rk, nb, SPL + lay.off(k), kIB / 8, st));
     }
-    MG_TRY(panel_split<3>(nb, r, nb, Dk, nb, A12, lda, nb, rk, SPUr + lay.off(k), rk / 8, su));
-    MG_TRY(pack_cols<3>(A12, lda, nb, r, rk, kIB / 8, spa12(k), st));
-    return panel_split<3>(r, nb, nb, A21, lda, Dk, nb, rk, nb, SPL + lay.off(k), kIB / 8, st);
+    if (su != st) MG_HIP(hipStreamWaitEvent(st, ws.evH, 0));
+    return MG_OK;
   };
   auto inverse = [&](int64_t k, hipStream_t st) -> int {
     const int64_t r0 = k * nb, h = (n - r0 < nb) ? n - r0 : nb;

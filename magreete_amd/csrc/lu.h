@@ -41,7 +41,7 @@ struct Workspace {
   hipStream_t s3 = nullptr;        // a third stream beside the look-ahead (default priority)
   hipStream_t s4 = nullptr;        // split LDU: Uh panels, which only the solves read (default priority)
   hipEvent_t evA = nullptr, evB = nullptr, evC = nullptr, evD = nullptr, evE = nullptr, evF = nullptr,
-             evG = nullptr;
+             evG = nullptr, evH = nullptr;
   float2* inv_exch = nullptr;      // LDU block inverse: two 256 x 32 published slabs
   int* inv_ctl = nullptr;          // LDU block inverse: two sets of barrier counter [0] and failure flag [32]
   int inv_parity = 0;              // the set the next block inverse uses (it clears the other)

@@ -575,6 +575,7 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipEventCreateWithFlags(&ws.evE, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evF, hipEventDisableTiming));
   MG_HIP(hipEventCreateWithFlags(&ws.evG, hipEventDisableTiming));
+  MG_HIP(hipEventCreateWithFlags(&ws.evH, hipEventDisableTiming));
   ws.ready = true;
   return MG_OK;
 }
@@ -618,6 +619,7 @@ void workspace_free(Workspace& ws) {
   if (ws.evE) (void)hipEventDestroy(ws.evE);
   if (ws.evF) (void)hipEventDestroy(ws.evF);
   if (ws.evG) (void)hipEventDestroy(ws.evG);
+  if (ws.evH) (void)hipEventDestroy(ws.evH);
   ws = Workspace();
 }
 
