@@ -194,6 +194,13 @@ int mg_maxpy(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64
  * order); work: mg_mdot_work_elems(1, T) complex128 elements. */
 int mg_maxpy_norm(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, double alpha, void* w,
                   int64_t ldw, int64_t n, int64_t T, void* work, double* norms, void* stream);
+/* w[r, t] -= sum_i V_i[r, t] h[i ldh + t] (mg_maxpy with alpha = -1), then
+ * out[i ldo + t] = sum_r conj(V_i[r, t]) w[r, t] with the updated w
+ * (mg_mdot's order), for 1 <= k <= 16 (no reference counterpart: the two
+ * middle passes of classical Gram-Schmidt twice in one read of V); work as
+ * mg_mdot's. */
+int mg_maxpy_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, void* w, int64_t ldw,
+                  int64_t n, int64_t T, void* work, void* out, int64_t ldo, void* stream);
 /* norms[t] = ||X[:, t]||_2 of an n x T complex128 matrix (row stride ldx),
  * summed in the fixed order of mg_maxpy_norm; work as there. */
 int mg_colnorm(const void* X, int64_t ldx, int64_t n, int64_t T, void* work, double* norms, void* stream);

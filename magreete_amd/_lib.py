@@ -65,6 +65,7 @@ SIGNATURES = {
     "mg_maxpy": [_vp, _i64, _i64, _int, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _vp],
     "mg_maxpy_norm": [_vp, _i64, _i64, _int, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
     "mg_colnorm": [_vp, _i64, _i64, _i64, _vp, _vp, _vp],
+    "mg_maxpy_mdot": [_vp, _i64, _i64, _int, _vp, _i64, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _vp],
     "mg_hess_solve": [_vp, _vp, _int, _int, _i64, _vp, _vp],
     "mg_split_rows_bytes": [_i64, _i64],
     "mg_split_rows": [_int, _vp, _i64, _i64, _i64, _vp, _vp],

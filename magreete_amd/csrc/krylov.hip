@@ -59,6 +59,79 @@ __global__ __launch_bounds__(256) void k_mdot_part(const double2* __restrict__ V
   }
 }
 
+// The first Gram-Schmidt update fused with the second pass's dots: over row
+// chunk blockIdx.x and 64 columns, w <- w - sum_i V_i h_i (the arithmetic of
+// k_maxpy), then part[(c k + i) T + t] = the chunk's sum of conj(V_i) w (as
+// k_mdot_part) with the updated w; V_i of a row is read once for both.
+constexpr int kFuseMax = 16;   // vectors held in registers
+
+__global__ __launch_bounds__(256) void k_maxpy_mdot(const double2* __restrict__ V, int64_t sV, int64_t ldv, int k,
+                                                    const double2* __restrict__ h, int64_t ldh,
+                                                    double2* __restrict__ w, int64_t ldw, int64_t n, int64_t T,
+                                                    int64_t rows, double2* __restrict__ part) {
+  __shared__ double2 red[4][kVec][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t c = blockIdx.x, t = (int64_t)blockIdx.y * 64 + tx;
+  const int64_t r0 = c * rows, r1 = (r0 + rows < n) ? r0 + rows : n;
+  double2 acc[kFuseMax], hv[kFuseMax];
+#pragma unroll
+  for (int i = 0; i < kFuseMax; ++i) {
+    acc[i] = make_double2(0.0, 0.0);
+    hv[i] = (i < k && t < T) ? h[(int64_t)i * ldh + t] : make_double2(0.0, 0.0);
+  }
+  if (t < T) {
+    for (int64_t r = r0 + ty; r < r1; r += 4) {
+      double2 v[kFuseMax];
+      double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < kFuseMax; ++i) {
+        if (i < k) {
+          v[i] = V[(int64_t)i * sV + r * ldv + t];
+          s.x += v[i].x * hv[i].x - v[i].y * hv[i].y;
+          s.y += v[i].x * hv[i].y + v[i].y * hv[i].x;
+        }
+      }
+      double2 x = w[r * ldw + t];
+      x.x -= s.x;
+      x.y -= s.y;
+      w[r * ldw + t] = x;
+#pragma unroll
+      for (int i = 0; i < kFuseMax; ++i) {
+        if (i < k) {
+          acc[i].x += v[i].x * x.x + v[i].y * x.y;   // conj(v) x
+          acc[i].y += v[i].x * x.y - v[i].y * x.x;
+        }
+      }
+    }
+  }
+  for (int i0 = 0; i0 < k; i0 += kVec) {
+#pragma unroll
+    for (int ii = 0; ii < kVec; ++ii) {
+      double2 a = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < kFuseMax; ++i)
+        if (i == i0 + ii) a = acc[i];
+      red[ty][ii][tx] = a;
+    }
+    __syncthreads();
+    if (ty == 0 && t < T) {
+#pragma unroll
+      for (int ii = 0; ii < kVec; ++ii) {
+        if (i0 + ii < k) {
+          double2 o = red[0][ii][tx];
+#pragma unroll
+          for (int q = 1; q < 4; ++q) {
+            o.x += red[q][ii][tx].x;
+            o.y += red[q][ii][tx].y;
+          }
+          part[(c * k + i0 + ii) * T + t] = o;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // out[i ldo + t] = sum over chunks of part: one workgroup per vector i and 64
 // columns, its 4 waves summing interleaved chunks, combined in a fixed order.
 __global__ __launch_bounds__(256) void k_mdot_reduce(const double2* __restrict__ part, int k, int64_t T, int nchunks,
@@ -316,6 +389,24 @@ int mg_maxpy_norm(const void* V, int64_t sV, int64_t ldv, int k, const void* h, 
                      reinterpret_cast<double2*>(w), ldw, n, T, rows, part);
   MG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)((T + 63) / 64)), dim3(256), 0, s, part, T, kChunks, norms);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+int mg_maxpy_mdot(const void* V, int64_t sV, int64_t ldv, int k, const void* h, int64_t ldh, void* w, int64_t ldw,
+                  int64_t n, int64_t T, void* work, void* out, int64_t ldo, void* stream) {
+  if (!V || !h || !w || !work || !out || k < 1 || k > kFuseMax || n < 0 || T < 0 || ldv < T || ldw < T ||
+      ldh < T || ldo < T)
+    return MG_ERR_ARG;
+  if (T == 0) return MG_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int64_t rows = (n + kChunks - 1) / kChunks;
+  hipLaunchKernelGGL(k_maxpy_mdot, dim3(kChunks, (unsigned)((T + 63) / 64)), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(V), sV, ldv, k, reinterpret_cast<const double2*>(h), ldh,
+                     reinterpret_cast<double2*>(w), ldw, n, T, rows, reinterpret_cast<double2*>(work));
+  MG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mdot_reduce, dim3((unsigned)((T + 63) / 64), (unsigned)k), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(work), k, T, kChunks, reinterpret_cast<double2*>(out), ldo);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }

@@ -408,6 +408,25 @@ def maxpy(V, h, w, alpha=1.0):
     return w
 
 
+def maxpy_mdot(V, h, w, out=None, work=None):
+    """w <- w - V h (per column), then out[i, t] = sum_r conj(V[i, r, t]) w[r, t]
+    with the updated w, in one read of V (k <= 16 vectors)."""
+    lib = L.load()
+    dev = w.device
+    k, n, T = V.shape
+    if out is None:
+        out = torch.empty((k, T), dtype=torch.complex128, device=dev)
+    need = lib.mg_mdot_work_elems(k, T)
+    if work is None or work.numel() < need:
+        work = torch.empty(max(need, 1), dtype=torch.complex128, device=dev)
+    pv, sv, ldv = _vecs(V)
+    with torch.cuda.device(dev):
+        st = lib.mg_maxpy_mdot(pv, sv, ldv, k, _p(h), _ld(h), _p(w), _ld(w), n, T, _p(work), _p(out), _ld(out),
+                               _stream(dev))
+    L.check(st, "mg_maxpy_mdot")
+    return out
+
+
 def maxpy_norm(V, h, w, norms, alpha=1.0, work=None):
     """maxpy, then norms[t] = ||w[:, t]|| of the updated w (fixed summation
     order); work: optional complex128 buffer of mg_mdot_work_elems(1, T)."""

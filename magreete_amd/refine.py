@@ -198,13 +198,15 @@ class _Krylov:
             else:
                 E.relaxed_product(self.relax.get(), self.M.shape[1], self.Z[j], w, prec)
             hn = self.hn
-            for it in range(2):
-                h = E.mdot(self.V[: j + 1], w, work=self.work)
-                if it == 0:
-                    E.maxpy(self.V[: j + 1], h, w, alpha=-1.0)
-                else:
-                    E.maxpy_norm(self.V[: j + 1], h, w, hn, alpha=-1.0, work=self.work)
-                self.H[: j + 1, j] += h
+            Vj = self.V[: j + 1]
+            h1 = E.mdot(Vj, w, work=self.work)
+            if j + 1 <= 16:
+                h2 = E.maxpy_mdot(Vj, h1, w, work=self.work)     # w -= V h1, then V^H w
+            else:
+                E.maxpy(Vj, h1, w, alpha=-1.0)
+                h2 = E.mdot(Vj, w, work=self.work)
+            E.maxpy_norm(Vj, h2, w, hn, alpha=-1.0, work=self.work)
+            self.H[: j + 1, j] += h1 + h2
             w /= torch.where(hn == 0, torch.ones_like(hn), hn)
             E.givens(self.H, self.H0, self.cs, self.sn, self.g, hn, self.bnorm, j, self.est_buf)
         self.iters += 1

@@ -287,3 +287,10 @@ def test_krylov_mdot_maxpy(k, n, T):
     assert np.abs(norms.cpu().numpy() - ref_n).max() < 1e-13 * ref_n.max()
     # read-only column norms, the same fixed summation order
     assert torch.equal(E.colnorm(w2), norms)
+    # the fused update-and-dots pass: the update of maxpy (alpha -1) bit for bit, the dots of mdot
+    w3 = torch.from_numpy(w).to(DEV)
+    w4 = w3.clone()
+    h3 = E.maxpy_mdot(Vd, h, w3)
+    E.maxpy(Vd, h, w4, alpha=-1.0)
+    assert torch.equal(w3, w4)
+    assert torch.equal(h3, E.mdot(Vd, w4))
