@@ -510,7 +510,7 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
     int64_t n, int64_t nrhs, int nblk, int ct, int* ctr, int* next) {
   typedef mg_ps::PTile<NS, BM, BN, WM, WN> Tile;
   constexpr int U = 2 * NS, KG = kIB / 8, KT = KG / 4;
-  __shared__ typename Tile::S sm;
+  __shared__ typename Tile::S sm[2];
   __shared__ int s_ticket, s_ok;
   if (blockIdx.x == 0)   // the next sweep's counters (that launch follows this one on the stream)
     for (int e = threadIdx.x; e <= nblk; e += blockDim.x)
@@ -570,32 +570,46 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
   // One loop over the k-tiles of all steps; the wait for block row j comes
   // before its first fetch.  When the next block row is not complete yet,
   // the current k-tile is multiplied before the wait.
-  int st = 0, kt = 0;
+  // Two LDS stages: k-tile i is multiplied from one while k-tile i + 1,
+  // fetched during that product, is stashed into the other; one barrier per
+  // k-tile.
+  int st = 0, kt = 0, cur = 0;
   if (steps > 0) {
     set_step(0);
     ok = poll_row(jcur, true) == 1;
-    if (ok) tile.template pfetch<true>(Aj, ldga, Xj, ldgx, M, N, 0);
+    if (ok) {
+      tile.template pfetch<true>(Aj, ldga, Xj, ldgx, M, N, 0);
+      tile.pstash(sm[0]);
+    }
+    __syncthreads();
   }
   while (ok && st < steps) {
-    __syncthreads();   // previous k-tile's LDS reads are done
-    tile.pstash(sm);
-    __syncthreads();
+    int nst = st, nkt = kt + 1;
+    if (nkt == KT) {
+      nkt = 0;
+      ++nst;
+    }
+    const bool have_next = nst < steps;
     bool computed = false;
-    if (++kt == KT) {
-      kt = 0;
-      if (++st < steps) {
-        set_step(st);
+    if (have_next) {
+      if (nkt == 0) {
+        set_step(nst);
         int r = poll_row(jcur, false);
-        if (r == 2) {
-          tile.compute(sm);
+        if (r == 2) {   // not complete yet: multiply the current k-tile first
+          tile.compute(sm[cur]);
           computed = true;
           r = poll_row(jcur, true);
         }
         ok = r == 1;
       }
+      if (ok) tile.template pfetch<true>(Aj, ldga, Xj, ldgx, M, N, (int64_t)nkt * 4);
     }
-    if (ok && st < steps) tile.template pfetch<true>(Aj, ldga, Xj, ldgx, M, N, (int64_t)kt * 4);
-    if (!computed) tile.compute(sm);
+    if (!computed) tile.compute(sm[cur]);
+    if (ok && have_next) tile.pstash(sm[cur ^ 1]);
+    __syncthreads();   // the next stage is complete, and this one's reads are done before it is refilled
+    cur ^= 1;
+    st = nst;
+    kt = nkt;
   }
   if (!ok) {
     const float nan = __builtin_nanf("");
@@ -674,7 +688,8 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
 // and the forward sweep's pre-split result), backward sweep.  64 x 64 tiles: on the
 // 12288-row, 256-column solve they measured 1.15 ms per sweep against 1.4 ms
 // for 64 x 128 and 1.2-1.5 ms for 32 x 64, 64 x 32 and 32 x 32 (the sweep's
-// time is its chain of 48 dependent block rows, not its MFMA work).
+// time is its chain of 48 dependent block rows, not its MFMA work); the two
+// LDS stages of the sweep then took it to 1.10 ms (one workgroup per CU).
 template <int NS>
 int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, const u32x4* DS, int64_t np, float2* B,
                int64_t ldb, int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
