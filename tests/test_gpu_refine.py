@@ -196,6 +196,24 @@ def test_relaxed_refinement(method):
     assert colerr(sm.X, X64) < 1e-9
 
 
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_hess_solve_matches_torch(k):
+    """The back substitution for the GMRES coefficients against torch's
+    triangular solve on the same rotated Hessenberg matrices."""
+    say("hess solve", k)
+    rng = np.random.default_rng(11 + k)
+    m, T = 6, 37
+    H = torch.from_numpy(crand((m + 1, m, T), rng)).to(DEV)
+    for i in range(m):
+        H[i, i] += 4.0                      # well-conditioned triangles
+    g = torch.from_numpy(crand((m + 1, T), rng)).to(DEV)
+    y = E.hess_solve(H, g, k)
+    Hk = H[:k, :k].permute(2, 0, 1).contiguous().cpu()
+    ref = torch.linalg.solve_triangular(Hk, g[:k].T.contiguous().unsqueeze(-1).cpu(), upper=True)
+    ref = ref.squeeze(-1).T
+    assert torch.allclose(y.cpu(), ref, rtol=1e-13, atol=1e-13)
+
+
 @pytest.mark.parametrize("j", [0, 1, 4])
 def test_givens_kernel_matches_torch(j):
     """The fused Arnoldi-column kernel against the same rotations written as
