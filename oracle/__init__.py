@@ -26,9 +26,11 @@ How each function is pinned (tests/test_oracle_*.py):
 * ``propagate`` / ``propagate_ss``: N=1 closed form, linearity over columns.
 * LDOS / mean DOS: the site identity (diag of G W G^T from diag(W)) and
   mean_DOS == mean(LDOS).
-* sources: plane-wave closed form and |E0| = 1, Gaussian-beam formula by mpmath.
+* sources: plane-wave closed form and |E0| = 1, Gaussian-beam formula by mpmath;
+  the 2D ``'point'`` sources as a dipole at R^T (-2, 0) through scipy's Hankel
+  functions (rotation covariance of the Green's functions).
 
 Parity unpinned: the multi-candidate branch of the ``regularize`` NaN fix-up
-(several scatterers within ``radius`` of one measurement point) and the
-``'point'`` sources are restated from the source text only.
+(several scatterers within ``radius`` of one measurement point) is restated
+from the source text only.
 """

@@ -72,6 +72,18 @@ def test_sources(kind, src):
     assert (a - b).abs().max() < 1e-12
 
 
+@pytest.mark.parametrize("kind", ["2d_tm", "2d_te"])
+def test_point_sources(kind):
+    """The 2D 'point' sources (a dipole at distance 2), evaluated at the
+    scatterers (inside the unit disk, away from the dipole)."""
+    say("point sources", kind)
+    pts, meas, k0, alpha, dirs = case(kind, 50)
+    a = sources(DUT[kind](pts, source="point"), kind, pts, k0, dirs)
+    b = sources(O.CLASSES[kind](pts, source="point"), kind, pts, k0, dirs)
+    assert a.shape == b.shape and a.device.type == "cpu"
+    assert (a - b).abs().max() <= 1e-12 * b.abs().max()
+
+
 @pytest.mark.parametrize("kind", list(DUT))
 @pytest.mark.parametrize("si", [False, True])
 def test_solve_propagate(kind, si):

@@ -1,0 +1,49 @@
+"""The 2D 'point' sources of the oracle (Transmission2D.py:145-166 for TE,
+662-677 for TM) against closed forms.  Both Green's functions are covariant
+under rotations, G(R x) = R G(x) R^T, so the source evaluated in the frame
+rotated by -theta is a dipole at r_s = R^T (-2, 0) in the original frame,
+with moment R^T (0, sqrt(I)) (TE) or strength sqrt(I) (TM), I = 4 pi k0.
+The references use scipy's Hankel functions and the TE tensor written in H0
+and H2, (i/4) [(H0 - H2)/2 I + H2 R^R^], not the H1/(kR) form of the source."""
+import numpy as np
+import scipy.special as sp
+import torch
+
+from oracle import transmission as O
+
+
+def _rot(theta):
+    c, s = np.cos(-theta), np.sin(-theta)
+    return np.array([[c, -s], [s, c]])
+
+
+def _g_tm(d, k0):
+    return 0.25j * sp.hankel1(0, k0 * np.linalg.norm(d, axis=-1))
+
+
+def _g_te(d, k0):
+    R = np.linalg.norm(d, axis=-1)
+    h0, h2 = sp.hankel1(0, k0 * R), sp.hankel1(2, k0 * R)
+    dd = d[:, :, None] * d[:, None, :] / (R * R)[:, None, None]
+    return 0.25j * (0.5 * (h0 - h2)[:, None, None] * np.eye(2) + h2[:, None, None] * dd)
+
+
+def test_point_sources_2d_closed_form():
+    rng = np.random.default_rng(5)
+    pts = torch.from_numpy(rng.uniform(-1.0, 1.0, size=(40, 2)))
+    k0 = 7.3
+    thetas = np.array([0.0, 0.4, 1.9, -2.7])
+    intensity = 1.0 * k0 * 2.0 * 2.0 * np.pi
+    tm = O.Transmission2D_scalar(pts, source="point").generate_source(pts, k0, thetas, 0.2).numpy()
+    te = O.Transmission2D_vector(pts, source="point").generate_source(pts, k0, thetas, 0.2).numpy()
+    assert tm.shape == (40, 4) and te.shape == (40, 2, 4)
+    x = pts.numpy()
+    for idx, th in enumerate(thetas):
+        R = _rot(th)
+        rs = R.T @ np.array([-2.0, 0.0])
+        p = R.T @ np.array([0.0, np.sqrt(intensity)])
+        ref_tm = np.sqrt(intensity) * _g_tm(x - rs, k0)
+        ref_te = _g_te(x - rs, k0) @ p
+        # kR in [7, 25]: torch's Bessel functions deviate up to ~3e-6 there
+        assert np.max(np.abs(tm[:, idx] - ref_tm)) <= 3e-6 * np.max(np.abs(ref_tm)), th
+        assert np.max(np.abs(te[:, :, idx] - ref_te)) <= 3e-6 * np.max(np.abs(ref_te)), th
