@@ -685,15 +685,17 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
 
 // The LDU solve with pre-split panels: forward sweep, the block-diagonal
 // product with the D inverses (split-bf16 MFMA from the pre-split inverses
-// and the forward sweep's pre-split result), backward sweep.  64 x 64 tiles: on the
-// 12288-row, 256-column solve they measured 1.15 ms per sweep against 1.4 ms
-// for 64 x 128 and 1.2-1.5 ms for 32 x 64, 64 x 32 and 32 x 32 (the sweep's
-// time is its chain of 48 dependent block rows, not its MFMA work); the two
-// LDS stages of the sweep then took it to 1.10 ms (one workgroup per CU).
+// and the forward sweep's pre-split result), backward sweep.  On the
+// 12288-row, 256-column solve with one LDS stage, 64 x 64 tiles measured
+// 1.15 ms per sweep against 1.4 ms for 64 x 128 and 1.2-1.5 ms for 32 x 64,
+// 64 x 32 and 32 x 32 (the sweep's time is its chain of 48 dependent block
+// rows, not its MFMA work); with two LDS stages 64 x 64 took 1.08-1.10 ms (one
+// workgroup per CU) and 64 x 32 1.05-1.07 ms (63 KB: two per CU; the row's
+// last product 8.7 us against 10.7), 0.7 ms less per refined solve.
 template <int NS>
 int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, const u32x4* DS, int64_t np, float2* B,
                int64_t ldb, int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
-  constexpr int BN = 64;
+  constexpr int BN = 32;
   const int64_t nblk = (n + kIB - 1) / kIB;
   const int ct = (int)((nrhs + 63) / 64);
   MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
