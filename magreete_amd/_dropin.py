@@ -92,6 +92,8 @@ class _Base:
 
     def _raise_singular(self, info):
         i = int(info.item())
+        if i < 0:
+            raise RuntimeError("the LU factorisation failed (status %d: a grid barrier of its leaf timed out)" % i)
         if i > 0:
             raise torch.linalg.LinAlgError(
                 "linalg.solve: The solver failed because the input matrix is singular "

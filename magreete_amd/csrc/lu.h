@@ -11,14 +11,19 @@ constexpr int kGetf2MinRows = 64;      // aim for at least this many rows per wo
 constexpr int kGetf2Lds = 48 * 1024;   // dynamic LDS per leaf workgroup (static use stays under 64 KB total)
 
 // Cross-workgroup exchange area of the leaf kernel (double-buffered by
-// column parity).
+// column parity).  The barrier counters come in two sets used by alternate
+// leaf launches of a handle: a launch counts in one set and clears the
+// other for the next launch, so no zeroing launch precedes a leaf.  Each set
+// holds its launch's failure word; a failure also sets the sticky flag,
+// which only the handle's check reads and clears.
 struct Getf2Work {
   double* cand_val = nullptr;   // [2][G] best |re|+|im| per workgroup
   int* cand_row = nullptr;      // [2][G] its row
   void* cand_data = nullptr;    // [2][G][kLeaf] the candidate rows
   void* rowj = nullptr;         // [2][kLeaf] row j before the swap
-  unsigned* bar = nullptr;      // barrier counter
-  int* err = nullptr;           // barrier failure flag
+  unsigned* bar = nullptr;      // the set this launch counts in (kernel argument); in the handle: both sets
+  unsigned* bar_next = nullptr; // the set this launch clears (kernel argument)
+  int* err = nullptr;           // sticky barrier-failure flag
 };
 
 // Optional timing of the LU trailing-update launches with HIP events on the
@@ -36,6 +41,7 @@ struct Workspace {
   int cus = 0;
   int coop_blocks = 0;
   Getf2Work getf2;
+  int leaf_parity = 0;             // the barrier set the next leaf counts in
   UpdateProfile prof;
   hipStream_t s2 = nullptr;        // look-ahead panel stream (default priority)
   hipStream_t s3 = nullptr;        // a third stream beside the look-ahead (default priority)
@@ -90,7 +96,8 @@ void workspace_free(Workspace& ws);
 // that run concurrently on one device keep the sum of their caps within the
 // CU count, so that every leaf can become resident.
 int workspace_leaf_blocks(Workspace& ws, int g);
-// Synchronises the stream and reports a failed grid barrier.
+// Synchronises the stream, reports a failed grid barrier of the LU leaf
+// since the last check and clears that report.
 int workspace_check(Workspace& ws, hipStream_t s);
 
 template <typename T>

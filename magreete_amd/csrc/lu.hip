@@ -18,7 +18,8 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kBarWords = 1024;   // top counter + 8 group counters, 64 words apart
+constexpr int kBarWords = 1024;   // one set: top counter + 8 group counters, 64 words apart, failure word
+constexpr int kErrWord = 1000;    // the set's failure word (past the group counters at 64 .. 512)
 constexpr long long kSpinLimit = 1ll << 22;   // polls (~1 s) before a barrier gives up
 
 template <typename T> struct C2;
@@ -88,9 +89,9 @@ __device__ inline T2 pub_load2(const T2* p) { T2 v; v.x = pub_load(&p->x); v.y =
 // to 8 group counters (g % 8) and the last arriver of a group bumps the top
 // counter, which keeps contention on one address bounded.  All counters
 // start at 0.  A wait that exceeds the spin limit, or sees another
-// workgroup's failure flag, sets the flag and returns, so every workgroup
-// always reaches the end of the kernel.
-__device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err) {
+// workgroup's failure flag, sets the flag (and the sticky flag, if given)
+// and returns, so every workgroup always reaches the end of the kernel.
+__device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err, int* sticky = nullptr) {
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
@@ -112,6 +113,7 @@ __device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err)
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
       if (++spins > kSpinLimit) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sticky) __hip_atomic_store(sticky, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
     }
@@ -122,7 +124,10 @@ __device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err)
 
 // Leaf LU of an m x w panel (w <= kLeaf) at A (row-major, lda).  Workgroup g
 // owns panel rows [g R, g R + R).  ipiv and info use global indices (row0 is
-// the panel's global row, which equals its global column).
+// the panel's global row, which equals its global column).  The launch
+// counts its barriers in wk.bar and clears wk.bar_next for the next leaf.
+// If a barrier fails, the panel is written as NaN and info (if still 0)
+// becomes MG_ERR_SYNC, so the failure reaches the factors and the caller.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restrict__ A, int64_t lda, int m, int w,
                                                    int R, int64_t row0, int* __restrict__ ipiv,
@@ -140,6 +145,8 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
   T2* cdata = reinterpret_cast<T2*>(wk.cand_data);
   T2* rowj = reinterpret_cast<T2*>(wk.rowj);
 
+  int* const lerr = reinterpret_cast<int*>(wk.bar) + kErrWord;
+  for (int i = g * kThreads + tid; i < kBarWords; i += G * kThreads) wk.bar_next[i] = 0u;
   for (int e = tid; e < nr * w; e += kThreads) tile[e] = A[(int64_t)(rb + e / w) * lda + e % w];
   __syncthreads();
 
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
     }
     if (brow != INT_MAX && tid < w) pub_store2(&cdata[(par * G + g) * mg::kLeaf + tid], tile[(brow - rb) * w + tid]);
     if (j >= rb && j < re && tid < w) pub_store2(&rowj[par * mg::kLeaf + tid], tile[(j - rb) * w + tid]);
-    if (G > 1) grid_barrier(wk.bar, (unsigned)j, G, wk.err);
+    if (G > 1) grid_barrier(wk.bar, (unsigned)j, G, lerr, wk.err);
     else __syncthreads();
 
     double pv = -1.0;
@@ -206,6 +213,12 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
         }
       }
     }
+    __syncthreads();
+  }
+  if (G > 1 && __hip_atomic_load(lerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    const T nan = __builtin_nan("");
+    for (int e = tid; e < nr * w; e += kThreads) tile[e] = T2{nan, nan};
+    if (g == 0 && tid == 0) atomicCAS(info, 0, MG_ERR_SYNC);
     __syncthreads();
   }
   for (int e = tid; e < nr * w; e += kThreads) A[(int64_t)(rb + e / w) * lda + e % w] = tile[e];
@@ -447,16 +460,20 @@ static int getf2(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int6
     }
     R = (int)((m + G - 1) / G);
   }
-  hipLaunchKernelGGL(k_zero_ints, dim3(2), dim3(512), 0, s, reinterpret_cast<int*>(ws.getf2.bar), kBarWords);
   T2* Ap = reinterpret_cast<T2*>(A) + k * lda + k;
   int mi = (int)m, wi = (int)w;
+  // alternate barrier sets: this launch counts in one and clears the other
+  // (leaves of a handle are stream-ordered, so the previous leaf is done)
   Getf2Work wk = ws.getf2;
+  wk.bar = ws.getf2.bar + ws.leaf_parity * kBarWords;
+  wk.bar_next = ws.getf2.bar + (1 - ws.leaf_parity) * kBarWords;
   const size_t shmem = (size_t)R * bytes_per_row;
   // G <= the CU count, one workgroup per CU: every workgroup becomes resident
   // once concurrently running kernels (which never wait on this one) drain,
   // and each barrier wait is bounded by the spin limit.
   hipLaunchKernelGGL(k_getf2<T>, dim3(G), dim3(kThreads), shmem, s, Ap, lda, mi, wi, R, k, ipiv, info, wk);
   MG_HIP(hipGetLastError());
+  ws.leaf_parity ^= 1;       // only a launched leaf has cleared the other set
   return MG_OK;
 }
 
@@ -558,7 +575,8 @@ int workspace_init(Workspace& ws) {
   MG_HIP(hipMalloc(&ws.getf2.cand_row, sizeof(int) * 2 * G));
   MG_HIP(hipMalloc(&ws.getf2.cand_data, sizeof(double2) * 2 * G * kLeaf));
   MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kLeaf));
-  MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * kBarWords));
+  MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * 2 * kBarWords));
+  MG_HIP(hipMemset(ws.getf2.bar, 0, sizeof(unsigned) * 2 * kBarWords));
   MG_HIP(hipMalloc(&ws.inv_exch, sizeof(float2) * 2 * 256 * 32));
   MG_HIP(hipMalloc(&ws.inv_ctl, sizeof(int) * 128));
   MG_HIP(hipMemset(ws.inv_ctl, 0, sizeof(int) * 128));
@@ -636,10 +654,15 @@ int workspace_leaf_blocks(Workspace& ws, int g) {
 }
 
 int workspace_check(Workspace& ws, hipStream_t s) {
+  if (!ws.ready) return MG_OK;
   int err = 0;
   MG_HIP(hipMemcpyAsync(&err, ws.getf2.err, sizeof(int), hipMemcpyDeviceToHost, s));
   MG_HIP(hipStreamSynchronize(s));
   if (err) {
+    // reported once: the next factorisation on this handle starts clean
+    // (each leaf launch has its own failure word)
+    MG_HIP(hipMemsetAsync(ws.getf2.err, 0, sizeof(int), s));
+    MG_HIP(hipStreamSynchronize(s));
     set_error("a grid barrier of the LU leaf did not complete");
     return MG_ERR_SYNC;
   }

@@ -259,9 +259,24 @@ def read_update_profile(device):
 
 
 def check_barriers(device):
+    """Synchronises the current stream and raises if a grid barrier of the
+    LU leaf on this thread's handle timed out since the last check (the
+    report is cleared by the check)."""
     lib = L.load()
     with torch.cuda.device(device):
         L.check(lib.mg_check(handle(device), _stream(device)), "mg_check")
+
+
+def check_info(info, pivoted=True):
+    """Raises on a factorisation status read from the device: negative is a
+    failed leaf (its panel was written as NaN), positive a zero pivot
+    (singular M) when the factor is the pivoted LU."""
+    i = int(info.item()) if torch.is_tensor(info) else int(info)
+    if i < 0:
+        raise L.MGError(i, "LU factorisation (a grid barrier of its leaf timed out)")
+    if i > 0 and pivoted:
+        raise torch.linalg.LinAlgError("the system matrix is singular (U[%d,%d] is exactly zero)" % (i, i))
+    return i
 
 
 def cgemm(A, B, C, alpha=1.0, beta=0.0, prec="c128", split=False):

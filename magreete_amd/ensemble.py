@@ -81,6 +81,7 @@ class Ensemble:
                         results[i] = {"X": s.X, "out": out, "stats": dict(s.stats), "info": s.info,
                                       "events": (t0, t1)}
                         del s
+                    E.check_barriers(self.dev)         # this worker's handle; synchronises its stream
                 stream.synchronize()
             except BaseException as e:   # reported to the caller after every worker stopped
                 errors.append(e)
@@ -98,4 +99,5 @@ class Ensemble:
         for r in results:
             t0, t1 = r.pop("events")
             r["ms"] = t0.elapsed_time(t1)
+            E.check_info(r["info"], pivoted=(self.method == "fp64"))
         return results

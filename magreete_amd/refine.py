@@ -13,7 +13,9 @@ updates and the triangular solve for the coefficients run as HIP kernels.
 Convergence is declared on the least-squares residual |g_{k+1}| / ||b|| <=
 tol of every column (the norm of V_{k+1} (beta e1 - H_k y), which equals
 b - A x up to the rounding of the products already taken); a cycle that
-ends unconverged forms that residual explicitly and restarts from it.
+ends unconverged forms that residual explicitly and restarts from it.  When
+relaxed (split-bf16) products were taken, b - A x is formed once more with an
+FP64 product before a solve is reported converged.
 
 The first factorisation is unpivoted (no pivot search, no grid barrier, no
 row swap); the refinement restores the accuracy.  If it breaks down (an exact
@@ -26,21 +28,24 @@ import torch
 from . import _lib as L
 from . import engine as E
 
-# Relaxed Krylov products (relax=True): once the residual estimate has
-# dropped to r, the next product M z may carry an error and still leave the
-# solution within about eps r of the FP64-product one, where eps is what that
-# product level adds to the solution per unit of residual: the error enters
-# the residual multiplied by that direction's coefficient, which is of the
-# size of r (inexact Krylov methods: Simoncini and Szyld 2003, Bouras and
-# Fraysse 2005), and the solution through M^{-1}.  A product takes the
-# cheapest split-bf16 level with eps r <= RELAX_SAFETY tol, else FP64.  eps
-# is twice what each level added on the 3D vector workload with N = 4096
-# (BF16x3 1.8e-4, BF16x6 1.3e-5, measured as the solution's deviation from
-# the all-FP64 refinement divided by r); it exceeds the products' own
-# relative errors there (BF16x3 1.4e-5, BF16x6 4.4e-6 against the FP64
-# product), so the true residual also stays within (1 + RELAX_SAFETY) tol.
-RELAX_LEVELS = (("bf16x3", 4e-4), ("bf16x6", 3e-5))
+# Relaxed Krylov products (relax=True).  In a Krylov method an inexact
+# product M z_j + E_j z_j moves the residual by E_j z_j y_j, and the
+# coefficient y_j is of the size of the residual r at the time of the product
+# (inexact Krylov methods: Simoncini and Szyld 2003, Bouras and Fraysse 2005),
+# so once r is small a product may carry a relative error eps with
+# eps r <= RELAX_SAFETY tol.  eps is measured per solve: the refinement's
+# first product is FP64, and beside it the split-bf16 levels multiply a
+# sample of RELAX_PROBE_COLS of its columns; a level's eps is RELAX_AMP times
+# its largest relative deviation there.  RELAX_AMP covers directions whose
+# product cancels more than the first's and the solution error such a
+# residual perturbation causes beyond GMRES's own residual of the same size
+# (the error passes through M^{-1}).  The residual itself is not left to the
+# bound: a solve that took a relaxed product forms b - M x once more with an
+# FP64 product and continues in FP64 if it misses tol.
+RELAX_LEVELS = ("bf16x3", "bf16x6")      # cheapest first
 RELAX_SAFETY = 0.5
+RELAX_AMP = 16.0
+RELAX_PROBE_COLS = 8
 
 
 class _SplitA:
@@ -124,6 +129,8 @@ class _Krylov:
         self.rel = None
         self.j = self.k = 0
         self.m = 0
+        self.eps = None             # level -> relaxed product error bound, measured at the first product
+        self._eps_dev = None
 
     def _ctx(self):
         import contextlib
@@ -177,13 +184,31 @@ class _Krylov:
 
     def product_prec(self):
         """Precision of the next product: FP64, or with relaxed products the
-        cheapest level whose error bound times the current residual estimate
-        is within RELAX_SAFETY tol."""
-        if self.relax is not None:
-            for prec, eps in RELAX_LEVELS:
-                if eps * self.res <= RELAX_SAFETY * self.tol:
+        cheapest level whose measured error bound times the current residual
+        estimate is within RELAX_SAFETY tol."""
+        if self.relax is not None and self.eps:
+            for prec in RELAX_LEVELS:
+                if self.eps[prec] * self.res <= RELAX_SAFETY * self.tol:
                     return prec
         return "fp64"
+
+    def _probe_levels(self, j, w):
+        """Relative deviations of the relaxed levels from the FP64 product
+        w = M z_j on a sample of its columns (device tensor, one per level)."""
+        T = w.shape[1]
+        cols = torch.arange(0, T, max(1, T // RELAX_PROBE_COLS), device=w.device)[:RELAX_PROBE_COLS]
+        z = self.Z[j][:, cols].contiguous()
+        ref = w[:, cols].contiguous()
+        rn = E.colnorm(ref)
+        rn = torch.where(rn == 0, torch.ones_like(rn), rn)
+        AS = self.relax.get()
+        errs = []
+        for prec in RELAX_LEVELS:
+            out = torch.empty_like(ref)
+            E.relaxed_product(AS, self.M.shape[1], z, out, prec)
+            out -= ref
+            errs.append((E.colnorm(out) / rn).max())
+        return torch.stack(errs)
 
     def product(self):
         """w = M z_j (FP64 or relaxed), classical Gram-Schmidt twice, the
@@ -195,6 +220,8 @@ class _Krylov:
             w = self.V[j + 1]
             if prec == "fp64":
                 E.cgemm(self.M, self.Z[j], w, alpha=1.0, beta=0.0, split=True)
+                if self.relax is not None and self.eps is None and self._eps_dev is None:
+                    self._eps_dev = self._probe_levels(j, w)
             else:
                 E.relaxed_product(self.relax.get(), self.M.shape[1], self.Z[j], w, prec)
             hn = self.hn
@@ -218,6 +245,11 @@ class _Krylov:
         when the cycle ended."""
         with self._ctx():
             est = self.est_buf.item()   # max over columns of |g_{j+1}| / ||b||
+            if self._eps_dev is not None:
+                dev_eps = self._eps_dev.tolist()
+                self._eps_dev = None
+                self.eps = {prec: RELAX_AMP * e for prec, e in zip(RELAX_LEVELS, dev_eps)}
+                self.measured = dict(zip(RELAX_LEVELS, dev_eps))
             if self.gate is not None:
                 gate, self.gate = self.gate, None
                 if not gate():
@@ -246,13 +278,15 @@ class _Krylov:
             return True
 
 
-def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=None, products=None, gate=None):
+def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=None, products=None, gate=None,
+              probe=None):
     """Solve M X = B by flexible GMRES from X = 0, right-preconditioned by F,
     all columns as one batched process on the current stream.
     history: optional list that receives the largest per-column relative
     residual estimate after every product.  relax: a _SplitA of M (relaxed
     products), or None.  products: optional list that receives the
-    precision of every product.  gate: optional callable run once after the
+    precision of every product.  probe: optional dict that receives the
+    relaxed levels' measured relative product errors.  gate: optional callable run once after the
     first host sync (the factorisation has completed by then); False stops
     the refinement unconverged.
     Returns (converged, products, cycles, residual).
@@ -274,6 +308,8 @@ def _gmres_ir(M, F, B, X, bnorm, tol, max_iters, restart, history=None, relax=No
                 break
     if products is not None:
         products.extend(K.products)
+    if probe is not None and getattr(K, "measured", None):
+        probe.update(K.measured)
     return K.ok, K.iters, K.cycles, K.rel
 
 
@@ -342,34 +378,70 @@ class MixedSolver:
     def info(self):
         return self.F.info
 
+    def _true_residual(self, B, X, bnorm):
+        """R = B - M X with an FP64 product; returns (R, max_t ||R_t|| / ||b_t||)."""
+        R = B.clone(memory_format=torch.contiguous_format)
+        E.cgemm(self.M, X, R, alpha=-1.0, beta=1.0, split=True)
+        return R, (E.colnorm(R) / bnorm).max().item()
+
     def solve(self, B, X):
         """X <- M^{-1} B to a relative FP64 residual <= tol per column.
         Returns stats: tier, fallback, iters (products), cycles, residual,
-        history (residual estimate after every product), products (the
-        precision of every product)."""
+        residual_kind, history (residual estimate after every product),
+        products (the precision of every product).
+
+        residual is max_t ||b_t - M x_t|| / ||b_t||.  With FP64 products only
+        it is GMRES's least-squares residual (the Arnoldi relation holds to
+        the rounding of FP64 products; residual_kind 'arnoldi').  When a
+        relaxed product was taken the relation carries that product's error,
+        so the residual is formed once more with an FP64 product (kind
+        'fp64'); if it misses tol, GMRES continues from it with FP64 products
+        only and the reported residual is that continuation's."""
         bnorm = E.colnorm(B.contiguous())
         bnorm = torch.where(bnorm == 0, torch.ones_like(bnorm), bnorm)
         total = 0
         hist = []
         prods = []
+        probe = {}
         while self.tier != "fp64":
             if self.relax and self.AS is None:
                 self.AS = _SplitA(self.M)
             gate, self._gate = self._gate, None
             ok, iters, cycles, rel = _gmres_ir(self.M, self.F, B, X, bnorm, self.tol, self.max_iters, self.restart,
-                                               hist, self.AS, prods, gate)
+                                               hist, self.AS, prods, gate, probe)
             total += iters
+            kind = "arnoldi"
+            if ok and any(p != "fp64" for p in prods):
+                R, rel = self._true_residual(B, X, bnorm)
+                prods.append("fp64-check")
+                total += 1
+                kind = "fp64"
+                hist.append(rel)
+                if rel > self.tol:
+                    # the relaxed products' errors left the true residual above tol:
+                    # solve for the correction from it, FP64 products only
+                    D = torch.empty_like(X)
+                    ok, iters, c2, rel = _gmres_ir(self.M, self.F, R, D, bnorm, self.tol, self.max_iters,
+                                                   self.restart, hist, None, prods, None)
+                    total += iters
+                    cycles += c2
+                    kind = "arnoldi"
+                    X += D
             if ok:
-                return {"tier": self.tier, "fallback": self.fell, "iters": total, "cycles": cycles,
-                        "residual": rel, "history": hist, "products": prods}
+                st = {"tier": self.tier, "fallback": self.fell, "iters": total, "cycles": cycles,
+                      "residual": rel, "residual_kind": kind, "history": hist, "products": prods}
+                if probe:
+                    st["relax_probe"] = dict(probe)
+                return st
             self._escalate()
         # refinement stalled in low precision: complex128 LU
         X.copy_(B)
         self.F.solve(X)
         R = B.clone()
         E.cgemm(self.M, X, R, alpha=-1.0, beta=1.0, split=True)
-        rel = (torch.linalg.norm(R, dim=0) / bnorm).max().item()
-        return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel}
+        rel = (E.colnorm(R) / bnorm).max().item()
+        return {"tier": "fp64", "fallback": True, "iters": total + 1, "cycles": 0, "residual": rel,
+                "residual_kind": "fp64"}
 
 
 def solve_mixed(M, M32, B, X, prec="c64", tol=1e-12, max_iters=30, restart=12, pivot=False, relax=False):

@@ -186,6 +186,20 @@ class _Entry:
             return {"tier": "fp64", "iters": 0}
         return self.solver.solve(B, X)
 
+    def held_bytes(self):
+        """Bytes this entry holds now: a mixed-precision solver that fell
+        back to the complex128 LU holds that factor (a copy of M) instead of
+        its split factors."""
+        n = self.prob.n
+        if self.method != "fp64" and getattr(self.solver, "tier", None) == "fp64":
+            return 16 * n * n + resident_bytes(n, "fp64")
+        return resident_bytes(n, self.method)
+
+    def check(self):
+        """Raises if the entry's factorisation failed (after the stream synchronised)."""
+        info = self.solver.info
+        E.check_info(info, pivoted=(self.method == "fp64"))
+
 
 class Server:
     """Serve requests on one device (see the module docstring)."""
@@ -304,6 +318,9 @@ class Server:
                         ent.done.record(stream)
                         stream.synchronize()
                         t1 = now()
+                        E.check_barriers(self.dev)          # this worker's handle (already synchronised)
+                        if not hit:
+                            ent.check()
                         res = {"rid": r.rid, "kind": r.kind, "n": r.n, "T": r.T, "hit": hit,
                                "wait_ms": 1e3 * (t0 - r.arrival * time_scale),
                                "service_ms": 1e3 * (t1 - t0),
@@ -313,6 +330,14 @@ class Server:
                         with cond:
                             results[i] = res
                             state["reserved"] -= need
+                            held = ent.held_bytes()
+                            if held != ent.bytes:
+                                # a fallback to the complex128 LU changed what the entry holds:
+                                # account for it from now on (the fallback's transient copy
+                                # during the solve was not reserved in advance)
+                                if hit:
+                                    state["reserved"] += held - ent.bytes
+                                ent.bytes = held
                             if hit:
                                 ent.users -= 1
                             else:

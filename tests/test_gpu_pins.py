@@ -1,0 +1,156 @@
+"""Drop-in (GPU) against the oracle on the branches the configurations never
+reach: the regularised NaN fix-up with several candidates, propagate_ss at
+coincident points, the 3D 'point' sources, the "full" self-interaction and
+the periodic Green's functions.  The oracle's side of each is pinned in
+tests/test_oracle_pins.py."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from magreete_amd import Transmission2D as T2
+from magreete_amd import Transmission3D as T3
+from magreete_amd import workloads as W
+from oracle import transmission as O
+
+DUT = {"2d_tm": T2.Transmission2D_scalar, "2d_te": T2.Transmission2D_vector,
+       "3d_vector": T3.Transmission3D_vector, "3d_scalar": T3.Transmission3D_scalar}
+DIM = {"2d_tm": 1, "2d_te": 2, "3d_vector": 3, "3d_scalar": 1}
+TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def colerr(x, ref):
+    x = x.reshape(-1, x.shape[-1])
+    ref = ref.reshape(-1, ref.shape[-1])
+    return (torch.linalg.norm(x - ref, dim=0) / torch.linalg.norm(ref, dim=0)).max().item()
+
+
+def _tiny(kind):
+    k0, alpha = 2.0, 0.05 - 0.01j
+    sp = 2 if kind.startswith("2d") else 3
+    r = np.zeros((4, sp))
+    r[:, 0] = [0.0, 0.05, 1.0, -1.2]
+    r[1, 1] = 0.02
+    r[3, 1] = 0.3
+    pts = np.zeros((3, sp))
+    pts[0, 0] = 0.04
+    pts[1, 0] = 1.01
+    pts[2, :2] = [0.4, 0.9]
+    return torch.from_numpy(r), torch.from_numpy(pts), k0, alpha
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+def test_regularize_multi_candidate(kind):
+    say("regularize multi-candidate", kind)
+    r, pts, k0, alpha = _tiny(kind)
+    d = DIM[kind]
+    radius = 0.1 * (k0 if kind == "2d_te" else 1.0)
+    rng = np.random.default_rng(2)
+    Ek = torch.from_numpy(rng.normal(size=(d * 4, 3)) + 1j * rng.normal(size=(d * 4, 3)))
+    E0 = torch.from_numpy(rng.normal(size=(3, d, 3)) + 1j * rng.normal(size=(3, d, 3)))
+    if d == 1:
+        E0 = E0[:, 0].contiguous()
+    a = DUT[kind](r, source='plane').propagate(pts, Ek, k0, alpha, E0, regularize=True, radius=radius)
+    b = O.CLASSES[kind](r, source='plane').propagate(pts, Ek, k0, alpha, E0, regularize=True, radius=radius)
+    assert a.shape == b.shape
+    assert torch.equal(a[:2], b[:2])                   # copied rows, bit for bit
+    assert colerr(a[2:], b[2:]) < TOL
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+def test_propagate_ss_coincident_and_single(kind):
+    """Measurement points on scatterers take E0_meas at the scatterer's index;
+    N = 1 Born fields."""
+    say("propagate_ss pins", kind)
+    rng = np.random.default_rng(9)
+    sp = 2 if kind.startswith("2d") else 3
+    d = DIM[kind]
+    k0, alpha = 3.0, 0.02 - 0.001j
+    r = torch.from_numpy(rng.uniform(-0.5, 0.5, size=(5, sp)))
+    pts = torch.cat([torch.from_numpy(rng.uniform(1.0, 2.0, size=(6, sp))), r[[3, 1]]])
+    T = 4
+    E0s = torch.from_numpy(rng.normal(size=(d * 5, T)) + 1j * rng.normal(size=(d * 5, T)))
+    shape = (8, d, T) if d > 1 else (8, T)
+    E0m = torch.from_numpy(rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    a = DUT[kind](r, source='plane').propagate_ss(pts, k0, alpha, E0m, E0s)
+    b = O.CLASSES[kind](r, source='plane').propagate_ss(pts, k0, alpha, E0m, E0s)
+    assert a.shape == b.shape
+    assert torch.equal(a[6:], b[6:]) and torch.equal(b[6], E0m[3]) and torch.equal(b[7], E0m[1])
+    assert colerr(a[:6], b[:6]) < TOL
+    one = r[:1]
+    a1 = DUT[kind](one, source='plane').propagate_ss(pts[:6], k0, alpha, E0m[:6], E0s[:d])
+    b1 = O.CLASSES[kind](one, source='plane').propagate_ss(pts[:6], k0, alpha, E0m[:6], E0s[:d])
+    assert colerr(a1, b1) < TOL
+
+
+@pytest.mark.parametrize("kind", ["3d_vector", "3d_scalar"])
+def test_point_sources_3d(kind):
+    say("3d point sources", kind)
+    rng = np.random.default_rng(4)
+    pts = torch.from_numpy(rng.uniform(-1.0, 1.0, size=(60, 3)))
+    k0 = 6.1
+    u = torch.from_numpy(W.fibonacci_sphere(6))
+    p1, _ = W.polarisation_pairs(u.numpy())
+    dut, ora = DUT[kind](pts, source="point"), O.CLASSES[kind](pts, source="point")
+    if kind == "3d_vector":
+        a = dut.generate_source(pts, k0, u, torch.from_numpy(p1), 0.2)
+        b = ora.generate_source(pts, k0, u, torch.from_numpy(p1), 0.2)
+    else:
+        a = dut.generate_source(pts, k0, u, 0.2)
+        b = ora.generate_source(pts, k0, u, 0.2)
+    assert a.shape == b.shape and a.device.type == "cpu"
+    assert (a - b).abs().max() <= 1e-12 * b.abs().max()
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+def test_full_self_interaction(kind):
+    """solve and LDOS with self_interaction_type='full'."""
+    n = {"2d_tm": 300, "2d_te": 200, "3d_vector": 120, "3d_scalar": 300}[kind]
+    say("full self-interaction", kind, n)
+    rng = np.random.default_rng(1)
+    pts = torch.from_numpy(W.unit_disk(n, rng) if kind.startswith("2d") else W.unit_ball(n, rng))
+    k0 = 1.0 / W.mean_nn_spacing(pts.numpy())
+    alpha = {"2d_tm": W.alpha_2d_tm, "2d_te": W.alpha_2d_te}.get(kind, W.alpha_3d)(k0)
+    radius = 0.2 / k0
+    dut, ora = DUT[kind](pts, source='plane'), O.CLASSES[kind](pts, source='plane')
+    T = 6
+    if kind.startswith("2d"):
+        th = 2 * np.pi * np.arange(T) / T
+        E0 = ora.generate_source(pts, k0, th, 0.2)
+    else:
+        u = torch.from_numpy(W.fibonacci_sphere(T))
+        if kind == "3d_scalar":
+            E0 = ora.generate_source(pts, k0, u, 0.2)
+        else:
+            E0 = ora.generate_source(pts, k0, u, torch.from_numpy(W.polarisation_pairs(u.numpy())[0]), 0.2)
+    a = dut.solve(k0, alpha, radius, E0, self_interaction=True, self_interaction_type="full")
+    b = ora.solve(k0, alpha, radius, E0, self_interaction=True, self_interaction_type="full")
+    assert colerr(a, b) < TOL
+    probes = torch.cat([0.5 * pts[:7], pts[:3]])
+    la = dut.LDOS_measurements(probes, k0, alpha, radius, self_interaction_type="full")
+    lb = ora.LDOS_measurements(probes, k0, alpha, radius, self_interaction_type="full")
+    assert (la - lb).abs().max() <= 1e-9 * lb.abs().max()
+
+
+@pytest.mark.parametrize("kind", list(DUT))
+def test_periodic_greens(kind):
+    """greens / torch_greens with a periodic axis, and the in-place wrap of
+    the caller's difference tensor (the reference mutates it)."""
+    say("periodic greens", kind)
+    rng = np.random.default_rng(3)
+    sp = 2 if kind.startswith("2d") else 3
+    pts = torch.from_numpy(rng.uniform(-1, 1, size=(20, sp)))
+    dut, ora = DUT[kind](pts, source='plane'), O.CLASSES[kind](pts, source='plane')
+    gname = "torch_greens" if kind.startswith("2d") else "greens"
+    r = torch.from_numpy(rng.uniform(-2, 2, size=(7, 20, sp)))
+    for per in (["x", "y", "xy"] if sp == 2 else ["x", "yz", "xyz"]):
+        ra, rb = r.clone(), r.clone()
+        a = getattr(dut, gname)(ra, 4.0, periodic=per)
+        b = getattr(ora, gname)(rb, 4.0, periodic=per)
+        assert torch.equal(ra, rb)
+        assert ((a - b).abs() <= 1e-12 * b.abs() + 1e-15).all(), per

@@ -44,6 +44,8 @@ def test_point_sources_2d_closed_form():
         p = R.T @ np.array([0.0, np.sqrt(intensity)])
         ref_tm = np.sqrt(intensity) * _g_tm(x - rs, k0)
         ref_te = _g_te(x - rs, k0) @ p
-        # kR in [7, 25]: torch's Bessel functions deviate up to ~3e-6 there
-        assert np.max(np.abs(tm[:, idx] - ref_tm)) <= 3e-6 * np.max(np.abs(ref_tm)), th
-        assert np.max(np.abs(te[:, :, idx] - ref_te)) <= 3e-6 * np.max(np.abs(ref_te)), th
+        # kR in [7, 25]: torch's j0/y0/j1/y1 differ from scipy's by up to ~3e-8
+        # there, and the sources from the closed forms by up to ~7e-8 of their
+        # largest value
+        assert np.max(np.abs(tm[:, idx] - ref_tm)) <= 3e-7 * np.max(np.abs(ref_tm)), th
+        assert np.max(np.abs(te[:, :, idx] - ref_te)) <= 3e-7 * np.max(np.abs(ref_te)), th
