@@ -59,6 +59,24 @@ struct MemA {
   int64_t lda;
 };
 
+// Matrix elements read and written through the global address space: the
+// operands arrive as untyped pointers, and flat accesses would also count
+// against the LDS counter, so the waits for a k-tile's LDS reads would wait
+// for the next k-tile's prefetch too.
+// (The qualifier applies to the device pass; the host pass never runs these
+// bodies and its vector types take no address-space-qualified operands.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using gcptr = const __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using gptr = T*;
+template <typename T>
+using gcptr = const T*;
+#endif
+
 template <int KIND>
 struct GreenA {
   static constexpr int kind = KIND;
@@ -89,7 +107,7 @@ template <typename T, int BM, int BK, int NT>
 __device__ inline void a_fetch(const MemA& src, int64_t M, int64_t K, int64_t i0, int64_t k0,
                                ARegs<T, BM, BK, NT>& r) {
   typedef typename Cplx<T>::t T2;
-  const T2* A = reinterpret_cast<const T2*>(src.A);
+  gcptr<T2> A = (gcptr<T2>)src.A;
 #pragma unroll
   for (int p = 0; p < r.PER; ++p) {
     const int e = p * NT + threadIdx.x;
@@ -158,7 +176,7 @@ template <typename T, int BN, int BK, int NT>
 __device__ inline void b_fetch(const void* Bp, int64_t ldb, int64_t K, int64_t N, int64_t k0, int64_t j0,
                                BRegs<T, BN, BK, NT>& r) {
   typedef typename Cplx<T>::t T2;
-  const T2* B = reinterpret_cast<const T2*>(Bp);
+  gcptr<T2> B = (gcptr<T2>)Bp;
 #pragma unroll
   for (int p = 0; p < r.PER; ++p) {
     const int e = p * NT + threadIdx.x;
@@ -228,7 +246,7 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
   // alpha and writes (alpha = 0 keeps C = beta C).
   // 4M: acc[0] = Cr, acc[1] = Ci.  3M: acc[0..2] = t1, t2, t3, started at
   // t1 = cr, t2 = 0, t3 = cr + ci so that t1 - t2 and t3 - t1 - t2 carry c.
-  T2* C = reinterpret_cast<T2*>(Cp);
+  gptr<T2> C = (gptr<T2>)Cp;
   constexpr bool kRowsStrided = std::is_same<T, double>::value;
   const bool alpha0 = (alpha.x == 0 && alpha.y == 0);
   const bool cload = !(beta.x == 0 && beta.y == 0) && !alpha0;

@@ -438,7 +438,9 @@ struct UpdSrc {
 // Trailing update of the split factorisation, C -= sum_s A_s B_s with every A
 // (M rows) and B (N columns) pre-split over one 256-block of k (the panels of
 // one to four steps); C complex float row-major.  The accumulators start from
-// -C (loaded before the k-loop).
+// -C (loaded before the k-loop).  The pre-split panels hold whole 256-row
+// blocks (zeros past n) and every update starts on a block boundary, so the
+// operand tiles are read without bounds checks.
 template <int NS, int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void k_ldu_update(int64_t M, int64_t N, UpdSrc src,
                                                              float2* __restrict__ C, int64_t ldc, int nbm, int nbn) {
@@ -457,7 +459,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_ldu_update(int64_t M, int64_t 
   }
   Tile t;
   t.init(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), true);
-  t.pmac_n(sm, A, B, src.n, KG, M - i0, N - j0, KG);
+  static_assert(kIB % BM == 0 && kIB % BN == 0, "tiles within the padded 256-row blocks");
+  t.template pmac_n<true>(sm, A, B, src.n, KG, M - i0, N - j0, KG);
   t.store(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
 

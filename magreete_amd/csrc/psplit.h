@@ -39,6 +39,18 @@ __device__ inline void split8(const float2* v, u32x4* out) {
   for (int q = 0; q < 2 * NS; ++q) out[q] = mg_pack8(t[q]);
 }
 
+// A 16-byte unit read through the global address space.  Pointers that
+// reach a load through a run-time-indexed array lose their address space, and
+// the compiler then emits flat loads, which also count against the LDS
+// counter: every wait for the tile's LDS reads would then wait for the next
+// k-tile's prefetch as well.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(1))) u32x4 g_u32x4;
+#else
+typedef const u32x4 g_u32x4;    // host pass: never executed
+#endif
+__device__ inline u32x4 global_load16(const u32x4* p) { return *(g_u32x4*)p; }
+
 // Agent-scope (cross-XCD coherent) 16-byte load and 8-byte store, for units
 // that other workgroups of the same launch write.
 __device__ inline u32x4 coherent_load16(const u32x4* p) {
@@ -82,11 +94,14 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
   // COHERENT_B: B with agent-scope loads (B written by other workgroups of
   // this launch).  SNA: parts per real / imaginary part stored in A (at
   // least NS); its first NS parts of each are read (A split once into more
-  // parts than a product uses).
-  template <bool COHERENT_B = false, int SNA = NS>
+  // parts than a product uses).  PADDED: both operands hold whole tiles
+  // (rows of A up to a multiple of BM, columns of B up to a multiple of BN,
+  // zeros past M and N), so the loads need no bounds checks.
+  template <bool COHERENT_B = false, int SNA = NS, bool PADDED = false>
   __device__ static void pfetch_to(Stage& st, const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb,
                                    int64_t M, int64_t N, int64_t g0) {
     static_assert(SNA >= NS, "A holds fewer parts than the product uses");
+    static_assert(!(PADDED && COHERENT_B), "padded fetches are plain loads");
     const int tid = Base::ltid();
 #pragma unroll
     for (int c = 0; c < AU; ++c) {
@@ -98,24 +113,27 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
         const int g = e / U, q = e - g * U;
         src = ((int64_t)r * ldga + g0 + g) * (2 * SNA) + (q / NS) * SNA + q % NS;
       }
-      st.a[c] = r < M ? A[src] : u32x4{0u, 0u, 0u, 0u};
+      if constexpr (PADDED) st.a[c] = global_load16(A + src);
+      else st.a[c] = r < M ? global_load16(A + src) : u32x4{0u, 0u, 0u, 0u};
     }
 #pragma unroll
     for (int c = 0; c < BU; ++c) {
       const int u = tid + c * NT, r = u / (4 * U), e = u - r * (4 * U);
-      if (r < N) {
-        const u32x4* p = B + ((int64_t)r * ldgb + g0) * U + e;
+      const u32x4* p = B + ((int64_t)r * ldgb + g0) * U + e;
+      if constexpr (PADDED) {
+        st.b[c] = global_load16(p);
+      } else if (r < N) {
         if constexpr (COHERENT_B) st.b[c] = coherent_load16(p);
-        else st.b[c] = *p;
+        else st.b[c] = global_load16(p);
       } else {
         st.b[c] = u32x4{0u, 0u, 0u, 0u};
       }
     }
   }
-  template <bool COHERENT_B = false, int SNA = NS>
+  template <bool COHERENT_B = false, int SNA = NS, bool PADDED = false>
   __device__ void pfetch(const u32x4* __restrict__ A, int64_t ldga, const u32x4* B, int64_t ldgb, int64_t M,
                          int64_t N, int64_t g0) {
-    pfetch_to<COHERENT_B, SNA>(cur, A, ldga, B, ldgb, M, N, g0);
+    pfetch_to<COHERENT_B, SNA, PADDED>(cur, A, ldga, B, ldgb, M, N, g0);
   }
 
   // A fetched k-tile into the LDS planes (no conversion).
@@ -151,18 +169,20 @@ struct PTile : mg_sg::Tile<NS, BM, BN, WM, WN> {
   }
 
   // acc += A_0 B_0 + ... + A_{nsrc-1} B_{nsrc-1}, every source KG groups (a
-  // multiple of 4) with ldg groups per row of A and per column of B.
+  // multiple of 4) with ldg groups per row of A and per column of B
+  // (PADDED: see pfetch_to).
+  template <bool PADDED = false>
   __device__ void pmac_n(S& sm, const u32x4* const* A, const u32x4* const* B, int nsrc, int64_t ldg, int64_t M,
                          int64_t N, int64_t KG) {
     const int64_t KT = KG / 4, T = nsrc * KT;
-    pfetch(A[0], ldg, B[0], ldg, M, N, 0);
+    pfetch<false, NS, PADDED>(A[0], ldg, B[0], ldg, M, N, 0);
     for (int64_t t = 0; t < T; ++t) {
       __syncthreads();   // previous tile's reads are done
       pstash(sm);
       __syncthreads();
       if (t + 1 < T) {
         const int64_t src = (t + 1) / KT, kt = (t + 1) - src * KT;
-        pfetch(A[src], ldg, B[src], ldg, M, N, kt * 4);
+        pfetch<false, NS, PADDED>(A[src], ldg, B[src], ldg, M, N, kt * 4);
       }
       this->compute(sm);
     }
