@@ -92,6 +92,22 @@ int64_t mg_dinv_elems(int64_t n);
 int mg_getrf_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, void* dinv, void* stream);
 int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv,
                   void* B, int64_t ldb, int64_t nrhs, void* work, void* stream);
+/* d x d diagonal blocks of M^{-1} for the sites listed in `sites` (int64,
+ * device; block s covers rows and columns d*sites[s] .. d*sites[s]+d-1),
+ * from getrf's complex128 factors and mg_getrf_dinv's inverses.  Replaces
+ * the full inverse W = solve(M, eye) that batch_LDOS_measurements forms
+ * (Transmission2D.py:416, 911; Transmission3D.py:403, 916) when the LDOS is
+ * measured at the scatterers themselves: there every term of
+ * diag(G W G^T) reduces to W's diagonal blocks.  LU is overwritten by
+ * U^{-1} and L^{-1} (the factor is consumed); a site index outside
+ * [0, n / d) gives a NaN block.  colperm (int32, device, n):
+ * the position ipiv's row swaps move each row to.  out: nsites * d * d
+ * complex128, row-major blocks; work: mg_inverse_diag_work_elems(n)
+ * complex128.  prec must be MG_C128. */
+int64_t mg_inverse_diag_work_elems(int64_t n);
+int mg_inverse_diag_blocks(mg_handle* h, int prec, void* LU, int64_t n, int64_t lda, const void* dinv,
+                           const int* colperm, int d, const int64_t* sites, int64_t nsites, void* out, void* work,
+                           int64_t work_elems, void* stream);
 /* Block LDU factorisation without pivoting, in place (complex float; prec
  * MG_C64, MG_BF16X3 or MG_BF16X6): A = Lh D Uh with 256 x 256 diagonal blocks; dinv
  * (mg_ldu_factor_elems(prec, n) elements, 16-byte aligned) receives the

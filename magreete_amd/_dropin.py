@@ -195,6 +195,61 @@ class _Base:
             del GT, X
         return torch.cat(out) if out else torch.zeros(0, dtype=torch.complex128, device=dev)
 
+    # LDOS at points that coincide with scatterers (and regularize off): from
+    # the diagonal blocks of M^{-1} instead of solves against G_p^T
+    ldos_site_blocks = True
+
+    def _site_index(self, pts):
+        """For every point, the index of the scatterer at exactly its position
+        (the points whose Green's-function entries are NaN), or -1."""
+        lut = {}
+        for i, row in enumerate(self.r.detach().cpu().tolist()):
+            lut.setdefault(tuple(row), i)
+        return onp.array([lut.get(tuple(p), -1) for p in pts.detach().cpu().tolist()], dtype=onp.int64)
+
+    def _site_dots(self, W, k0, alpha, radius, self_interaction, si_type):
+        """diag(G_p W G_p^T) at scatterer positions from W's d x d diagonal
+        blocks.  There G_p = (dval I - M) / c + F with c = k0^2 alpha, dval
+        M's diagonal value and F the blocks (S/V) ones(d, d) that the NaN
+        fix-up puts on the self blocks (0 without self-interaction); M is
+        complex-symmetric and W M = I, so the block of G_p W G_p^T is
+        dval (dval W - I) / c^2 + (F (dval W - I) + (dval W - I) F) / c + F W F."""
+        d = self.d
+        c = complex(alpha * k0 * k0)
+        dval, f = 1.0, 0.0
+        if self_interaction:
+            sh = complex(self._shift(k0, alpha, radius, si_type))
+            dval = 1.0 - complex(onp.complex64(sh))        # as _system assembles M's diagonal
+            f = complex(self._self_int(k0, radius, si_type) / self._volume(radius))
+        eye = torch.eye(d, dtype=torch.complex128, device=W.device)
+        A = dval * W - eye
+        D = dval * A / (c * c)
+        if f != 0.0:
+            Fm = torch.full((d, d), f, dtype=torch.complex128, device=W.device)
+            D = D + (Fm @ A + A @ Fm) / c + Fm @ W @ Fm
+        return torch.diagonal(D, dim1=1, dim2=2).reshape(-1)
+
+    def _dots(self, F, pts, k0, alpha, radius, self_interaction, si_type, regularize):
+        """Rows of diag(G_p W G_p^T) for all points (d per point): probes by
+        solves against G_p^T, then the scatterer positions from the diagonal
+        blocks of M^{-1}, which consumes F."""
+        dev = F.device
+        d = self.d
+        npts = pts.shape[0]
+        site = self._site_index(pts) if (self.ldos_site_blocks and not regularize) else onp.full(npts, -1)
+        out = torch.empty((npts, d), dtype=torch.complex128, device=dev)
+        probes = onp.nonzero(site < 0)[0]
+        sites = onp.nonzero(site >= 0)[0]
+        if probes.size:
+            sel = torch.from_numpy(probes)
+            dots = self._ldos_dots(F, pts[sel], k0, radius, self_interaction, si_type, regularize)
+            out[sel.to(dev)] = dots.reshape(-1, d)
+        if sites.size:
+            W = E.inverse_diag_blocks(F, torch.from_numpy(site[sites]), d)
+            out[torch.from_numpy(sites).to(dev)] = self._site_dots(W, k0, alpha, radius, self_interaction,
+                                                                  si_type).reshape(-1, d)
+        return out.reshape(-1)
+
     def _ldos_finish(self, dots, npts, pref, alpha, discard_absorption):
         alpha_ = onp.real(alpha) if discard_absorption else alpha
         ldos = torch.imag(dots * (pref * alpha_))
@@ -207,7 +262,7 @@ class _Base:
     def _ldos(self, measure_points, k0, alpha, radius, self_interaction, si_type, regularize, discard_absorption,
               pref):
         F = self._factor(k0, alpha, radius, self_interaction, si_type)
-        dots = self._ldos_dots(F, measure_points, k0, radius, self_interaction, si_type, regularize)
+        dots = self._dots(F, measure_points, k0, alpha, radius, self_interaction, si_type, regularize)
         E.check_barriers(F.device)
         self._raise_singular(F.info)
         return _like(self._ldos_finish(dots, measure_points.shape[0], pref, alpha, discard_absorption),
@@ -215,20 +270,23 @@ class _Base:
 
     def _batch_ldos(self, batches, k0, alpha, radius, self_interaction, si_type, regularize, discard_absorption,
                     pref):
+        """All batches' points in one pass: the result is the concatenation of
+        the per-batch LDOS (Transmission3D.py:428-439), each point's value
+        independent of its batch."""
         F = self._factor(k0, alpha, radius, self_interaction, si_type)
-        outs = []
-        for b in range(len(batches)):
-            pts = batches[b]
-            dots = self._ldos_dots(F, pts, k0, radius, self_interaction, si_type, regularize)
-            outs.append(self._ldos_finish(dots, pts.shape[0], pref, alpha, discard_absorption))
+        batches = list(batches)
+        if not batches:
+            return torch.zeros(0, dtype=torch.float64)
+        pts = torch.cat([b.to(torch.float64).reshape(-1, self.space).cpu() for b in batches])
+        dots = self._dots(F, pts, k0, alpha, radius, self_interaction, si_type, regularize)
         E.check_barriers(F.device)
         self._raise_singular(F.info)
-        return _like(torch.cat(outs), batches[0] if len(batches) else None)
+        return _like(self._ldos_finish(dots, pts.shape[0], pref, alpha, discard_absorption), batches[0])
 
     def _mean_dos(self, measure_points, k0, alpha, radius, self_interaction, si_type, regularize,
                   discard_absorption, pref):
         F = self._factor(k0, alpha, radius, self_interaction, si_type)
-        dots = self._ldos_dots(F, measure_points, k0, radius, self_interaction, si_type, regularize)
+        dots = self._dots(F, measure_points, k0, alpha, radius, self_interaction, si_type, regularize)
         E.check_barriers(F.device)
         self._raise_singular(F.info)
         alpha_ = onp.real(alpha) if discard_absorption else alpha

@@ -142,6 +142,21 @@ int mg_getrs_dinv(mg_handle* h, int prec, const void* LU, int64_t n, int64_t lda
   }
 }
 
+int64_t mg_inverse_diag_work_elems(int64_t n) { return n < 0 ? 0 : mg::inverse_diag_work_elems(n); }
+
+// Diagonal blocks of M^{-1} from the pivoted complex128 LU (consumed).
+int mg_inverse_diag_blocks(mg_handle* h, int prec, void* LU, int64_t n, int64_t lda, const void* dinv,
+                           const int* colperm, int d, const int64_t* sites, int64_t nsites, void* out, void* work,
+                           int64_t work_elems, void* stream) {
+  if (!h || !LU || !dinv || !colperm || (nsites > 0 && (!sites || !out)) || (n > 0 && !work)) return MG_ERR_ARG;
+  if (prec != MG_C128) {
+    mg::set_error("inverse diagonal blocks: complex128 factors only (the LDOS needs them to FP64 accuracy)");
+    return MG_ERR_UNSUPPORTED;
+  }
+  return mg::inverse_diag_blocks(h->ws, LU, n, lda, dinv, colperm, d, sites, nsites, out, work, work_elems,
+                                 reinterpret_cast<hipStream_t>(stream));
+}
+
 int64_t mg_ldu_dinv_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_dinv_elems(n); }
 int64_t mg_ldu_work_elems(int64_t n) { return n < 0 ? 0 : mg::ldu_work_elems(n); }
 

@@ -77,6 +77,18 @@ template <typename T>
 using gcptr = const T*;
 #endif
 
+// A triangular operand read from LU-factor storage (both triangles share
+// one matrix): UPPER takes the upper triangle with the diagonal (entries
+// below it read as 0); LOWER_UNIT the strict lower triangle with a unit
+// diagonal (entries above it read as 0).  Row and column indices are those
+// of the operand, a principal submatrix of the storage.
+struct TriA {
+  enum { UPPER = 0, LOWER_UNIT = 1 };
+  const void* A;
+  int64_t lda;
+  int mode;
+};
+
 template <int KIND>
 struct GreenA {
   static constexpr int kind = KIND;
@@ -87,6 +99,11 @@ struct GreenA {
   int flags;
   double radius;
 };
+
+template <typename X>
+struct IsGreenA : std::false_type {};
+template <int KIND>
+struct IsGreenA<GreenA<KIND>> : std::true_type {};
 
 // LDS planes for one stage: [re, im (, re + im)][BK][BM + PAD].  The third
 // plane serves the three-product (Gauss) complex multiply.
@@ -116,6 +133,43 @@ __device__ inline void a_fetch(const MemA& src, int64_t M, int64_t K, int64_t i0
     v.x = 0; v.y = 0;
     if (e < BM * BK && i0 + row < M && k0 + kk < K) v = A[(i0 + row) * src.lda + k0 + kk];
     r.v[p] = v;
+  }
+}
+
+template <typename T, int BM, int BK, int NT>
+__device__ inline void a_fetch(const TriA& src, int64_t M, int64_t K, int64_t i0, int64_t k0,
+                               ARegs<T, BM, BK, NT>& r) {
+  typedef typename Cplx<T>::t T2;
+  gcptr<T2> A = (gcptr<T2>)src.A;
+  const bool upper = src.mode == TriA::UPPER;
+#pragma unroll
+  for (int p = 0; p < r.PER; ++p) {
+    const int e = p * NT + threadIdx.x;
+    const int64_t row = i0 + e / BK, k = k0 + e % BK;
+    T2 v;
+    v.x = 0; v.y = 0;
+    if (e < BM * BK && row < M && k < K) {
+      if (upper ? k >= row : k < row) v = A[row * src.lda + k];
+      else if (k == row) v.x = 1;
+    }
+    r.v[p] = v;
+  }
+}
+
+// k-range of the product that can hold nonzero A entries for rows [i0, i0 + BM).
+__device__ inline void a_krange(const MemA&, int64_t, int, int64_t K, int64_t& lo, int64_t& hi) { lo = 0; hi = K; }
+template <int KIND>
+__device__ inline void a_krange(const GreenA<KIND>&, int64_t, int, int64_t K, int64_t& lo, int64_t& hi) {
+  lo = 0;
+  hi = K;
+}
+__device__ inline void a_krange(const TriA& src, int64_t i0, int BM, int64_t K, int64_t& lo, int64_t& hi) {
+  if (src.mode == TriA::UPPER) {
+    lo = i0 < K ? i0 : K;
+    hi = K;
+  } else {
+    lo = 0;
+    hi = i0 + BM < K ? i0 + BM : K;
   }
 }
 
@@ -225,19 +279,22 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
   const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
   const int li = lane & 15, lk = lane >> 4;
 
-  const int64_t KT = (K + BK - 1) / BK;
-  constexpr bool kMem = std::is_same<ASrc, MemA>::value;
+  // k-tiles [kt0, kt1): those where A can be nonzero for this tile's rows
+  int64_t klo, khi;
+  a_krange(asrc, i0, BM, K, klo, khi);
+  const int64_t kt0 = klo / BK, kt1 = (khi + BK - 1) / BK;
+  constexpr bool kMem = !IsGreenA<ASrc>::value;    // A read from memory (MemA, TriA)
   ARegs<T, BM, BK, NT> ar;
   BRegs<T, BN, BK, NT> br;
 
   // prologue
   if constexpr (kMem) {
-    a_fetch<T, BM, BK, NT>(asrc, M, K, i0, 0, ar);
+    a_fetch<T, BM, BK, NT>(asrc, M, K, i0, kt0 * BK, ar);
     a_store<T, BM, BN, BK, NT, NP>(sm[0], ar);
   } else {
     a_green<ASrc::kind, T, BM, BN, BK, NT, NP>(sm[0], asrc, i0, 0);
   }
-  b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, 0, j0, br);
+  b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, kt0 * BK, j0, br);
 
   // Accumulator item r of lane l holds column l%16 and row l/16 + 4r (f64
   // MFMA) or 4(l/16) + r (f32 MFMA), as measured on gfx950.  The
@@ -286,9 +343,9 @@ __device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx
   b_store<T, BM, BN, BK, NT, NP>(sm[0], br);
   __syncthreads();
 
-  for (int64_t kt = 0; kt < KT; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < KT;
+  for (int64_t kt = kt0; kt < kt1; ++kt) {
+    const int cur = (kt - kt0) & 1;
+    const bool more = kt + 1 < kt1;
     if (more) {
       if constexpr (kMem) a_fetch<T, BM, BK, NT>(asrc, M, K, i0, (kt + 1) * BK, ar);
       b_fetch<T, BN, BK, NT>(Bp, ldb, K, N, (kt + 1) * BK, j0, br);

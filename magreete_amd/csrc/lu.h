@@ -145,6 +145,18 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
 int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void* dinv, void* B, int64_t ldb,
               int64_t nrhs, void* work, hipStream_t s, GemmFn gemm, int sweep_ns);
 
+// d x d diagonal blocks of M^{-1} for the listed sites (block s of rows and
+// columns d sites[s] .. d sites[s] + d - 1), from the complex128 pivoted LU
+// (getrf) and its diagonal-block inverses (diag_inverses), which are
+// overwritten by U^{-1} and L^{-1}.  colperm[c]: the column of L^{-1} P
+// (P M = L U) that equals column c of L^{-1} P's product with the swaps,
+// i.e. the row that ipiv's swaps move row c to.  out: nsites d^2 complex128
+// (row-major blocks); work: inverse_diag_work_elems(n) complex128.
+int64_t inverse_diag_work_elems(int64_t n);
+int inverse_diag_blocks(Workspace& ws, void* LU, int64_t n, int64_t lda, const void* dinv, const int* colperm, int d,
+                        const int64_t* sites, int64_t nsites, void* out, void* work, int64_t work_elems,
+                        hipStream_t s);
+
 // gemm: GEMM of the block solves (nullptr: the FP64/FP32 MFMA GEMM of T).
 template <typename T>
 int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const void* dinv, void* B, int64_t ldb,

@@ -209,6 +209,8 @@ class Factor:
                 self.A = None
         self._work = None
 
+    consumed = False      # set when inverse_diag_blocks overwrote the factor
+
     def work_elems(self, nrhs):
         """Elements of the factor's dtype that a solve with nrhs columns needs as work."""
         if self.pivot:
@@ -220,6 +222,8 @@ class Factor:
         work: work_elems(nrhs) elements of the factor's dtype owned by the
         caller (solves that run at the same time on one factor each bring
         their own); default: a buffer kept by the factor."""
+        if self.consumed:
+            raise RuntimeError("this factor was consumed by inverse_diag_blocks")
         lib = L.load()
         dev = self.device
         need = self.work_elems(B.shape[1])
@@ -240,6 +244,47 @@ class Factor:
                                       _p(self.dinv), _p(B), _ld(B), B.shape[1], _p(work), _stream(dev))
         L.check(st, "mg_getrs")
         return B
+
+
+def colperm_from_ipiv(ipiv):
+    """The position that getrf's row swaps (ipiv, applied in order) move each
+    row to, as an int32 tensor on ipiv's device: with P M = L U, column c of
+    L^{-1} P is column colperm[c] of L^{-1}."""
+    ip = ipiv.cpu().tolist()
+    n = len(ip)
+    perm = list(range(n))
+    for i, j in enumerate(ip):
+        if j != i:
+            perm[i], perm[j] = perm[j], perm[i]
+    q = [0] * n
+    for pos, row in enumerate(perm):
+        q[row] = pos
+    return torch.tensor(q, dtype=torch.int32, device=ipiv.device)
+
+
+def inverse_diag_blocks(F, sites, d):
+    """(len(sites), d, d) complex128: the diagonal blocks of M^{-1} at the
+    given site indices, from a pivoted complex128 Factor of M, which this
+    consumes (its storage is overwritten by the triangular inverses)."""
+    if not F.pivot or F.dtype != torch.complex128 or F.A is None:
+        raise ValueError("inverse_diag_blocks needs a resident complex128 LU with partial pivoting")
+    lib = L.load()
+    dev = F.device
+    n = F.n
+    sites = torch.as_tensor(sites, dtype=torch.int64)
+    if sites.numel() and (int(sites.min()) < 0 or d * (int(sites.max()) + 1) > n):
+        raise ValueError("site indices must lie in [0, n / d)")
+    sites = sites.to(dev).contiguous()
+    out = torch.empty((sites.shape[0], d, d), dtype=torch.complex128, device=dev)
+    q = colperm_from_ipiv(F.ipiv)
+    we = int(lib.mg_inverse_diag_work_elems(n))
+    work = torch.empty(max(we, 1), dtype=torch.complex128, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_inverse_diag_blocks(handle(dev), L.MG_C128, _p(F.A), n, _ld(F.A), _p(F.dinv), _p(q), int(d),
+                                        _p(sites), sites.shape[0], _p(out), _p(work), we, _stream(dev))
+    L.check(st, "mg_inverse_diag_blocks")
+    F.consumed = True
+    return out
 
 
 def profile_updates(device, on):
