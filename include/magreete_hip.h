@@ -161,6 +161,19 @@ int mg_propagate_ws(int kind, const double* meas, int64_t n_meas, const double* 
                     double alpha_re, double alpha_im, int flags, double radius, const void* Ek, int64_t ld_ek,
                     int64_t nrhs, double beta_re, double beta_im, void* E, int64_t ld_e, void* work,
                     int64_t work_bytes, void* stream);
+/* mg_propagate_ws without allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * that forbid hipMalloc).  The split's partial sums go to a fixed device
+ * array of mg_propagate_noalloc_max_bytes() bytes (64 MiB) that the library
+ * holds on every device it runs on until the process ends; a call whose
+ * mg_propagate_work_bytes exceeds it returns MG_ERR_ARG.  Same kernels,
+ * split and summation order as mg_propagate_ws, hence the same result.
+ * Calls that overlap on one device share the array and race. */
+int64_t mg_propagate_noalloc_max_bytes(void);
+int mg_propagate_noalloc(int kind, const double* meas, int64_t n_meas, const double* scat, int64_t n_scat,
+                         double k0, double alpha_re, double alpha_im, int flags, double radius, const void* Ek,
+                         int64_t ld_ek, int64_t nrhs, double beta_re, double beta_im, void* E, int64_t ld_e,
+                         void* stream);
 
 /* H_nu^(1)(x) for nu in {0, 1} (torch_hankel1, Transmission2D.py:46-61, 578-593);
  * exact = 0: the reference's torch values, 1: accurate to ~1e-15. */

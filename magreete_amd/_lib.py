@@ -71,6 +71,9 @@ SIGNATURES = {
     "mg_split_rows": [_int, _vp, _i64, _i64, _i64, _vp, _vp],
     "mg_relaxed_product_work_bytes": [_int, _i64, _i64, _i64],
     "mg_relaxed_product": [_int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
+    "mg_propagate_noalloc_max_bytes": [],
+    "mg_propagate_noalloc": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
+                             _vp, _i64, _vp],
     "mg_inverse_diag_work_elems": [_i64],
     "mg_inverse_diag_blocks": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _int, _vp, _i64, _vp, _vp, _i64, _vp],
 }
@@ -78,7 +81,8 @@ _RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64, "mg_cgem
              "mg_mdot_work_elems": _c.c_int64, "mg_ldu_dinv_elems": _c.c_int64, "mg_ldu_work_elems": _c.c_int64,
              "mg_ldu_factor_elems": _c.c_int64, "mg_ldu_solve_work_elems": _c.c_int64,
              "mg_split_rows_bytes": _c.c_int64, "mg_relaxed_product_work_bytes": _c.c_int64,
-             "mg_propagate_work_bytes": _c.c_int64, "mg_inverse_diag_work_elems": _c.c_int64}
+             "mg_propagate_work_bytes": _c.c_int64, "mg_inverse_diag_work_elems": _c.c_int64,
+             "mg_propagate_noalloc_max_bytes": _c.c_int64}
 
 _lib = None
 

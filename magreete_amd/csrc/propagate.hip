@@ -7,6 +7,7 @@
 // NaN, so the affected output rows are NaN exactly where the reference's are,
 // and the caller applies the reference's row fix-up.
 #include <algorithm>
+#include <mutex>
 
 #include "cgemm.h"
 #include "mg_internal.h"
@@ -172,4 +173,56 @@ extern "C" int mg_propagate_ws(int kind, const double* meas, int64_t n_meas, con
                                void* work, int64_t work_bytes_, void* stream) {
   return propagate_any(kind, meas, n_meas, scat, n_scat, k0, alpha_re, alpha_im, flags, radius, Ek, ld_ek, nrhs,
                        beta_re, beta_im, E, ld_e, work, work_bytes_, stream);
+}
+
+// ---- propagation without allocation -----------------------------------------
+// For callers that cannot allocate (code under graph capture, serving loops
+// that forbid it): the K-split partial sums live in a fixed device array.
+// Every device that runs the entry point holds its own copy of the array
+// until the process ends; calls that overlap on one device share it and
+// race.  Each partial is written whole before it is summed, so nothing in
+// the array needs clearing between calls.
+namespace {
+
+constexpr int64_t kPropScratchBytes = 64ll << 20;
+__device__ __align__(256) unsigned char scratch[kPropScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_dev[64] = {};   // scratch's device address, per device
+
+// The address of this device's copy of scratch (looked up once per device,
+// with that device current).
+int scratch_address(void** out) {
+  int dev = 0;
+  MG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return MG_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    MG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return MG_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t mg_propagate_noalloc_max_bytes(void) { return kPropScratchBytes; }
+
+extern "C" int mg_propagate_noalloc(int kind, const double* meas, int64_t n_meas, const double* scat,
+                                    int64_t n_scat, double k0, double alpha_re, double alpha_im, int flags,
+                                    double radius, const void* Ek, int64_t ld_ek, int64_t nrhs, double beta_re,
+                                    double beta_im, void* E, int64_t ld_e, void* stream) {
+  const int64_t need = mg_propagate_work_bytes(kind, n_meas, n_scat, nrhs);
+  if (need < 0) return MG_ERR_KIND;
+  if (need > kPropScratchBytes) {
+    mg::set_error("propagate_noalloc: the K-split partials need %lld bytes, over the %lld-byte scratch",
+                  (long long)need, (long long)kPropScratchBytes);
+    return MG_ERR_ARG;
+  }
+  void* work = nullptr;
+  if (need > 0) MG_TRY(scratch_address(&work));
+  return propagate_any(kind, meas, n_meas, scat, n_scat, k0, alpha_re, alpha_im, flags, radius, Ek, ld_ek, nrhs,
+                       beta_re, beta_im, E, ld_e, work, need, stream);
 }
