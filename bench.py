@@ -45,7 +45,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--methods", default="fp64,c64,bf16x3,bf16x6")
+    ap.add_argument("--methods", default="fp64,c64,bf16x3,bf16x6,bf16x3-norelax",
+                    help="fp64, c64, bf16x3, bf16x6; a '-norelax' suffix takes every refinement product in FP64")
     ap.add_argument("--n-points", type=int, default=4096)
     ap.add_argument("--n-dirs", type=int, default=128)
     ap.add_argument("--n-measure", type=int, default=2048)
@@ -63,9 +64,10 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256,
                     help="configurations of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--serving", type=int, default=40, help="requests per serving load level (0: skip)")
-    ap.add_argument("--ldos-sites", type=int, default=0,
-                    help="2D TE perturbed-lattice LDOS (sites + 4096 probes) at about this many sites "
-                         "(0: skip; 20000 is the full configuration)")
+    ap.add_argument("--ldos-sites", type=int, default=20000,
+                    help="2D TE perturbed lattice at about this many sites, for each k0 a in --k0a: LDOS at every "
+                         "site and 4096 probes, and the 256 incidence fields in mixed precision (0: skip)")
+    ap.add_argument("--k0a", default="0.6,0.9,1.2,1.5", help="k0 a values of the lattice measurement")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="GiB copied device to device once before the workload (untimed): a transfer of known "
                          "size that calibrates profiler memory counters")
@@ -104,7 +106,9 @@ def main():
     outputs = {}
     for method in [m for m in args.methods.split(",") if m]:
         try:
-            s = Solver(prob, method=method, tol=args.tol, relax=not args.no_relax)
+            base = method.split("-")[0]
+            relax = not args.no_relax and not method.endswith("-norelax")
+            s = Solver(prob, method=base, tol=args.tol, relax=relax)
             for _ in range(args.warmup):
                 s.run()
             torch.cuda.synchronize()
@@ -127,6 +131,8 @@ def main():
                 "update_ms": ms / args.steps, "refine": its[-1],
             }
             outputs[method] = (s.X.clone(), s.out.clone())
+            if method == best_candidate(args):
+                results[method]["phases"] = phase_rooflines(s, prob, args)
             print("# %s: %.3f solves/s, %.1f ms/step, trailing GEMM %.1f TF/s, stats %s" % (
                 method, args.steps / dt, 1e3 * dt / args.steps, results[method]["update_tflops"] or 0, its[-1]),
                 file=sys.stderr, flush=True)
@@ -149,16 +155,28 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import transmission as O
         torch.set_num_threads(os.cpu_count() or 1)
+        # warm-up on a quarter-size problem, then the median of three full solves
+        ww = W.cfg_3d_vector(n_points=max(64, args.n_points // 4), n_dirs=max(2, args.n_dirs // 4),
+                             n_measure=max(16, args.n_measure // 4), seed=args.seed)
+        ow = O.Transmission3D_vector(ww.points, source="plane")
+        ow.propagate(ww.measure, ow.solve(ww.k0, ww.alpha, 0.0, ow.generate_source(ww.points, ww.k0, ww.u, ww.p, 0.2),
+                                          self_interaction=False), ww.k0, ww.alpha,
+                     ow.generate_source(ww.measure, ww.k0, ww.u, ww.p, 0.2))
         o = O.Transmission3D_vector(w.points, source="plane")
         cE0s = o.generate_source(w.points, w.k0, w.u, w.p, 0.2)
         cE0m = o.generate_source(w.measure, w.k0, w.u, w.p, 0.2)
-        t0 = time.perf_counter()
-        cEk = o.solve(w.k0, w.alpha, 0.0, cE0s, self_interaction=False)
-        cE = o.propagate(w.measure, cEk, w.k0, w.alpha, cE0m)
-        tc = time.perf_counter() - t0
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cEk = o.solve(w.k0, w.alpha, 0.0, cE0s, self_interaction=False)
+            cE = o.propagate(w.measure, cEk, w.k0, w.alpha, cE0m)
+            times.append(time.perf_counter() - t0)
+        tc = float(np.median(times))
         cpu = {"value": 1.0 / tc, "unit": "solves/s", "cores": torch.get_num_threads(), "kind": "port",
-               "sample": "one full solve of this workload (assemble + torch.linalg.solve + propagate, "
-                         "complex128) by the oracle, %.1f s" % tc}
+               "sample": "full solves of this workload (assemble + torch.linalg.solve + propagate, complex128) by "
+                         "the oracle: median of 3 after a quarter-size warm-up; %s s" % ", ".join(
+                             "%.2f" % t for t in times),
+               "host": host_info()}
         for m, (X, F) in outputs.items():
             errors[m + "_vs_oracle"] = {"Ek": colerr(X.cpu(), cEk), "E": colerr(F.cpu().reshape(cE.shape), cE),
                                         "I": colerr((F.cpu().abs() ** 2).reshape(cE.shape).sum(1),
@@ -167,7 +185,7 @@ def main():
 
     # headline: fastest method whose error vs the FP64 path is <= 1e-9 (FP64 itself otherwise)
     best = "fp64" if "solves_per_s" in results.get("fp64", {}) else None
-    for m in ("bf16x6", "bf16x3", "c64"):
+    for m in ("bf16x6", "bf16x3", "c64", "bf16x3-norelax"):
         r = results.get(m, {})
         e = errors.get(m + "_vs_fp64")
         if "solves_per_s" in r and e and max(e.values()) <= 1e-9 and not r["refine"].get("fallback"):
@@ -182,12 +200,18 @@ def main():
         r = results.get(method, {})
         if not r.get("update_tflops"):
             return None
-        return {"bound": "mfma", "achieved": r["update_tflops"], "peak": PEAK[method], "unit": "TFLOP/s",
-                "frac": r["update_tflops"] / PEAK[method], "traffic": TRAFFIC.get(method),
-                "kernel": KERNEL[method], "launches_per_step": r["update_launches"] / args.steps,
+        base = method.split("-")[0]
+        return {"bound": "mfma", "achieved": r["update_tflops"], "peak": PEAK[base], "unit": "TFLOP/s",
+                "frac": r["update_tflops"] / PEAK[base], "traffic": TRAFFIC.get(base),
+                "kernel": KERNEL[base], "launches_per_step": r["update_launches"] / args.steps,
                 "ms_per_step": r["update_ms"]}
 
     roof = roofline(best)
+    speedup = {}
+    if "solves_per_s" in results.get("fp64", {}):
+        for m, r in results.items():
+            if m != "fp64" and "solves_per_s" in r:
+                speedup[m] = r["solves_per_s"] / results["fp64"]["solves_per_s"]
     ldos = run_ldos(args, dev) if args.ldos_sites > 0 else None
     ens = run_ensemble(args, dev) if args.ensemble > 0 else None
     srv = run_serving(args, dev) if args.serving > 0 else None
@@ -196,50 +220,164 @@ def main():
         "warmup": args.warmup, "ms_per_step": rb["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": {"fp64": "complex128", "c64": "complex64 LU + complex128 GMRES-IR",
                                        "bf16x3": "complex64 (bf16x3 products) LU + complex128 GMRES-IR",
-                                       "bf16x6": "complex64 (bf16x6 products) LU + complex128 GMRES-IR"}[best]
-        + ("" if best == "fp64" or args.no_relax else " (late Krylov products split-bf16 where the residual allows)"),
+                                       "bf16x6": "complex64 (bf16x6 products) LU + complex128 GMRES-IR"}[best.split("-")[0]]
+        + ("" if best == "fp64" or args.no_relax or best.endswith("-norelax")
+           else " (late Krylov products split-bf16 where the residual allows, FP64 residual check)"),
         "data": "synthetic (seeded uniform ball, Fibonacci directions)",
         "config": {"workload": "3d_vector N=%d (n=%d) T=%d (%d dirs x 2 pol) measure=%d" % (
             args.n_points, n, w.T, args.n_dirs, args.n_measure), "method": best, "k0": w.k0,
-            "seed": args.seed, "tol": args.tol, "relaxed_products": not args.no_relax},
+            "seed": args.seed, "tol": args.tol,
+            "relaxed_products": not args.no_relax and not best.endswith("-norelax")},
         "roofline": roof, "roofline_fp64": roofline("fp64") if best != "fp64" else None,
+        "speedup_vs_fp64": speedup,
+        "rooflines_other": dict(method=best_candidate(args),
+                                **(results.get(best_candidate(args), {}).get("phases") or {})),
         "cpu_baseline": cpu, "methods": results, "errors": errors,
         "ensemble": ens, "serving": srv, "ldos_lattice": ldos,
     }
     print(json.dumps(line))
 
 
+def host_info():
+    """Host cores and CPU model of the machine the CPU baseline ran on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def best_candidate(args):
+    """The method whose other kernels get rooflines (the expected headline method)."""
+    ms = [m for m in args.methods.split(",") if m]
+    return "bf16x3" if "bf16x3" in ms else (ms[0] if ms else None)
+
+
+def phase_rooflines(s, prob, args):
+    """One more solve with HIP events around its phases (on the solve's
+    stream), and one preconditioner application of a fresh split factor:
+    assembly (HBM write: the complex128 matrix and its complex64 copy),
+    propagation (FP64 MFMA, 8 (d M)(d N) T flops, the Green tile evaluated
+    in-kernel), and the factor's two triangular sweeps plus the
+    block-diagonal product (4 n^2 T algorithmic flops per sweep)."""
+    from magreete_amd import engine as E
+    ph = {}
+    s.run(phases=ph)
+    n, T = prob.n, prob.T
+    dM = prob.d * prob.measure.shape[0]
+    out = {}
+    wbytes = n * n * (16 + (8 if s.M32 is not None or s.method != "fp64" else 0))
+    if ph.get("assemble"):
+        t = ph["assemble"] * 1e-3
+        out["assembly"] = {"bound": "hbm", "achieved": wbytes / t / 1e9, "peak": 8000.0, "unit": "GB/s",
+                           "frac": wbytes / t / 8e12, "ms": ph["assemble"],
+                           "bytes": wbytes, "kernel": "k_assemble<3D_VECTOR> (complex128 + complex64 outputs)"}
+    if ph.get("propagate"):
+        fl = 8.0 * dM * n * T
+        t = ph["propagate"] * 1e-3
+        out["propagation"] = {"bound": "mfma", "achieved": fl / t / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                              "frac": fl / t / 78.6e12, "ms": ph["propagate"], "flops": fl,
+                              "kernel": "k_propagate<3D_VECTOR> (+ k_sum_parts)"}
+    if s.method in ("bf16x3", "bf16x6"):
+        M32 = torch.empty((n, n), dtype=torch.complex64, device=s.M.device)
+        E.convert(s.M, M32)
+        F = E.Factor(M32, s.method, pivot=False)
+        B = torch.randn((n, T), dtype=torch.complex64, device=s.M.device)
+        X = B.clone()
+        F.solve(X)
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record(st)
+        for _ in range(reps):
+            X.copy_(B)
+            F.solve(X)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        fl = 8.0 * n * n * T
+        pk = PEAK[s.method]
+        out["ldu_solve"] = {"bound": "mfma", "achieved": fl / (ms * 1e-3) / 1e12, "peak": pk, "unit": "TFLOP/s",
+                            "frac": fl / (ms * 1e-3) / (pk * 1e12), "ms": ms, "flops": fl,
+                            "kernel": "k_ldu_sweep forward + k_d_product + k_ldu_sweep backward (one "
+                                      "preconditioner application, latency-bound chain of 256-row blocks)"}
+        del F, M32
+        torch.cuda.empty_cache()
+    return out
+
+
 def run_ldos(args, dev):
-    """LDOS of a 2D TE perturbed triangular lattice (unit spacing, sigma 0.1, k0 a = 0.6) at every site
-    and at 4096 probes in the disk, through the drop-in's batch_LDOS_measurements (FP64 LU, batches
-    of 1000 points, self_interaction=False: alpha is already dressed); with the CPU oracle timed at
-    1500 sites and scaled by n^3 when the CPU baseline is on."""
+    """The 2D TE perturbed triangular lattice (unit spacing, sigma 0.1, ~args.ldos_sites sites in a disk)
+    at every k0 a of args.k0a:
+      * LDOS at every site and at 4096 probes through the drop-in's batch_LDOS_measurements (batches of
+        1000 points, self_interaction=False: alpha is already dressed): the sites from the diagonal blocks
+        of M^-1 (complex128 LU, in-place triangular inverses), the probes by solves against G_p^T;
+      * the 256 incidence fields by the BF16x3 factor with FP64 GMRES refinement (relative residual
+        1e-10), their error measured against a complex128 LU solve of the same system (untimed).
+    With the CPU baseline on, the oracle's LDOS and field solve at 1500 sites, scaled by n^3."""
     from magreete_amd import workloads as W
     from magreete_amd.Transmission2D import Transmission2D_vector
-    w = W.cfg_2d_te_lattice(n_sites=args.ldos_sites, k0a=0.6, n_angles=8, n_probes=4096, seed=args.seed)
-    n = 2 * w.points.shape[0]
-    pts = torch.cat([w.points, w.probes]).to(dev)
-    cls = Transmission2D_vector(w.points.to(dev), "plane")
-    batches = torch.split(pts, 1000)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ld = cls.batch_LDOS_measurements(batches, w.k0, w.alpha, 0.1, self_interaction=False)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    out = {"workload": "2d_te lattice, %d sites (n=%d), LDOS at sites + 4096 probes, k0 a = 0.6" % (
-        w.points.shape[0], n), "seconds": dt, "points": int(pts.shape[0]), "finite": bool(torch.isfinite(ld).all())}
-    print("# ldos lattice n=%d: %.1f s" % (n, dt), file=sys.stderr, flush=True)
+    from magreete_amd.pipeline import Solver, problem
+    out = {"workload": "2d_te perturbed triangular lattice, ~%d sites, LDOS at sites + 4096 probes, "
+                       "256 incidence fields (BF16x3 LU + FP64 GMRES-IR, tol 1e-10)" % args.ldos_sites,
+           "levels": {}}
+    for k0a in [float(x) for x in args.k0a.split(",") if x]:
+        w = W.cfg_2d_te_lattice(n_sites=args.ldos_sites, k0a=k0a, n_angles=256, n_probes=4096, seed=args.seed)
+        n = 2 * w.points.shape[0]
+        pts = torch.cat([w.points, w.probes]).to(dev)
+        cls = Transmission2D_vector(w.points.to(dev), "plane")
+        batches = torch.split(pts, 1000)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ld = cls.batch_LDOS_measurements(batches, w.k0, w.alpha, 0.1, self_interaction=False)
+        torch.cuda.synchronize()
+        t_ldos = time.perf_counter() - t0
+        del cls, ld
+        torch.cuda.empty_cache()
+        prob = problem(w, dev, measure=False)
+        s = Solver(prob, "bf16x3", tol=1e-10, relax=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run()
+        torch.cuda.synchronize()
+        t_fields = time.perf_counter() - t0
+        X = s.X.clone()
+        st = dict(s.stats)
+        del s
+        torch.cuda.empty_cache()
+        s64 = Solver(prob, "fp64")
+        s64.run()
+        err = colerr(X, s64.X)
+        del s64, X, prob
+        torch.cuda.empty_cache()
+        # LU (8/3) n^3, triangular inverses (8/3) n^3, probe solves 8 n^2 (2 x 4096)
+        fl = (16.0 / 3.0) * n ** 3 + 8.0 * n * n * 2 * 4096
+        lvl = {"n": n, "sites": int(w.points.shape[0]), "ldos_seconds": t_ldos, "ldos_tflops": fl / t_ldos / 1e12,
+               "fields_seconds": t_fields, "fields_err_vs_fp64": err, "fields_products": st.get("products"),
+               "fields_residual": st.get("residual"), "fields_tier": st.get("tier")}
+        out["levels"]["k0a=%.1f" % k0a] = lvl
+        print("# lattice k0a=%.1f n=%d: LDOS %.2f s (%.1f TF/s), fields %.2f s, err %.2e" % (
+            k0a, n, t_ldos, lvl["ldos_tflops"], t_fields, err), file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         from oracle import transmission as O
-        ws = W.cfg_2d_te_lattice(n_sites=1500, k0a=0.6, n_angles=8, n_probes=4096, seed=args.seed)
+        ws = W.cfg_2d_te_lattice(n_sites=1500, k0a=0.6, n_angles=256, n_probes=4096, seed=args.seed)
         o = O.Transmission2D_vector(ws.points, "plane")
         t0 = time.perf_counter()
         o.batch_LDOS_measurements(torch.split(torch.cat([ws.points, ws.probes]), 1000), ws.k0, ws.alpha, 0.1,
                                   self_interaction=False)
+        E0 = o.generate_source(ws.points, ws.k0, ws.thetas, 0.2)
+        o.solve(ws.k0, ws.alpha, 0.0, E0, self_interaction=False)
         tc = time.perf_counter() - t0
         ns = 2 * ws.points.shape[0]
-        out["cpu_oracle_seconds_at_n%d" % ns] = tc
-        out["cpu_oracle_seconds_scaled_n3"] = tc * (n / ns) ** 3
+        nfull = max(l["n"] for l in out["levels"].values()) if out["levels"] else ns
+        out["cpu_oracle"] = {"seconds_at_n": tc, "n": ns, "seconds_scaled_n3": tc * (nfull / ns) ** 3,
+                             "note": "one k0: LDOS (full inverse) + 256 fields; the reference form at full size "
+                                     "needs M, eye and W (~77 GB) in host memory"}
     return out
 
 

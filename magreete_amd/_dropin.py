@@ -17,6 +17,7 @@ import torch
 
 from . import _lib as L
 from . import engine as E
+from . import spectral as S
 
 
 def _like(t, ref):
@@ -294,12 +295,30 @@ class _Base:
         dos = dos * (pref * alpha_)
         return _like(torch.imag(dos), measure_points)
 
-    # -- out of the accelerated path ---------------------------------------------
-    def compute_hamiltonian_DOS(self, *args, **kwargs):
-        raise NotImplementedError("spectral methods (eigvals of G or M) are outside the accelerated path")
+    # -- spectral methods ----------------------------------------------------------
+    # The matrices come from the assembly kernel; the eigen-decomposition runs
+    # on the host (spectral.py).  Results are on the device of self.r.
+    def _on_r(self, t):
+        return t.to(self.r.device) if torch.is_tensor(t) else t
 
-    def compute_eigenvalues_and_scatterer_LDOS(self, *args, **kwargs):
-        raise NotImplementedError("spectral methods (eigvals of G or M) are outside the accelerated path")
+    def compute_hamiltonian_DOS(self, k0, deltas, file_name, write_eigenvalues=True):
+        """Transmission3D.py:440-461, 951-971; Transmission2D.py:455-475, 955-973."""
+        r = self._r()
+        G, _ = E.assemble(self.kind, r, r, k0, scale=S.hamiltonian_prefactor(self.kind, k0),
+                          flags=L.MG_F_ZERO_SELF | self._bf())
+        return self._on_r(S.hamiltonian_dos(G, k0, deltas, file_name, write_eigenvalues))
 
-    def compute_eigenmodes_IPR(self, *args, **kwargs):
-        raise NotImplementedError("spectral methods (eig of M) are outside the accelerated path")
+    def compute_eigenvalues_and_scatterer_LDOS(self, k0, alpha, radius, file_name, self_interaction=True,
+                                               self_interaction_type="Rayleigh", write_eigenvalues=True):
+        """Transmission3D.py:463-500, 973-1009; Transmission2D.py:477-516, 975-1016."""
+        M, _ = self._system(k0, alpha, radius, self_interaction, self_interaction_type)
+        return self._on_r(S.eigen_ldos(M, self.kind, k0, alpha, self.N, file_name, write_eigenvalues))
+
+    def compute_eigenmodes_IPR(self, k0, alpha, radius, file_name, self_interaction=True,
+                               self_interaction_type="Rayleigh", number_eigenmodes=1, write_eigenvalues=True,
+                               sorting_type='IPR'):
+        """Transmission3D.py:502-564, 1011-1079; Transmission2D.py:518-569, 1018-1066."""
+        M, _ = self._system(k0, alpha, radius, self_interaction, self_interaction_type)
+        lam, vecs, ipr = S.eigenmodes_ipr(M, self.kind, k0, alpha, radius, file_name, number_eigenmodes,
+                                          write_eigenvalues, sorting_type)
+        return self._on_r(lam), self._on_r(vecs), self._on_r(ipr)

@@ -661,6 +661,144 @@ class Transmission3D_scalar:
         return torch.cat(out)
 
 
+# ---------------------------------------------------------------------------
+# spectral methods (eigenvalues of G or M, computed with torch's LAPACK)
+# ---------------------------------------------------------------------------
+# per class: Hamiltonian prefactor of G (as a function of k0), the LDOS
+# prefactor's numerator and power of k0, the CSV suffix of the deltas
+SPECTRAL = {
+    "3d_vector": dict(ham=lambda k0: 6 * onp.pi / k0, ldos=(2.0 * onp.pi, 3), deltas_suffix=""),   # Transmission3D.py:451, 498
+    "3d_scalar": dict(ham=lambda k0: 4 * onp.pi / k0, ldos=(4.0 * onp.pi, 3), deltas_suffix=""),   # Transmission3D.py:960, 1004
+    "2d_te": dict(ham=lambda k0: 8.0, ldos=(4.0, 2), deltas_suffix="_vector"),                   # Transmission2D.py:465, 513
+    "2d_tm": dict(ham=lambda k0: 4.0, ldos=(4.0, 2), deltas_suffix="_scalar"),                   # Transmission2D.py:964, 1008
+}
+
+
+def hamiltonian_dos(G, k0, deltas, file_name, write_eigenvalues):
+    """Transmission3D.py:440-461 (and siblings): eigenvalues of the scaled
+    Green's matrix with a zero diagonal, the resolvent mean at deltas - i."""
+    G.fill_diagonal_(0)
+    lambdas = torch.linalg.eigvals(G)
+    k0_ = onp.round(k0 / (2.0 * onp.pi), 1)
+    if write_eigenvalues:
+        onp.savetxt(file_name + '_hamiltonian_lambdas_' + str(k0_) + '_vector.csv',
+                    onp.stack([torch.real(lambdas).numpy(), torch.imag(lambdas).numpy()]).T)
+    resonance = deltas - 1j
+    resolvant = torch.mean(1 / (resonance.reshape(-1, 1) - lambdas.reshape(1, -1)), axis=-1)
+    return torch.imag(resolvant) / onp.pi
+
+
+def eigen_ldos(M, kind, k0, alpha, npoints, file_name, write_eigenvalues):
+    """Transmission3D.py:463-500 (and siblings): eigenvalues of M and the
+    scatterer-averaged LDOS Im(pref sum (1 - delta)^2 / delta / N)."""
+    deltas = torch.linalg.eigvals(M)
+    k0_ = onp.round(k0 / (2.0 * onp.pi), 1)
+    if write_eigenvalues:
+        onp.savetxt(file_name + '_deltas_' + str(k0_) + SPECTRAL[kind]["deltas_suffix"] + '.csv',
+                    onp.stack([torch.real(deltas).numpy(), torch.imag(deltas).numpy()]).T)
+    num, power = SPECTRAL[kind]["ldos"]
+    dos = ((1 - deltas) ** 2 / deltas).sum() / npoints
+    dos *= num / (k0 ** power * alpha)
+    return torch.imag(dos)
+
+
+def eigenmodes_ipr(M, kind, k0, alpha, radius, file_name, number_eigenmodes, write_eigenvalues, sorting_type):
+    """Transmission3D.py:502-564, 1011-1078; Transmission2D.py:518-569,
+    1018-1066 without the plots and prints: eigenpairs of M, inverse
+    participation ratios of the eigenvectors, and the rescaled eigenvalues
+    (3D: Monsarrat's dimensionless lambdas; 2D: 1 - k0^2 alpha delta),
+    sorted by IPR or by damping.  The particle volume is defined whether or
+    not self-interaction is on (the reference's 3D forms use it unbound
+    without self-interaction)."""
+    deltas, vecs = torch.linalg.eig(M)
+    iprs = torch.sum(torch.abs(vecs ** 4), axis=0) / (torch.sum(torch.abs(vecs ** 2), axis=0)) ** 2
+    k0_ = onp.round(k0 / (2.0 * onp.pi), 1)
+    if kind.startswith("3d"):
+        volume = 4 * onp.pi * (radius ** 3) / 3
+        lam = (1.0 - deltas) / (k0 ** 2 * alpha)
+        if kind == "3d_vector":
+            kares = onp.sqrt(1.0 + 3.0 / onp.real(alpha / volume))
+            gamma0 = (2.0 / 9.0) * (1 + 3.0 / onp.real(alpha / volume))
+            lam = 6.0 * volume * k0 ** 2 * lam / (kares * gamma0)
+        else:
+            kares = onp.sqrt(2.0 / onp.real(alpha / volume))
+            gamma0 = 4.0 / (3.0 * onp.real(alpha / volume))
+            lam = 4.0 * volume * k0 ** 2 * lam / (kares * gamma0)
+        name = '_lambdas_'
+    else:
+        lam = 1.0 - k0 ** 2 * alpha * deltas
+        name = '_deltas_'
+    if write_eigenvalues:
+        onp.savetxt(file_name + name + str(k0_) + '.csv',
+                    onp.stack([torch.real(lam).numpy(), torch.imag(lam).numpy(), iprs.numpy()]).T)
+    if sorting_type == 'IPR':
+        iprs, idx = torch.sort(iprs, descending=True)
+        lam = lam[idx]
+        vecs = vecs[:, idx]
+    elif sorting_type == 'damping':
+        idx = torch.argsort(torch.imag(lam), descending=False)
+        lam = lam[idx]
+        iprs = iprs[idx]
+        vecs = vecs[:, idx]
+    else:
+        raise NotImplementedError
+    return lam, vecs[:, 0:number_eigenmodes], iprs
+
+
+def _spectral_methods(kind, d, system_matrix, green_matrix):
+    """The three spectral methods of a class, over its M and zero-self G."""
+    def compute_hamiltonian_DOS(self, k0, deltas, file_name, write_eigenvalues=True):
+        G = SPECTRAL[kind]["ham"](k0) * green_matrix(self, k0)
+        return hamiltonian_dos(G, k0, deltas, file_name, write_eigenvalues)
+
+    def compute_eigenvalues_and_scatterer_LDOS(self, k0, alpha, radius, file_name, self_interaction=True,
+                                               self_interaction_type="Rayleigh", write_eigenvalues=True):
+        M = system_matrix(self, k0, alpha, radius, self_interaction, self_interaction_type)
+        return eigen_ldos(M, kind, k0, alpha, self.r.shape[0], file_name, write_eigenvalues)
+
+    def compute_eigenmodes_IPR(self, k0, alpha, radius, file_name, self_interaction=True,
+                               self_interaction_type="Rayleigh", number_eigenmodes=1, write_eigenvalues=True,
+                               sorting_type='IPR'):
+        M = system_matrix(self, k0, alpha, radius, self_interaction, self_interaction_type)
+        return eigenmodes_ipr(M, kind, k0, alpha, radius, file_name, number_eigenmodes, write_eigenvalues,
+                              sorting_type)
+    return compute_hamiltonian_DOS, compute_eigenvalues_and_scatterer_LDOS, compute_eigenmodes_IPR
+
+
+def _m_2d_te(self, k0, alpha, radius, si, kind):
+    S = self_interaction_integral_vector_2d(k0, radius, kind) if si else 0
+    return _system_matrix(self.G0_vector(None, k0), alpha, k0, S, onp.pi * radius * radius, si, order2d=True)
+
+
+def _m_2d_tm(self, k0, alpha, radius, si, kind):
+    S = self_interaction_integral_scalar_2d(k0, radius, kind) if si else 0
+    return _system_matrix(self.G0(self.r, k0), alpha, k0, S, onp.pi * radius * radius, si, order2d=True)
+
+
+def _m_3d_vector(self, k0, alpha, radius, si, kind):
+    S = self_interaction_integral_vector_3d(k0, radius, kind) if si else 0
+    return _system_matrix(self.G0(None, k0), alpha, k0, S, 4 * onp.pi * (radius ** 3) / 3, si, order2d=False)
+
+
+def _m_3d_scalar(self, k0, alpha, radius, si, kind):
+    S = self_interaction_integral_scalar_3d(k0, radius, kind) if si else 0
+    return _system_matrix(self.G0(None, k0), alpha, k0, S, 4 * onp.pi * (radius ** 3) / 3, si, order2d=False)
+
+
+(Transmission2D_vector.compute_hamiltonian_DOS, Transmission2D_vector.compute_eigenvalues_and_scatterer_LDOS,
+ Transmission2D_vector.compute_eigenmodes_IPR) = _spectral_methods(
+    "2d_te", 2, _m_2d_te, lambda self, k0: self.G0_vector(None, k0))
+(Transmission2D_scalar.compute_hamiltonian_DOS, Transmission2D_scalar.compute_eigenvalues_and_scatterer_LDOS,
+ Transmission2D_scalar.compute_eigenmodes_IPR) = _spectral_methods(
+    "2d_tm", 1, _m_2d_tm, lambda self, k0: self.G0(self.r, k0))
+(Transmission3D_vector.compute_hamiltonian_DOS, Transmission3D_vector.compute_eigenvalues_and_scatterer_LDOS,
+ Transmission3D_vector.compute_eigenmodes_IPR) = _spectral_methods(
+    "3d_vector", 3, _m_3d_vector, lambda self, k0: self.G0(None, k0))
+(Transmission3D_scalar.compute_hamiltonian_DOS, Transmission3D_scalar.compute_eigenvalues_and_scatterer_LDOS,
+ Transmission3D_scalar.compute_eigenmodes_IPR) = _spectral_methods(
+    "3d_scalar", 1, _m_3d_scalar, lambda self, k0: self.G0(None, k0))
+
+
 CLASSES = {
     "2d_tm": Transmission2D_scalar,
     "2d_te": Transmission2D_vector,
