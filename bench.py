@@ -24,8 +24,9 @@ PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0, "bf16x6": 2500.0 / 6
 # the factorisation's trailing-update launches (timed with HIP events on their stream)
 KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
           "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
-          "bf16x3": "k_ldu_update<2, 64, 128, 2, 2> (block-LDU trailing update over pre-split panels, one or two "
-                    "steps per launch; BF16x3 MFMA, 64x128 tiles)",
+          "bf16x3": "k_ldu_update_dma<1|2> (block-LDU trailing update over pre-split panels, one or two steps per "
+                    "launch; BF16x3 MFMA, 128x128 tiles, operands DMA'd to LDS) for updates with a tile per CU, "
+                    "k_ldu_update<2, 64, 128, 2, 2> for the smaller ones",
           "bf16x6": "k_ldu_update<3, 64, 64, 2, 2> (block-LDU trailing update over pre-split panels, one or two "
                     "steps per launch; BF16x6 MFMA, 64x64 tiles)"}
 # HBM bytes per trailing-update launch (average over one factorisation of this workload), measured

@@ -37,14 +37,13 @@ __device__ inline v4f mg_mfma_nega(float a, float b, v4f c) {
 
 // Map a linear workgroup id to (tile_m, tile_n) so that the tiles one XCD
 // works on are contiguous groups of rows: ids are dealt round-robin over the
-// 8 XCDs, so undo that first, then walk the tile grid in row groups of 8.
+// 8 XCDs, so undo that first (for the leading multiple of 8 ids; the few
+// left over keep their place), then walk the tile grid in row groups of 8.
 __device__ inline void mg_tile_order(int bid, int nbm, int nbn, int& tm, int& tn) {
   const int total = nbm * nbn;
   int id = bid;
-  if (total >= 64 && (total % 8) == 0) {
-    const int per = total / 8;
-    id = (bid % 8) * per + bid / 8;
-  }
+  const int per = total / 8;
+  if (total >= 64 && bid < 8 * per) id = (bid % 8) * per + bid / 8;
   const int group = 8;
   const int in_group = group * nbn;
   const int g = id / in_group;
