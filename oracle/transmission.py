@@ -799,6 +799,67 @@ def _m_3d_scalar(self, k0, alpha, radius, si, kind):
     "3d_scalar", 1, _m_3d_scalar, lambda self, k0: self.G0(None, k0))
 
 
+# ---------------------------------------------------------------------------
+# full-field images (magreete.py:568-741), per angle and per grid batch
+# ---------------------------------------------------------------------------
+def image_fields_reference(solver, measurement_points, Ej, thetas, thetas_plot, k0, alpha, beam_waist, batch_size,
+                           incidence=None, regularize=False, radius=0.0):
+    """magreete.py:639-677: for every plotted angle, the grid in chunks of
+    batch_size (torch.split), the incident field of that one angle and a
+    one-column propagate of Ej[:, index]; returns the list over angles of
+    Eall, (P,) for scalar and (P, d) for vector classes.  incidence(angle)
+    gives the 3D (u, p) rows of one angle (utils.vector_3d_u_and_p)."""
+    scalar = solver.d == 1
+    ndim = measurement_points.shape[1]
+    out = []
+    for angle in thetas_plot:
+        index = onp.where((thetas - angle) ** 2 < 1e-8)[0][0]
+        Eall = []
+        for batch_points in torch.split(measurement_points, batch_size):
+            if ndim == 2:
+                E0_meas = solver.generate_source(batch_points, k0, [angle], beam_waist)
+            elif scalar:
+                u_angle, _ = incidence(angle)
+                E0_meas = solver.generate_source(batch_points, k0, u_angle.reshape(1, 3), beam_waist)
+            else:
+                u_angle, p_angle = incidence(angle)
+                E0_meas = solver.generate_source(batch_points, k0, u_angle.reshape(1, 3), p_angle.reshape(1, 3),
+                                                 beam_waist)
+            if scalar:
+                E0_meas = E0_meas.reshape(batch_points.shape[0], 1)
+            else:
+                E0_meas = E0_meas.reshape(batch_points.shape[0], ndim if ndim == 2 else 3, 1)
+            E_meas = solver.propagate(batch_points, Ej[:, index].unsqueeze(-1), k0, alpha, E0_meas,
+                                      regularize=regularize, radius=radius)
+            Eall.append(E_meas)
+        Eall = torch.cat(Eall, dim=0)
+        if not scalar:
+            Eall = Eall.squeeze(-1)
+        out.append(Eall)
+    return out
+
+
+def field_maps_reference(Eall, measurement_points, ngridx, ngridy, scalar):
+    """magreete.py:683-703 for one angle."""
+    if scalar:
+        return {"field": Eall.reshape(ngridy, ngridx)}
+    ndim = measurement_points.shape[1]
+    if ndim == 2:
+        viewing_angle = torch.arctan2(measurement_points[:, 1], measurement_points[:, 0])
+    else:
+        viewing_unit_vector = measurement_points / torch.linalg.norm(measurement_points, axis=-1).unsqueeze(-1)
+    amp = torch.sqrt(torch.sum(torch.absolute(Eall) ** 2, axis=-1))
+    if ndim == 2:
+        lon = Eall[:, 0] * torch.cos(viewing_angle) - Eall[:, 1] * torch.sin(viewing_angle)
+        tra = Eall[:, 0] * torch.sin(viewing_angle) + Eall[:, 1] * torch.cos(viewing_angle)
+    else:
+        lon = torch.sum(Eall * viewing_unit_vector, axis=1)
+        tra = Eall - lon.reshape(-1, 1) * viewing_unit_vector
+        tra = torch.sqrt(torch.sum(torch.absolute(tra) ** 2, axis=1))
+    return {"amplitude": amp.reshape(ngridy, ngridx), "longitudinal": lon.reshape(ngridy, ngridx),
+            "transverse": tra.reshape(ngridy, ngridx)}
+
+
 CLASSES = {
     "2d_tm": Transmission2D_scalar,
     "2d_te": Transmission2D_vector,
