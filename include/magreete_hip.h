@@ -161,6 +161,19 @@ int mg_propagate_ws(int kind, const double* meas, int64_t n_meas, const double* 
                     double alpha_re, double alpha_im, int flags, double radius, const void* Ek, int64_t ld_ek,
                     int64_t nrhs, double beta_re, double beta_im, void* E, int64_t ld_e, void* work,
                     int64_t work_bytes, void* stream);
+/* mg_propagate_ws with the incident field evaluated in the kernels' final
+ * pass instead of read from E: E = alpha G(meas, scat) Ek + E0 where E0 is
+ * the plane wave (src_type 0) or Gaussian beam of waist w (1) of every
+ * column, as mg_sources builds it (dirs: T x 2 of (cos(-theta),
+ * sin(-theta)) in 2D, T x 3 directions in 3D; pol: T x d polarisations for
+ * the vector kinds, else ignored).  E is write-only.  Replaces
+ * generate_source at the measurement points followed by propagate
+ * (Transmission3D.py:79-145, 147-179; Transmission2D.py:93-174, 176-205)
+ * when the caller needs no E0 of its own. */
+int mg_propagate_src(int kind, const double* meas, int64_t n_meas, const double* scat, int64_t n_scat, double k0,
+                     double alpha_re, double alpha_im, int flags, double radius, const void* Ek, int64_t ld_ek,
+                     int64_t nrhs, int src_type, const double* dirs, const double* pol, double w, void* E,
+                     int64_t ld_e, void* work, int64_t work_bytes, void* stream);
 /* mg_propagate_ws without allocation: no counterpart in the reference; for
  * callers that cannot allocate (code under HIP graph capture, serving loops
  * that forbid hipMalloc).  The split's partial sums go to a fixed device

@@ -71,6 +71,8 @@ SIGNATURES = {
     "mg_split_rows": [_int, _vp, _i64, _i64, _i64, _vp, _vp],
     "mg_relaxed_product_work_bytes": [_int, _i64, _i64, _i64],
     "mg_relaxed_product": [_int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
+    "mg_propagate_src": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _int, _vp, _vp,
+                         _dbl, _vp, _i64, _vp, _i64, _vp],
     "mg_propagate_noalloc_max_bytes": [],
     "mg_propagate_noalloc": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
                              _vp, _i64, _vp],

@@ -2,6 +2,7 @@
 // beams, and column dot products for the LDOS.
 #include "green.h"
 #include "mg_internal.h"
+#include "sources.h"
 
 namespace {
 
@@ -18,51 +19,16 @@ __global__ void k_hankel1(int nu, int exact, const double* __restrict__ x, int64
   }
 }
 
-// E[m, a, t] = f_t(x_m) * pol[t, a]; f is exp(i k0 r_par) (plane, type 0) or
-// the paraxial Gaussian beam exp(i k0 r_par - r_perp^2/(w^2 (1 + i a_t)))/sqrt(1 + i a_t)
-// with a_t = 2 r_perp/(w^2 k0) (type 1).  2D: dirs[t] = (cos(-theta), sin(-theta)),
-// r_par = c x - s y, r_perp = s x + c y (Transmission2D.py:118-122, 629-633).
-// 3D: dirs[t] = u_t, r_par = x . u_t, r_perp = |x - r_par u_t| (Transmission3D.py:96-100).
+// E[m, a, t] = f_t(x_m) * pol[t, a] with f the plane wave or Gaussian beam
+// of sources.h (dirs[t]: 2D (cos(-theta), sin(-theta)), 3D the direction).
 __global__ void k_sources(int space, int type, const double* __restrict__ pts, int64_t npts,
                           const double* __restrict__ dirs, int64_t T, const double* __restrict__ pol, int d,
                           double k0, double w, double2* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   for (int64_t m = blockIdx.y; m < npts; m += gridDim.y) {
-    double rpar, rperp;
-    if (space == 2) {
-      const double c = dirs[2 * t], s = dirs[2 * t + 1];
-      const double x = pts[2 * m], y = pts[2 * m + 1];
-      rpar = c * x - s * y;
-      rperp = s * x + c * y;
-    } else {
-      const double ux = dirs[3 * t], uy = dirs[3 * t + 1], uz = dirs[3 * t + 2];
-      const double x = pts[3 * m], y = pts[3 * m + 1], z = pts[3 * m + 2];
-      rpar = x * ux + y * uy + z * uz;
-      const double ex = x - rpar * ux, ey = y - rpar * uy, ez = z - rpar * uz;
-      rperp = sqrt(ex * ex + ey * ey + ez * ez);
-    }
-    double2 e;
-    if (type == 0) {
-      double sn, cs;
-      sincos(rpar * k0, &sn, &cs);
-      e = make_double2(cs, sn);
-    } else {
-      const double a = 2.0 * rperp / (w * w * k0);
-      // q = rperp^2 / (w^2 (1 + i a)) = rperp^2/w^2 * (1 - i a)/(1 + a^2)
-      const double f = rperp * rperp / (w * w) / (1.0 + a * a);
-      const double zr = -f, zi = rpar * k0 + f * a;
-      double sn, cs;
-      sincos(zi, &sn, &cs);
-      const double mag = exp(zr);
-      // principal sqrt(1 + i a)
-      const double r = sqrt(1.0 + a * a);
-      const double sr = sqrt(0.5 * (r + 1.0));
-      const double si = 0.5 * a / sr;   // sr * si = a / 2, no cancellation at small a
-      const double nr = mag * cs, ni = mag * sn;
-      const double den = sr * sr + si * si;
-      e = make_double2((nr * sr + ni * si) / den, (ni * sr - nr * si) / den);
-    }
+    const mg_src_d2 v = mg_source_value(space, type, pts + space * m, dirs + space * t, k0, w);
+    const double2 e = make_double2(v.x, v.y);
     if (d == 1 && pol == nullptr) {
       out[m * T + t] = e;
     } else {

@@ -11,6 +11,7 @@
 
 #include "cgemm.h"
 #include "mg_internal.h"
+#include "sources.h"
 
 namespace {
 
@@ -72,6 +73,34 @@ __global__ __launch_bounds__(256) void k_sum_parts(int64_t M, int64_t N, int par
   E[i * lde + j] = acc;
 }
 
+// The same sum with the incident field evaluated in place of beta E:
+// E[i, t] = sum_q W_q[i, t] + E0[i, t], E0 = f_t(x_m) pol[t, a] for row
+// i = d m + a (sources.h; pol null for scalar fields).  parts == 0: E already
+// holds the propagated part, and the incident field is added to it.
+__global__ __launch_bounds__(256) void k_sum_parts_src(int64_t M, int64_t N, int parts,
+                                                       const double2* __restrict__ W, int space, int type,
+                                                       const double* __restrict__ meas,
+                                                       const double* __restrict__ dirs,
+                                                       const double* __restrict__ pol, int d, double k0, double w,
+                                                       double2* __restrict__ E, int64_t lde) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int64_t i = idx / N, j = idx - i * N;
+  double2 acc = parts > 0 ? W[idx] : E[i * lde + j];
+  for (int q = 1; q < parts; ++q) {
+    const double2 v = W[(int64_t)q * M * N + idx];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  const int64_t m = i / d;
+  const int a = (int)(i - m * d);
+  const mg_src_d2 e = mg_source_value(space, type, meas + space * m, dirs + space * j, k0, w);
+  const double pa = pol ? pol[j * d + a] : 1.0;
+  acc.x += e.x * pa;
+  acc.y += e.y * pa;
+  E[i * lde + j] = acc;
+}
+
 // K parts that give every CU two workgroups (at most 8, each at least 64
 // k-tiles deep); kchunk a multiple of the k-tile.
 template <int KIND>
@@ -119,6 +148,38 @@ int launch(const double* meas, int64_t n_meas, const double* scat, int64_t n_sca
   MG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((M * N + 255) / 256)), dim3(256), 0, s, M, N, parts,
                      reinterpret_cast<const double2*>(work), beta, reinterpret_cast<double2*>(E), lde);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+// E = alpha G(meas, scat) Ek + E0 with the incident field evaluated in the
+// final pass (no E0 buffer is read): the split partials are summed with it,
+// or, unsplit, it is added to the propagated field in place.
+template <int KIND>
+int launch_src(const double* meas, int64_t n_meas, const double* scat, int64_t n_scat, double k0, double2 alpha,
+               int flags, double radius, const void* Ek, int64_t ldek, int64_t nrhs, int type, const double* dirs,
+               const double* pol, double w, void* E, int64_t lde, void* work, int64_t work_bytes_, hipStream_t s) {
+  typedef PropCfg<KIND> P;
+  constexpr int D = (KIND == MG_2D_TE) ? 2 : (KIND == MG_3D_VECTOR ? 3 : 1);
+  constexpr int SP = (KIND == MG_2D_TM || KIND == MG_2D_TE) ? 2 : 3;
+  const int64_t M = D * n_meas, K = D * n_scat, N = nrhs;
+  const int nbm = (int)((M + P::BM - 1) / P::BM), nbn = (int)((N + P::BN - 1) / P::BN);
+  GreenA<KIND> g{meas, scat, n_meas, n_scat, k0, flags, radius};
+  int parts = 1;
+  int64_t kchunk = K;
+  if (work != nullptr) plan_parts<KIND>(M, N, K, parts, kchunk);
+  if (parts > 1 && work_bytes_ < (int64_t)parts * M * N * 16) parts = 1;
+  if (parts == 1) {
+    hipLaunchKernelGGL(k_propagate<KIND>, dim3(nbm * nbn), dim3(256), 0, s, M, N, K, alpha, g, Ek, ldek,
+                       make_double2(0.0, 0.0), E, lde, nbm, nbn, K, (void*)nullptr);
+  } else {
+    hipLaunchKernelGGL(k_propagate<KIND>, dim3(nbm * nbn * parts), dim3(256), 0, s, M, N, K, alpha, g, Ek, ldek,
+                       make_double2(0.0, 0.0), E, lde, nbm, nbn, kchunk, work);
+  }
+  MG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_parts_src, dim3((unsigned)((M * N + 255) / 256)), dim3(256), 0, s, M, N,
+                     parts == 1 ? 0 : parts, reinterpret_cast<const double2*>(work), SP, type, meas, dirs,
+                     D > 1 ? pol : nullptr, D, k0, w, reinterpret_cast<double2*>(E), lde);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
@@ -225,4 +286,28 @@ extern "C" int mg_propagate_noalloc(int kind, const double* meas, int64_t n_meas
   if (need > 0) MG_TRY(scratch_address(&work));
   return propagate_any(kind, meas, n_meas, scat, n_scat, k0, alpha_re, alpha_im, flags, radius, Ek, ld_ek, nrhs,
                        beta_re, beta_im, E, ld_e, work, need, stream);
+}
+
+// Propagation with the plane-wave or beam incident field evaluated in the
+// final pass instead of read from E: E = alpha G Ek + E0 (the same sum as
+// mg_sources followed by mg_propagate_ws with beta 1).
+extern "C" int mg_propagate_src(int kind, const double* meas, int64_t n_meas, const double* scat, int64_t n_scat,
+                                double k0, double alpha_re, double alpha_im, int flags, double radius,
+                                const void* Ek, int64_t ld_ek, int64_t nrhs, int src_type, const double* dirs,
+                                const double* pol, double w, void* E, int64_t ld_e, void* work,
+                                int64_t work_bytes_, void* stream) {
+  if (n_meas < 0 || n_scat < 0 || nrhs < 0 || (src_type != 0 && src_type != 1)) return MG_ERR_ARG;
+  if (n_meas == 0 || nrhs == 0) return MG_OK;
+  if (ld_ek < nrhs || ld_e < nrhs || !dirs) return MG_ERR_ARG;
+  if ((kind == MG_2D_TE || kind == MG_3D_VECTOR) && !pol) return MG_ERR_ARG;
+  if (flags & (MG_F_ZERO_SELF | MG_F_DIAG | MG_F_NANFILL)) return MG_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const double2 al = make_double2(alpha_re, alpha_im);
+  switch (kind) {
+    case MG_2D_TM: return launch_src<MG_2D_TM>(meas, n_meas, scat, n_scat, k0, al, flags, radius, Ek, ld_ek, nrhs, src_type, dirs, pol, w, E, ld_e, work, work_bytes_, s);
+    case MG_2D_TE: return launch_src<MG_2D_TE>(meas, n_meas, scat, n_scat, k0, al, flags, radius, Ek, ld_ek, nrhs, src_type, dirs, pol, w, E, ld_e, work, work_bytes_, s);
+    case MG_3D_VECTOR: return launch_src<MG_3D_VECTOR>(meas, n_meas, scat, n_scat, k0, al, flags, radius, Ek, ld_ek, nrhs, src_type, dirs, pol, w, E, ld_e, work, work_bytes_, s);
+    case MG_3D_SCALAR: return launch_src<MG_3D_SCALAR>(meas, n_meas, scat, n_scat, k0, al, flags, radius, Ek, ld_ek, nrhs, src_type, dirs, pol, w, E, ld_e, work, work_bytes_, s);
+    default: return MG_ERR_KIND;
+  }
 }

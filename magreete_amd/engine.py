@@ -418,6 +418,33 @@ def propagate(kind, meas, scat, k0, alpha, Ek, out, flags=0, radius=0.0, beta=1.
     return out
 
 
+def propagate_src(kind, meas, scat, k0, alpha, Ek, out, beam, dirs, pol, w, flags=0, radius=0.0):
+    """out <- alpha * G(meas, scat) @ Ek + E0(meas), the incident plane waves
+    (beam=False) or Gaussian beams evaluated in the propagation's final pass
+    (dirs, pol as for sources; out (d M, T) is write-only)."""
+    lib = L.load()
+    dev = out.device
+    meas = _pts(meas, dev)
+    scat = _pts(scat, dev)
+    alpha = complex(alpha)
+    dirs = dirs.to(device=dev, dtype=torch.float64).contiguous()
+    if dirs.shape[0] != Ek.shape[1] or out.shape != (DIM[kind] * meas.shape[0], Ek.shape[1]):
+        raise ValueError("propagate_src: dirs (T, S), Ek (d n, T) and out (d M, T) must agree")
+    if pol is not None:
+        pol = pol.to(device=dev, dtype=torch.float64).contiguous()
+        if pol.shape != (dirs.shape[0], DIM[kind]):
+            raise ValueError("propagate_src: pol must be (T, d)")
+    nb = int(lib.mg_propagate_work_bytes(KINDS[kind], meas.shape[0], scat.shape[0], Ek.shape[1]))
+    work = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        st = lib.mg_propagate_src(KINDS[kind], _p(meas), meas.shape[0], _p(scat), scat.shape[0], float(k0),
+                                  alpha.real, alpha.imag, int(flags), float(radius), _p(Ek), _ld(Ek), Ek.shape[1],
+                                  1 if beam else 0, _p(dirs), _p(pol), float(w), _p(out), _ld(out), _p(work), nb,
+                                  _stream(dev))
+    L.check(st, "mg_propagate_src")
+    return out
+
+
 def coldot(Y, X):
     """out[c] = sum_r Y[r, c] X[r, c] (no conjugation), complex128."""
     lib = L.load()

@@ -25,6 +25,7 @@ class Problem:
     measure: torch.Tensor = None   # (M, S) float64, device
     E0m: torch.Tensor = None       # (d M, T) complex128, device: incident field at the measurement points
     bessel_exact: bool = False
+    source: tuple = None           # (beam, dirs, pol, waist): E0m evaluated inside the propagation instead
 
     @property
     def d(self):
@@ -89,8 +90,13 @@ class Solver:
         p = self.p
         if self.out is None:
             return None
-        self.out.copy_(p.E0m)
         flags = L.MG_F_BESSEL_EXACT if p.bessel_exact else 0
+        if p.source is not None:
+            beam, dirs, pol, waist = p.source
+            E.propagate_src(p.kind, p.measure, p.points, p.k0, p.alpha * p.k0 * p.k0, self.X, self.out, beam, dirs,
+                            pol, waist, flags=flags)
+            return self.out
+        self.out.copy_(p.E0m)
         E.propagate(p.kind, p.measure, p.points, p.k0, p.alpha * p.k0 * p.k0, self.X, self.out, flags=flags)
         return self.out
 
@@ -115,34 +121,51 @@ class Solver:
         return out
 
 
-def incident(kind, points, w):
-    """Incident field of workload w at points (device tensor), as (d M, T)
-    complex128: plane waves or beams along w's directions, with the drop-in
-    classes' conventions (2D: dirs (cos(-theta), sin(-theta)), TE polarisation
-    (-sin theta, cos theta); 3D vector: p rebuilt through atan2/arccos)."""
+def source_spec(kind, w, device=None):
+    """(space, beam, dirs, pol) of workload w's incident field with the
+    drop-in classes' conventions (2D: dirs (cos(-theta), sin(-theta)), TE
+    polarisation (-sin theta, cos theta); 3D vector: p rebuilt through
+    atan2/arccos); pol None for scalar fields."""
     beam = w.source == "beam"
     if kind in ("2d_tm", "2d_te"):
         th = torch.as_tensor(w.thetas, dtype=torch.float64).reshape(-1)
         dirs = torch.stack([torch.cos(-th), torch.sin(-th)], dim=1)
         pol = None if kind == "2d_tm" else torch.stack([dirs[:, 1], dirs[:, 0]], dim=1).float().double()
-        out = E.sources(2, beam, points, dirs, pol, w.k0, w.beam_waist)
+        space = 2
     elif kind == "3d_vector":
         from .Transmission3D import _pvec
-        out = E.sources(3, beam, points, w.u, _pvec(w.p), w.k0, w.beam_waist)
+        dirs, pol, space = torch.as_tensor(w.u, dtype=torch.float64), _pvec(w.p), 3
     elif kind == "3d_scalar":
-        out = E.sources(3, beam, points, w.u, None, w.k0, w.beam_waist)
+        dirs, pol, space = torch.as_tensor(w.u, dtype=torch.float64), None, 3
     else:
         raise NotImplementedError(kind)
+    if device is not None:
+        dirs = dirs.to(device).contiguous()
+        pol = None if pol is None else pol.to(device=device, dtype=torch.float64).contiguous()
+    return space, beam, dirs, pol
+
+
+def incident(kind, points, w):
+    """Incident field of workload w at points (device tensor), as (d M, T)
+    complex128: plane waves or beams along w's directions."""
+    space, beam, dirs, pol = source_spec(kind, w)
+    out = E.sources(space, beam, points, dirs, pol, w.k0, w.beam_waist)
     return out.reshape(-1, out.shape[-1]).contiguous()
 
 
-def problem(w, device, measure=True):
+def problem(w, device, measure=True, fused_source=False):
     """The Problem of a workload on a device (positions and incident fields
-    moved there; measurement points when the workload has them and measure)."""
+    moved there; measurement points when the workload has them and measure).
+    fused_source: the incident field at the measurement points is not built;
+    the propagation evaluates it in its final pass."""
     pts = w.points.to(device=device, dtype=torch.float64).contiguous()
     E0s = incident(w.kind, pts, w)
-    meas = E0m = None
+    meas = E0m = src = None
     if measure and w.measure is not None:
         meas = w.measure.to(device=device, dtype=torch.float64).contiguous()
-        E0m = incident(w.kind, meas, w)
-    return Problem(w.kind, pts, w.k0, w.alpha, E0s, meas, E0m)
+        if fused_source:
+            _, beam, dirs, pol = source_spec(w.kind, w, device)
+            src = (beam, dirs, pol, w.beam_waist)
+        else:
+            E0m = incident(w.kind, meas, w)
+    return Problem(w.kind, pts, w.k0, w.alpha, E0s, meas, E0m, source=src)

@@ -35,7 +35,7 @@ from . import _lib as L
 from . import engine as E
 from . import refine
 from . import workloads as W
-from .pipeline import Problem, incident
+from .pipeline import Problem, incident, source_spec
 
 KINDS = ("2d_tm", "2d_te", "3d_vector")
 
@@ -302,19 +302,20 @@ class Server:
                             prob = ent.prob
                             w = workload(r)
                             E0s = incident(r.kind, prob.points, w)
-                            E0m = incident(r.kind, prob.measure, w)
                         else:
                             w = workload(r)
                             pts = w.points.to(self.dev)
                             meas = w.measure.to(self.dev)
-                            prob = Problem(r.kind, pts, w.k0, w.alpha, incident(r.kind, pts, w), meas,
-                                           incident(r.kind, meas, w))
-                            E0s, E0m = prob.E0s, prob.E0m
+                            prob = Problem(r.kind, pts, w.k0, w.alpha, incident(r.kind, pts, w), meas)
+                            E0s = prob.E0s
                             ent = _Entry(prob, self.method, self.tol, stream)
                         X = torch.empty_like(E0s)
                         st = ent.solve(E0s, X)
-                        F = E0m.clone()
-                        E.propagate(r.kind, prob.measure, prob.points, prob.k0, prob.alpha * prob.k0 * prob.k0, X, F)
+                        # fields at the measurement points, the incident part evaluated in the final pass
+                        _, beam, dirs, pol = source_spec(r.kind, w, self.dev)
+                        F = torch.empty((prob.d * prob.measure.shape[0], X.shape[1]), dtype=X.dtype, device=self.dev)
+                        E.propagate_src(r.kind, prob.measure, prob.points, prob.k0, prob.alpha * prob.k0 * prob.k0,
+                                        X, F, beam, dirs, pol, w.beam_waist)
                         ent.done.record(stream)
                         stream.synchronize()
                         t1 = now()
