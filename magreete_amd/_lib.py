@@ -73,6 +73,12 @@ SIGNATURES = {
     "mg_relaxed_product": [_int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
     "mg_propagate_src": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _int, _vp, _vp,
                          _dbl, _vp, _i64, _vp, _i64, _vp],
+    "mg_group_create": [_int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _c.POINTER(_vp)],
+    "mg_group_destroy": [_vp],
+    "mg_group_configure": [_vp, _int, _int, _int],
+    "mg_group_getrf": [_vp, _vp],
+    "mg_group_getrs": [_vp, _vp],
+    "mg_group_launches": [_vp],
     "mg_propagate_noalloc_max_bytes": [],
     "mg_propagate_noalloc": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
                              _vp, _i64, _vp],
@@ -84,7 +90,7 @@ _RESTYPES = {"mg_last_error": _c.c_char_p, "mg_dinv_elems": _c.c_int64, "mg_cgem
              "mg_ldu_factor_elems": _c.c_int64, "mg_ldu_solve_work_elems": _c.c_int64,
              "mg_split_rows_bytes": _c.c_int64, "mg_relaxed_product_work_bytes": _c.c_int64,
              "mg_propagate_work_bytes": _c.c_int64, "mg_inverse_diag_work_elems": _c.c_int64,
-             "mg_propagate_noalloc_max_bytes": _c.c_int64}
+             "mg_propagate_noalloc_max_bytes": _c.c_int64, "mg_group_launches": _c.c_int64}
 
 _lib = None
 

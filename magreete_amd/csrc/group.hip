@@ -1,0 +1,783 @@
+// Grouped LU factorisation and solve of many independent complex matrices of
+// uneven order, in lockstep: every launch of the blocked algorithm serves
+// every problem of the group that still has work at that column.
+//
+// An ensemble of small systems solved one at a time (magreete.py:346-452,
+// one torch.linalg.solve per configuration, Transmission3D.py:205) is bound
+// by the dependency chain of each factorisation: the pivot search of every
+// column, the small triangular solves and the small products inside each
+// panel.  Here the problems of a group advance column by column together, so
+// the chain is paid once per group instead of once per problem, and every
+// trailing update, panel product and triangular solve is one launch over a
+// grouped tile table.
+//
+// Layout: the problems are sorted by decreasing order; problem p owns its
+// own row-major matrix (A, lda, n), pivots ipiv (n ints, 0-based, LAPACK's
+// meaning), info (first zero pivot, 1-based) and right-hand sides (B, ldb,
+// nrhs columns).  The table of these descriptors lives on the device.  A
+// launch serves the problems p < np (those with work at its column); problem
+// p owns workgroups [first[p], first[p+1]) of the launch, and the tile table
+// (first[]) travels in the kernel arguments, so a captured graph carries it.
+//
+// Algorithm (per problem the same as getrf in lu.hip, with its pivoting
+// rule: the largest |re| + |im| on or below the diagonal, lowest row on
+// ties): right-looking with panels of nb columns; a panel is factored
+// recursively (left half, row swaps + triangular solve + product on the
+// right half, right half) down to leaves of at most kGLeaf columns.  A leaf
+// is factored by one workgroup per problem that streams the leaf's rows
+// through the caches, one pass per column (rank-1 update by the previous
+// pivot, scaling of the previous column and this column's pivot search
+// fused), so no workgroup ever waits for another and any number of problems
+// can share a launch.
+#include "cgemm.h"
+#include "lu.h"
+#include "mg_internal.h"
+#include "split_gemm.h"
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+namespace mg {
+
+constexpr int kMaxGroup = 256;   // problems of one group
+
+struct GrpProb {
+  void* A;
+  int64_t lda;
+  int64_t n;
+  int* ipiv;
+  int* info;
+  void* B;
+  int64_t ldb;
+  int64_t pad;
+};
+
+// Workgroups of one grouped launch: problem p < np owns [first[p], first[p+1]).
+struct GrpMap {
+  int np;
+  int first[kMaxGroup + 1];
+};
+
+// Tunables of a group's factorisation.
+struct GrpCfg {
+  int nb = 128;      // panel width (trailing-update depth)
+  int leaf = 16;     // leaf width, <= kGLeaf
+  int tile = 0;      // trailing-update tile: 0 = 128 x 64 (three-product), 1 = 64 x 64
+};
+
+}  // namespace mg
+
+namespace {
+
+using mg::GrpMap;
+using mg::GrpProb;
+
+constexpr int kGLeaf = 16;                    // widest leaf
+constexpr int kGThreads = 1024;               // leaf workgroup: 16 lanes per row, 64 rows per sweep
+constexpr int kGRows = kGThreads / kGLeaf;
+constexpr int kGUnroll = 4;                   // rows in flight per lane
+constexpr int kGTri = 64;                     // widest triangular block of the grouped solves
+
+template <typename T> struct C2;
+template <> struct C2<double> { typedef double2 t; };
+template <> struct C2<float> { typedef float2 t; };
+
+template <typename T2>
+__device__ inline T2 crecip(T2 a) {
+  // Smith's method for 1/(x + iy)
+  T2 r;
+  if (fabs(a.x) >= fabs(a.y)) {
+    const auto q = a.y / a.x, d = a.x + a.y * q;
+    r.x = 1 / d; r.y = -q / d;
+  } else {
+    const auto q = a.x / a.y, d = a.y + a.x * q;
+    r.x = q / d; r.y = -1 / d;
+  }
+  return r;
+}
+
+template <typename T2>
+__device__ inline T2 cmul(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+__device__ inline bool better(double v, int r, double bv, int br) { return v > bv || (v == bv && r < br); }
+
+// The problem owning workgroup `bid`: the last p with first[p] <= bid (the
+// first[] of problems without workgroups repeat their successor's).
+__device__ inline int grp_find(const GrpMap& m, int bid) {
+  int lo = 0, hi = m.np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (m.first[mid] <= bid) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Workgroup ids are dealt round-robin over the 8 XCDs: renumber the leading
+// multiple of 8 so that consecutive ids (one problem's tiles) share an XCD.
+__device__ inline int xcd_local(int bid, int total) {
+  const int per = total / 8;
+  return (total >= 64 && bid < 8 * per) ? (bid % 8) * per + bid / 8 : bid;
+}
+
+__device__ inline int64_t clamp_extent(int64_t end, int64_t n, int64_t start, bool clamp) {
+  const int64_t e = clamp ? (end < n ? end : n) : end;
+  return e > start ? e - start : 0;
+}
+
+// ---- leaf: one workgroup per problem ---------------------------------------
+// Leaf LU of rows [k, n) x columns [k, k + w') of problem blockIdx.x, w' =
+// min(w, n - k).  Lane c of each 16-lane group holds column c of one row.
+// Pass j (0 <= j <= w'): rows r >= j get column j-1 scaled by the inverse of
+// pivot j-1 and columns >= j updated by it, and column j's pivot search runs
+// on the updated values; then rows j and the pivot row swap within the leaf.
+template <typename T>
+__global__ __launch_bounds__(kGThreads) void k_grp_getf2(const GrpProb* __restrict__ tab, int64_t k, int w) {
+  typedef typename C2<T>::t T2;
+  __shared__ T2 urow[kGLeaf];
+  __shared__ T2 sinv;
+  __shared__ int snz;
+  __shared__ double sv[kGThreads / 64];
+  __shared__ int sr[kGThreads / 64];
+  const GrpProb pb = tab[blockIdx.x];
+  if (pb.n <= k) return;
+  const int m = (int)(pb.n - k);
+  const int wp = (int)(w < pb.n - k ? w : pb.n - k);
+  const int64_t lda = pb.lda;
+  T2* A = reinterpret_cast<T2*>(pb.A) + k * lda + k;
+  const int tid = threadIdx.x, c = tid & (kGLeaf - 1), rr = tid / kGLeaf;
+  const int lane = tid & 63, gbase = lane & ~(kGLeaf - 1), wave = tid >> 6;
+  if (tid == 0) {
+    snz = 0;
+    sinv.x = 0;
+    sinv.y = 0;
+  }
+  __syncthreads();
+  for (int j = 0; j <= wp; ++j) {
+    const bool upd = j > 0;
+    const bool nz = snz != 0;
+    const T2 inv = sinv;
+    T2 u;
+    u.x = 0;
+    u.y = 0;
+    if (upd && c < wp) u = urow[c];
+    const bool lc = c < wp && (upd ? c >= j - 1 : c == 0);
+    double best = -1.0;
+    int brow = INT_MAX;
+    for (int base = j; base < m; base += kGRows * kGUnroll) {
+      T2 x[kGUnroll];
+#pragma unroll
+      for (int q = 0; q < kGUnroll; ++q) {
+        const int r = base + rr + q * kGRows;
+        x[q].x = 0;
+        x[q].y = 0;
+        if (lc && r < m) x[q] = A[(int64_t)r * lda + c];
+      }
+#pragma unroll
+      for (int q = 0; q < kGUnroll; ++q) {
+        const int r = base + rr + q * kGRows;
+        if (upd) {
+          T2 l;
+          l.x = __shfl(x[q].x, gbase + j - 1);
+          l.y = __shfl(x[q].y, gbase + j - 1);
+          if (nz) l = cmul(l, inv);
+          if (c == j - 1) {
+            x[q] = l;
+          } else if (c >= j) {
+            x[q].x -= l.x * u.x - l.y * u.y;
+            x[q].y -= l.x * u.y + l.y * u.x;
+          }
+        }
+        if (c == j && r < m) {
+          const double a = fabs((double)x[q].x) + fabs((double)x[q].y);
+          if (a > best) {
+            best = a;
+            brow = r;
+          }
+        }
+        if (lc && r < m) A[(int64_t)r * lda + c] = x[q];
+      }
+    }
+    if (j == wp) break;
+    // workgroup arg-max (every thread ends with the result)
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double ov = __shfl_xor(best, off);
+      const int orow = __shfl_xor(brow, off);
+      if (better(ov, orow, best, brow)) {
+        best = ov;
+        brow = orow;
+      }
+    }
+    if (lane == 0) {
+      sv[wave] = best;
+      sr[wave] = brow;
+    }
+    __syncthreads();   // also orders this pass's stores before the swap below
+    best = sv[0];
+    brow = sr[0];
+#pragma unroll
+    for (int q = 1; q < kGThreads / 64; ++q)
+      if (better(sv[q], sr[q], best, brow)) {
+        best = sv[q];
+        brow = sr[q];
+      }
+    const int p = (brow == INT_MAX) ? j : brow;   // whole column NaN: keep row j
+    if (tid < wp) {
+      const T2 a = A[(int64_t)j * lda + tid];
+      const T2 b = A[(int64_t)p * lda + tid];
+      if (p != j) {
+        A[(int64_t)j * lda + tid] = b;
+        A[(int64_t)p * lda + tid] = a;
+      }
+      urow[tid] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      pb.ipiv[k + j] = (int)(k + p);
+      const T2 piv = urow[j];
+      const bool z = piv.x == 0 && piv.y == 0;
+      snz = z ? 0 : 1;
+      if (z) {
+        sinv.x = 0;
+        sinv.y = 0;
+        atomicCAS(pb.info, 0, (int)(k + j + 1));
+      } else {
+        sinv = crecip(piv);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- row swaps -------------------------------------------------------------
+// Swaps k1 .. k2-1 (clamped to n) of each problem, in order, applied to
+// columns [c0, c1) of its matrix (c1 clamped to n) or of its right-hand
+// sides (rhs: c1 not clamped).  256 columns per workgroup.
+struct GrpSwp {
+  int64_t k1, k2, c0, c1;
+  int rhs;
+};
+
+template <typename T2>
+__global__ __launch_bounds__(256) void k_grp_laswp(const GrpProb* __restrict__ tab, GrpMap map, GrpSwp op) {
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int64_t c = op.c0 + (int64_t)(blockIdx.x - map.first[p]) * 256 + threadIdx.x;
+  const int64_t c1 = op.rhs ? op.c1 : (op.c1 < pb.n ? op.c1 : pb.n);
+  if (c >= c1) return;
+  T2* A = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A);
+  const int64_t ld = op.rhs ? pb.ldb : pb.lda;
+  const int64_t k2 = op.k2 < pb.n ? op.k2 : pb.n;
+  for (int64_t i = op.k1; i < k2; ++i) {
+    const int64_t r = pb.ipiv[i];
+    if (r != i) {
+      const T2 a = A[i * ld + c];
+      A[i * ld + c] = A[r * ld + c];
+      A[r * ld + c] = a;
+    }
+  }
+}
+
+// ---- triangular solves -----------------------------------------------------
+// X := T^{-1} X for the diagonal block T at (r0, r0) of each problem's
+// factors, of order w' = min(w, n - r0) <= kGTri: unit lower (forward) or
+// upper (backward).  X: rows [r0, r0 + w'), columns [c0, c1) of the matrix
+// (c1 clamped to n) or of the right-hand sides.  One row per lane; each wave
+// carries kTriCols columns through the substitution together, four waves per
+// workgroup.
+struct GrpTri {
+  int64_t r0, c0, c1;
+  int w, rhs;
+};
+
+constexpr int kTriCols = 8;
+constexpr int kTriWgCols = 4 * kTriCols;
+
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ tab, GrpMap map, GrpTri op) {
+  typedef typename C2<T>::t T2;
+  __shared__ T2 Ts[kGTri][kGTri + 1];
+  __shared__ T2 dinv[kGTri];
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int wp = (int)(op.w < pb.n - op.r0 ? op.w : pb.n - op.r0);
+  const T2* L = reinterpret_cast<const T2*>(pb.A) + op.r0 * pb.lda + op.r0;
+  for (int e = threadIdx.x; e < wp * wp; e += blockDim.x) Ts[e / wp][e % wp] = L[(e / wp) * pb.lda + e % wp];
+  __syncthreads();
+  if (UPPER && threadIdx.x < wp) dinv[threadIdx.x] = crecip(Ts[threadIdx.x][threadIdx.x]);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t c1 = op.rhs ? op.c1 : (op.c1 < pb.n ? op.c1 : pb.n);
+  const int64_t cb = op.c0 + (int64_t)(blockIdx.x - map.first[p]) * kTriWgCols + (threadIdx.x >> 6) * kTriCols;
+  const int64_t ldx = op.rhs ? pb.ldb : pb.lda;
+  T2* X = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A) + op.r0 * ldx;
+  T2 x[kTriCols];
+#pragma unroll
+  for (int q = 0; q < kTriCols; ++q) {
+    x[q].x = 0;
+    x[q].y = 0;
+    if (cb + q < c1 && lane < wp) x[q] = X[lane * ldx + cb + q];
+  }
+  if (!UPPER) {
+    for (int i = 0; i < wp - 1; ++i) {
+      const T2 l = Ts[lane < wp ? lane : 0][i];
+      const bool act = lane > i && lane < wp;
+#pragma unroll
+      for (int q = 0; q < kTriCols; ++q) {
+        T2 xi;
+        xi.x = __shfl(x[q].x, i);
+        xi.y = __shfl(x[q].y, i);
+        if (act) {
+          x[q].x -= l.x * xi.x - l.y * xi.y;
+          x[q].y -= l.x * xi.y + l.y * xi.x;
+        }
+      }
+    }
+  } else {
+    for (int i = wp - 1; i >= 0; --i) {
+      const T2 u = Ts[lane < wp ? lane : 0][i];
+      const T2 di = dinv[i];
+      const bool act = lane < i;
+#pragma unroll
+      for (int q = 0; q < kTriCols; ++q) {
+        if (lane == i) x[q] = cmul(x[q], di);
+        T2 xi;
+        xi.x = __shfl(x[q].x, i);
+        xi.y = __shfl(x[q].y, i);
+        if (act) {
+          x[q].x -= u.x * xi.x - u.y * xi.y;
+          x[q].y -= u.x * xi.y + u.y * xi.x;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kTriCols; ++q)
+    if (cb + q < c1 && lane < wp) X[lane * ldx + cb + q] = x[q];
+}
+
+// ---- products --------------------------------------------------------------
+// C -= A B per problem: A = rows [m0, m1) x columns [k0, k1) of its matrix,
+// B = rows [k0, k1) x columns [n0, n1) of its matrix (or right-hand sides),
+// C = rows [m0, m1) x columns [n0, n1) of the same operand as B.  Ends are
+// clamped to the problem's order (columns of right-hand sides are not).
+struct GrpGemm {
+  int64_t m0, m1, n0, n1, k0, k1;
+  int rhs;
+};
+
+struct GrpDims {
+  int64_t M, N, K;
+  int64_t ldb;
+  const void* A;
+  const void* B;
+  void* C;
+};
+
+template <typename T2>
+__device__ inline GrpDims grp_dims(const GrpProb& pb, const GrpGemm& op) {
+  GrpDims d;
+  d.M = clamp_extent(op.m1, pb.n, op.m0, true);
+  d.N = clamp_extent(op.n1, pb.n, op.n0, !op.rhs);
+  d.K = clamp_extent(op.k1, pb.n, op.k0, true);
+  const T2* A = reinterpret_cast<const T2*>(pb.A);
+  T2* X = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A);
+  d.ldb = op.rhs ? pb.ldb : pb.lda;
+  d.A = A + op.m0 * pb.lda + op.k0;
+  d.B = X + op.k0 * d.ldb + op.n0;
+  d.C = X + op.m0 * d.ldb + op.n0;
+  return d;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool G3>
+__global__ __launch_bounds__(64 * WM * WN) void k_grp_gemm(const GrpProb* __restrict__ tab, GrpMap map, GrpGemm op) {
+  typedef typename Cplx<T>::t T2;
+  const int id = xcd_local(blockIdx.x, gridDim.x);
+  const int p = grp_find(map, id);
+  const GrpProb pb = tab[p];
+  const GrpDims d = grp_dims<T2>(pb, op);
+  const int nbm = (int)((d.M + BM - 1) / BM), nbn = (int)((d.N + BN - 1) / BN);
+  // rows of 8 tiles walked column by column, as mg_tile_order's second step
+  const int local = id - map.first[p];
+  const int in_group = 8 * nbn, g = local / in_group, first_m = g * 8;
+  const int gsize = min(nbm - first_m, 8);
+  const int tm = first_m + (local % in_group) % gsize, tn = (local % in_group) / gsize;
+  T2 alpha, beta;
+  alpha.x = -1;
+  alpha.y = 0;
+  beta.x = 1;
+  beta.y = 0;
+  MemA a{d.A, pb.lda};
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA, G3>(d.M, d.N, d.K, alpha, a, d.B, d.ldb, beta, d.C, d.ldb, tm, tn);
+}
+
+// The grouped trailing update of complex-float factors on split-bf16 MFMA
+// products (NS = 2: BF16x3), C -= A B.
+template <int NS, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_grp_gemm_split(const GrpProb* __restrict__ tab, GrpMap map,
+                                                                 GrpGemm op) {
+  typedef mg_sg::Tile<NS, BM, BN, WM, WN> Tile;
+  __shared__ typename Tile::S sm;
+  const int id = xcd_local(blockIdx.x, gridDim.x);
+  const int p = grp_find(map, id);
+  const GrpProb pb = tab[p];
+  const GrpDims d = grp_dims<float2>(pb, op);
+  const int nbm = (int)((d.M + BM - 1) / BM), nbn = (int)((d.N + BN - 1) / BN);
+  const int local = id - map.first[p];
+  const int in_group = 8 * nbn, g = local / in_group, first_m = g * 8;
+  const int gsize = min(nbm - first_m, 8);
+  const int tm = first_m + (local % in_group) % gsize, tn = (local % in_group) / gsize;
+  const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
+  const float2* A = reinterpret_cast<const float2*>(d.A);
+  const float2* B = reinterpret_cast<const float2*>(d.B);
+  float2* C = reinterpret_cast<float2*>(d.C);
+  Tile t;
+  // C - A B = -(A B - C): accumulators start at -C, the epilogue scales by -1
+  t.init(C + i0 * d.ldb + j0, d.ldb, d.M - i0, d.N - j0, make_float2(-1.f, 0.f), true);
+  t.mac(sm, A + i0 * pb.lda, pb.lda, B + j0, d.ldb, d.M - i0, d.N - j0, d.K);
+  t.store(C + i0 * d.ldb + j0, d.ldb, d.M - i0, d.N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+}
+
+__global__ void k_grp_zero_info(const GrpProb* __restrict__ tab, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < P) *tab[p].info = 0;
+}
+
+inline int64_t split_half(int64_t w) {
+  // left part: half, rounded up to a multiple of 16 (keeps product tiles whole)
+  int64_t h = ((w / 2) + 15) / 16 * 16;
+  if (h >= w) h = w / 2;
+  return h > 0 ? h : 1;
+}
+
+}  // namespace
+
+namespace mg {
+
+// A group of problems on one device: sizes sorted by decreasing order, the
+// device table, the tunables and an optional captured graph of the factor
+// and solve sequence.
+struct Group {
+  int prec = MG_C128;
+  int P = 0;
+  std::vector<int64_t> n;     // sorted, decreasing
+  std::vector<GrpProb> host;  // the table, same order
+  std::vector<int> order;     // order[i]: caller index of sorted entry i
+  GrpProb* dtab = nullptr;
+  int64_t nrhs = 0;
+  GrpCfg cfg;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t s = nullptr;    // the stream launches go to (the call's, or s2 for look-ahead panels)
+  hipStream_t s2 = nullptr;   // look-ahead panel stream (default priority)
+  hipEvent_t evA = nullptr, evB = nullptr;
+  int64_t launches = 0;
+
+  // problems with rows at or beyond row k (a prefix: orders decrease)
+  int active(int64_t k) const {
+    int a = 0;
+    while (a < P && n[a] > k) ++a;
+    return a;
+  }
+
+  // tile table of a launch: problem p gets cnt(p) workgroups
+  template <typename F>
+  bool make_map(int np, F cnt, GrpMap& m, int& total) const {
+    m.np = np;
+    int64_t acc = 0;
+    for (int p = 0; p < np; ++p) {
+      m.first[p] = (int)acc;
+      acc += cnt(p);
+    }
+    m.first[np] = (int)acc;
+    for (int p = np + 1; p <= kMaxGroup; ++p) m.first[p] = (int)acc;
+    total = (int)acc;
+    return acc > 0 && acc < INT_MAX;
+  }
+};
+
+namespace {
+
+template <typename T>
+int grp_getf2(Group& g, int64_t k, int w) {
+  const int np = g.active(k);
+  if (np == 0) return MG_OK;
+  hipLaunchKernelGGL(k_grp_getf2<T>, dim3(np), dim3(kGThreads), 0, g.s, g.dtab, k, w);
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+template <typename T>
+int grp_laswp(Group& g, int64_t k1, int64_t k2, int64_t c0, int64_t c1, bool rhs) {
+  typedef typename C2<T>::t T2;
+  const int np = g.active(k1);
+  if (np == 0 || c1 <= c0 || k2 <= k1) return MG_OK;
+  GrpMap m;
+  int total;
+  if (!g.make_map(np, [&](int p) -> int64_t {
+        const int64_t e = rhs ? c1 : std::min(c1, g.n[p]);
+        return e > c0 ? (e - c0 + 255) / 256 : 0;
+      }, m, total))
+    return MG_OK;
+  GrpSwp op{k1, k2, c0, c1, rhs ? 1 : 0};
+  hipLaunchKernelGGL(k_grp_laswp<T2>, dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+template <typename T>
+int grp_trsm_leaf(Group& g, bool upper, int64_t r0, int w, int64_t c0, int64_t c1, bool rhs) {
+  const int np = g.active(r0);
+  if (np == 0 || c1 <= c0) return MG_OK;
+  GrpMap m;
+  int total;
+  if (!g.make_map(np, [&](int p) -> int64_t {
+        const int64_t e = rhs ? c1 : std::min(c1, g.n[p]);
+        return e > c0 ? (e - c0 + kTriWgCols - 1) / kTriWgCols : 0;
+      }, m, total))
+    return MG_OK;
+  GrpTri op{r0, c0, c1, w, rhs ? 1 : 0};
+  if (upper) hipLaunchKernelGGL((k_grp_trsm<T, true>), dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
+  else hipLaunchKernelGGL((k_grp_trsm<T, false>), dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+// C -= A B over the group; split: the complex-float trailing update on BF16x3
+// products; tile: 0 = 128 x 64 (three-product FP64 / FP32), 1 = 64 x 64.
+template <typename T>
+int grp_gemm(Group& g, const GrpGemm& op, bool split, int tile) {
+  const int np = g.active(op.k0);   // the depth needs rows beyond k0
+  if (np == 0) return MG_OK;
+  const int BM = split ? (tile == 0 ? 128 : 64) : (tile == 0 ? 128 : 64);
+  const int BN = split ? 128 : 64;
+  auto ext = [&](int64_t e, int64_t s, int64_t n, bool clamp) {
+    const int64_t x = clamp ? std::min(e, n) : e;
+    return x > s ? x - s : 0;
+  };
+  GrpMap m;
+  int total;
+  if (!g.make_map(np, [&](int p) -> int64_t {
+        const int64_t M = ext(op.m1, op.m0, g.n[p], true), N = ext(op.n1, op.n0, g.n[p], !op.rhs),
+                      K = ext(op.k1, op.k0, g.n[p], true);
+        if (M == 0 || N == 0 || K == 0) return 0;
+        return ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+      }, m, total))
+    return MG_OK;
+  if (split) {
+    if (tile == 0) hipLaunchKernelGGL((k_grp_gemm_split<2, 128, 128, 4, 2>), dim3(total), dim3(512), 0, g.s, g.dtab, m, op);
+    else hipLaunchKernelGGL((k_grp_gemm_split<2, 64, 128, 2, 2>), dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
+  } else if (tile == 0) {
+    hipLaunchKernelGGL((k_grp_gemm<T, 128, 64, 16, 4, 2, true>), dim3(total), dim3(512), 0, g.s, g.dtab, m, op);
+  } else {
+    constexpr bool F64 = std::is_same<T, double>::value;
+    hipLaunchKernelGGL((k_grp_gemm<T, 64, 64, 16, 2, 2, !F64>), dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
+  }
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+// X := T^{-1} X for the diagonal block of order w at r0 (recursive above
+// kGTri: halves, with the product between them).
+template <typename T>
+int grp_trsm(Group& g, bool upper, int64_t r0, int64_t w, int64_t c0, int64_t c1, bool rhs) {
+  if (w <= kGTri) return grp_trsm_leaf<T>(g, upper, r0, (int)w, c0, c1, rhs);
+  const int64_t w1 = split_half(w);
+  if (!upper) {
+    MG_TRY(grp_trsm<T>(g, false, r0, w1, c0, c1, rhs));
+    MG_TRY(grp_gemm<T>(g, GrpGemm{r0 + w1, r0 + w, c0, c1, r0, r0 + w1, rhs ? 1 : 0}, false, 1));
+    return grp_trsm<T>(g, false, r0 + w1, w - w1, c0, c1, rhs);
+  }
+  MG_TRY(grp_trsm<T>(g, true, r0 + w1, w - w1, c0, c1, rhs));
+  MG_TRY(grp_gemm<T>(g, GrpGemm{r0, r0 + w1, c0, c1, r0 + w1, r0 + w, rhs ? 1 : 0}, false, 1));
+  return grp_trsm<T>(g, true, r0, w1, c0, c1, rhs);
+}
+
+// Panel: columns [k, k + w) of every problem, rows [k, n).
+template <typename T>
+int grp_panel(Group& g, int64_t k, int64_t w) {
+  if (w <= g.cfg.leaf) return grp_getf2<T>(g, k, (int)w);
+  const int64_t w1 = split_half(w);
+  MG_TRY(grp_panel<T>(g, k, w1));
+  MG_TRY(grp_laswp<T>(g, k, k + w1, k + w1, k + w, false));
+  MG_TRY(grp_trsm<T>(g, false, k, w1, k + w1, k + w, false));
+  MG_TRY(grp_gemm<T>(g, GrpGemm{k + w1, INT64_MAX, k + w1, k + w, k, k + w1, 0}, false, w - w1 >= 64 ? 0 : 1));
+  MG_TRY(grp_panel<T>(g, k + w1, w - w1));
+  return grp_laswp<T>(g, k + w1, k + w, k, k + w1, false);
+}
+
+// Look-ahead of depth one: once step k has updated the next panel's
+// columns, that panel is factored on the group's second stream while the
+// rest of step k's trailing update runs on the call's stream (the two touch
+// disjoint columns); step k+1 waits for the panel before its row swaps.
+template <typename T>
+int grp_getrf(Group& g, bool split) {
+  const int64_t nmax = g.P ? g.n[0] : 0;
+  hipStream_t s = g.s;
+  hipLaunchKernelGGL(k_grp_zero_info, dim3((g.P + 255) / 256), dim3(256), 0, s, g.dtab, g.P);
+  MG_HIP(hipGetLastError());
+  const int64_t nb = g.cfg.nb;
+  MG_TRY(grp_panel<T>(g, 0, nb));
+  for (int64_t k = 0; k < nmax; k += nb) {
+    const int64_t kn = k + nb;
+    if (kn < nmax) {
+      MG_TRY(grp_laswp<T>(g, k, kn, kn, INT64_MAX, false));
+      MG_TRY(grp_trsm<T>(g, false, k, nb, kn, INT64_MAX, false));
+      MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn, kn + nb, k, kn, 0}, split, g.cfg.tile));
+      MG_HIP(hipEventRecord(g.evA, s));
+      MG_HIP(hipStreamWaitEvent(g.s2, g.evA, 0));
+      g.s = g.s2;
+      const int st = grp_panel<T>(g, kn, nb);
+      g.s = s;
+      MG_TRY(st);
+      MG_HIP(hipEventRecord(g.evB, g.s2));
+      MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn + nb, INT64_MAX, k, kn, 0}, split, g.cfg.tile));
+      MG_HIP(hipStreamWaitEvent(s, g.evB, 0));
+    }
+    if (k > 0) MG_TRY(grp_laswp<T>(g, k, kn, 0, k, false));
+  }
+  return MG_OK;
+}
+
+template <typename T>
+int grp_getrs(Group& g) {
+  const int64_t nmax = g.P ? g.n[0] : 0, nr = g.nrhs;
+  if (nr == 0 || nmax == 0) return MG_OK;
+  MG_TRY(grp_laswp<T>(g, 0, INT64_MAX, 0, nr, true));
+  const int64_t sb = kGTri;
+  for (int64_t r0 = 0; r0 < nmax; r0 += sb) {
+    MG_TRY(grp_trsm_leaf<T>(g, false, r0, (int)sb, 0, nr, true));
+    MG_TRY(grp_gemm<T>(g, GrpGemm{r0 + sb, INT64_MAX, 0, nr, r0, r0 + sb, 1}, false, 1));
+  }
+  for (int64_t r0 = ((nmax - 1) / sb) * sb; r0 >= 0; r0 -= sb) {
+    MG_TRY(grp_trsm_leaf<T>(g, true, r0, (int)sb, 0, nr, true));
+    if (r0 > 0) MG_TRY(grp_gemm<T>(g, GrpGemm{0, r0, 0, nr, r0, r0 + sb, 1}, false, 1));
+  }
+  return MG_OK;
+}
+
+int run_factor(Group& g) {
+  switch (g.prec) {
+    case MG_C128: return grp_getrf<double>(g, false);
+    case MG_C64: return grp_getrf<float>(g, false);
+    case MG_BF16X3: return grp_getrf<float>(g, true);
+    default: return MG_ERR_ARG;
+  }
+}
+
+int run_solve(Group& g) {
+  return g.prec == MG_C128 ? grp_getrs<double>(g) : grp_getrs<float>(g);
+}
+
+}  // namespace
+}  // namespace mg
+
+struct mg_group {
+  mg::Group g;
+};
+
+extern "C" {
+
+int mg_group_destroy(mg_group* h);
+
+int mg_group_create(int prec, int count, void* const* A, const int64_t* n, const int64_t* lda, int* const* ipiv,
+                    int* const* info, void* const* B, const int64_t* ldb, int64_t nrhs, mg_group** out) {
+  if (!out || count < 0 || count > mg::kMaxGroup || nrhs < 0) return MG_ERR_ARG;
+  if (prec != MG_C128 && prec != MG_C64 && prec != MG_BF16X3) return MG_ERR_ARG;
+  if (count > 0 && (!A || !n || !lda || !ipiv || !info)) return MG_ERR_ARG;
+  if (nrhs > 0 && count > 0 && (!B || !ldb)) return MG_ERR_ARG;
+  for (int i = 0; i < count; ++i) {
+    if (!A[i] || !ipiv[i] || !info[i] || n[i] < 1 || lda[i] < n[i] || n[i] > INT_MAX / 2) return MG_ERR_ARG;
+    if (nrhs > 0 && (!B[i] || ldb[i] < nrhs)) return MG_ERR_ARG;
+  }
+  mg_group* h = new (std::nothrow) mg_group();
+  if (!h) return MG_ERR_NOMEM;
+  mg::Group& g = h->g;
+  g.prec = prec;
+  g.P = count;
+  g.nrhs = nrhs;
+  g.order.resize(count);
+  std::iota(g.order.begin(), g.order.end(), 0);
+  std::stable_sort(g.order.begin(), g.order.end(), [&](int a, int b) { return n[a] > n[b]; });
+  for (int i = 0; i < count; ++i) {
+    const int c = g.order[i];
+    mg::GrpProb e{A[c], lda[c], n[c], ipiv[c], info[c], nrhs > 0 ? B[c] : nullptr, nrhs > 0 ? ldb[c] : 0, 0};
+    g.host.push_back(e);
+    g.n.push_back(n[c]);
+  }
+  if (hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&g.evA, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g.evB, hipEventDisableTiming) != hipSuccess) {
+    mg_group_destroy(h);
+    mg::set_error("mg_group_create: stream or event creation failed");
+    return MG_ERR_HIP;
+  }
+  if (count > 0) {
+    if (hipMalloc(&g.dtab, sizeof(mg::GrpProb) * count) != hipSuccess) {
+      g.dtab = nullptr;
+      mg_group_destroy(h);
+      mg::set_error("mg_group_create: table allocation failed");
+      return MG_ERR_NOMEM;
+    }
+    if (hipMemcpy(g.dtab, g.host.data(), sizeof(mg::GrpProb) * count, hipMemcpyHostToDevice) != hipSuccess) {
+      mg_group_destroy(h);
+      mg::set_error("mg_group_create: table upload failed");
+      return MG_ERR_HIP;
+    }
+  }
+  *out = h;
+  return MG_OK;
+}
+
+int mg_group_destroy(mg_group* h) {
+  if (!h) return MG_OK;
+  if (h->g.s2) (void)hipStreamSynchronize(h->g.s2);
+  if (h->g.exec) (void)hipGraphExecDestroy(h->g.exec);
+  if (h->g.graph) (void)hipGraphDestroy(h->g.graph);
+  if (h->g.dtab) (void)hipFree(h->g.dtab);
+  if (h->g.evA) (void)hipEventDestroy(h->g.evA);
+  if (h->g.evB) (void)hipEventDestroy(h->g.evB);
+  if (h->g.s2) (void)hipStreamDestroy(h->g.s2);
+  delete h;
+  return MG_OK;
+}
+
+int mg_group_configure(mg_group* h, int nb, int leaf, int tile) {
+  if (!h) return MG_ERR_ARG;
+  if (nb < 16 || nb > 1024 || nb % 16 != 0 || leaf < 1 || leaf > kGLeaf || (tile != 0 && tile != 1))
+    return MG_ERR_ARG;
+  h->g.cfg.nb = nb;
+  h->g.cfg.leaf = leaf;
+  h->g.cfg.tile = tile;
+  return MG_OK;
+}
+
+int mg_group_getrf(mg_group* h, void* stream) {
+  if (!h) return MG_ERR_ARG;
+  h->g.s = reinterpret_cast<hipStream_t>(stream);
+  return mg::run_factor(h->g);
+}
+
+int mg_group_getrs(mg_group* h, void* stream) {
+  if (!h) return MG_ERR_ARG;
+  h->g.s = reinterpret_cast<hipStream_t>(stream);
+  return mg::run_solve(h->g);
+}
+
+int64_t mg_group_launches(const mg_group* h) { return h ? h->g.launches : -1; }
+
+}  // extern "C"

@@ -445,6 +445,90 @@ def propagate_src(kind, meas, scat, k0, alpha, Ek, out, beam, dirs, pol, w, flag
     return out
 
 
+class GroupLU:
+    """Grouped LU with partial pivoting of up to 256 independent square
+    matrices of uneven order, factored in lockstep (one launch per step of
+    the blocked algorithm for every problem with work at that column), and
+    the grouped solve of their right-hand sides.
+
+    mats: device tensors (n_i, n_i), complex128 (prec 'c128') or complex64
+    ('c64', 'bf16x3': trailing updates on BF16x3 products), row-major,
+    factored in place; rhs: optional (n_i, nrhs) tensors of the same dtype
+    (one nrhs for all), solved in place by solve().  The factors, pivots and
+    info per problem follow getrf."""
+
+    MAX = 256
+
+    def __init__(self, mats, rhs=None, prec="c128", nb=128, leaf=16, tile=0):
+        lib = L.load()
+        if not 0 < len(mats) <= self.MAX:
+            raise ValueError("a group holds 1 to %d problems" % self.MAX)
+        dt = torch.complex128 if prec == "c128" else torch.complex64
+        dev = mats[0].device
+        for A in mats:
+            if A.dtype != dt or A.device != dev or A.dim() != 2 or A.shape[0] != A.shape[1] or A.stride(1) != 1:
+                raise ValueError("group matrices: square, row-major, %s on one device" % dt)
+        if rhs is not None:
+            if len(rhs) != len(mats):
+                raise ValueError("one right-hand side block per matrix")
+            nrhs = rhs[0].shape[1]
+            for A, B in zip(mats, rhs):
+                if B.dtype != dt or B.device != dev or B.shape != (A.shape[0], nrhs) or B.stride(1) != 1:
+                    raise ValueError("right-hand sides: (n_i, nrhs) row-major, the matrices' dtype")
+        self.mats, self.rhs, self.prec, self.device = list(mats), rhs, prec, dev
+        P = len(mats)
+        ns = [A.shape[0] for A in mats]
+        self.off = [0]
+        for n in ns:
+            self.off.append(self.off[-1] + n)
+        self.ipiv_all = torch.empty(self.off[-1], dtype=torch.int32, device=dev)
+        self.info = torch.zeros(P, dtype=torch.int32, device=dev)
+        arr = lambda ct, xs: (ct * P)(*xs)
+        vp = ctypes.c_void_p
+        self._keep = (
+            arr(vp, [_p(A) for A in mats]), arr(ctypes.c_int64, ns), arr(ctypes.c_int64, [_ld(A) for A in mats]),
+            arr(vp, [self.ipiv_all.data_ptr() + 4 * o for o in self.off[:-1]]),
+            arr(vp, [self.info.data_ptr() + 4 * p for p in range(P)]),
+            arr(vp, [_p(B) for B in rhs]) if rhs is not None else None,
+            arr(ctypes.c_int64, [_ld(B) for B in rhs]) if rhs is not None else None)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            L.check(lib.mg_group_create(PRECS[prec], P, *self._keep, nrhs if rhs is not None else 0,
+                                        ctypes.byref(h)), "mg_group_create")
+        self.h = h
+        L.check(lib.mg_group_configure(self.h, int(nb), int(leaf), int(tile)), "mg_group_configure")
+
+    def ipiv(self, i):
+        return self.ipiv_all[self.off[i]:self.off[i + 1]]
+
+    def factor(self):
+        with torch.cuda.device(self.device):
+            L.check(L.load().mg_group_getrf(self.h, _stream(self.device)), "mg_group_getrf")
+        return self
+
+    def solve(self):
+        if self.rhs is None:
+            raise ValueError("this group has no right-hand sides")
+        with torch.cuda.device(self.device):
+            L.check(L.load().mg_group_getrs(self.h, _stream(self.device)), "mg_group_getrs")
+        return self.rhs
+
+    @property
+    def launches(self):
+        return int(L.load().mg_group_launches(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().mg_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def coldot(Y, X):
     """out[c] = sum_r Y[r, c] X[r, c] (no conjugation), complex128."""
     lib = L.load()

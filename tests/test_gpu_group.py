@@ -1,0 +1,131 @@
+"""Grouped LU of uneven-order problems in lockstep (mg_group_*) against
+LAPACK's zgetrf / cgetrf (scipy's lu_factor: the same pivoting rule, the
+largest |re| + |im|, first on ties), against the per-problem pivoted LU of
+the library, and against solves in complex128."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from magreete_amd import engine as E
+
+DEV = "cuda:0"
+SIZES = [700, 33, 1, 256, 1029, 130, 64, 65, 17, 128, 129, 300]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _mats(sizes, seed, dtype=np.complex128, shrink=0.05):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in sizes:
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        A[np.arange(n), np.arange(n)] *= shrink     # small diagonal: every column pivots
+        out.append(A.astype(dtype))
+    return out
+
+
+def _unpack(LU):
+    n = LU.shape[0]
+    return np.tril(LU, -1) + np.eye(n), np.triu(LU)
+
+
+@pytest.mark.parametrize("cfg", [(128, 16, 0), (64, 8, 1), (256, 16, 0), (32, 4, 1)])
+def test_group_factors_match_lapack(cfg):
+    """Pivots equal LAPACK's for every problem; P A = L U to rounding; the
+    factors agree with zgetrf's."""
+    nb, leaf, tile = cfg
+    say("group getrf vs lapack", cfg)
+    host = _mats(SIZES, 1)
+    mats = [torch.from_numpy(A).to(DEV) for A in host]
+    g = E.GroupLU(mats, prec="c128", nb=nb, leaf=leaf, tile=tile).factor()
+    torch.cuda.synchronize()
+    assert (g.info.cpu() == 0).all()
+    for i, A in enumerate(host):
+        lu, piv = sl.lu_factor(A)
+        mine = mats[i].cpu().numpy()
+        ip = g.ipiv(i).cpu().numpy()
+        assert np.array_equal(ip, piv), (i, A.shape)
+        err = np.abs(mine - lu).max() / np.abs(lu).max()
+        assert err < 1e-12, (i, A.shape, err)
+        Lm, Um = _unpack(mine)
+        PA = A.copy()
+        for r, p in enumerate(ip):
+            PA[[r, p]] = PA[[p, r]]
+        assert np.abs(Lm @ Um - PA).max() <= 1e-12 * np.abs(A).max() * A.shape[0]
+    say("launches", g.launches)
+    g.close()
+
+
+def test_group_solve_matches_sequential_and_numpy():
+    """Grouped solve of 32 right-hand sides equals the library's per-problem
+    pivoted LU solve and numpy's solve."""
+    say("group getrs")
+    host = _mats(SIZES, 2, shrink=1.0)
+    rng = np.random.default_rng(3)
+    rhs_host = [rng.normal(size=(A.shape[0], 32)) + 1j * rng.normal(size=(A.shape[0], 32)) for A in host]
+    mats = [torch.from_numpy(A).to(DEV) for A in host]
+    rhs = [torch.from_numpy(B).to(DEV) for B in rhs_host]
+    g = E.GroupLU(mats, rhs, prec="c128")
+    g.factor()
+    g.solve()
+    torch.cuda.synchronize()
+    for i, (A, B) in enumerate(zip(host, rhs_host)):
+        X = rhs[i].cpu().numpy()
+        ref = np.linalg.solve(A, B)
+        err = np.linalg.norm(X - ref) / np.linalg.norm(ref)
+        assert err < 1e-11, (i, err)
+        F = E.Factor(torch.from_numpy(A).to(DEV))
+        Y = torch.from_numpy(B).to(DEV)
+        F.solve(Y)
+        seq = Y.cpu().numpy()
+        # two backward-stable solves differ by ~eps cond(A): up to ~1e3 here
+        assert np.linalg.norm(X - seq) / np.linalg.norm(seq) < 1e-11, i
+    g.close()
+
+
+def test_group_zero_pivot_info():
+    """A zero column makes info the first zero pivot (1-based), as zgetrf,
+    and leaves the other problems' factors untouched by it."""
+    say("group info")
+    host = _mats([90, 200, 40], 4, shrink=1.0)
+    host[1][:, 77] = 0
+    mats = [torch.from_numpy(A).to(DEV) for A in host]
+    g = E.GroupLU(mats, prec="c128").factor()
+    torch.cuda.synchronize()
+    info = g.info.cpu().numpy()
+    assert info[0] == 0 and info[2] == 0
+    assert info[1] == 78
+    lu, piv = sl.lu_factor(host[0])
+    assert np.array_equal(g.ipiv(0).cpu().numpy(), piv)
+    g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("c64", 2e-4), ("bf16x3", 2e-3)])
+def test_group_complex_float(prec, tol):
+    """complex64 factors (the trailing updates on FP32 or BF16x3 products):
+    solutions within the precision of the factor of the complex128 ones."""
+    say("group", prec)
+    host = _mats(SIZES, 5, dtype=np.complex64, shrink=1.0)
+    rng = np.random.default_rng(6)
+    rhs_host = [(rng.normal(size=(A.shape[0], 8)) + 1j * rng.normal(size=(A.shape[0], 8))).astype(np.complex64)
+                for A in host]
+    for i in range(len(host)):
+        host[i] = (host[i] + 4 * np.sqrt(host[i].shape[0]) * np.eye(host[i].shape[0])).astype(np.complex64)
+    mats = [torch.from_numpy(A).to(DEV) for A in host]
+    rhs = [torch.from_numpy(B).to(DEV) for B in rhs_host]
+    g = E.GroupLU(mats, rhs, prec=prec)
+    g.factor()
+    g.solve()
+    torch.cuda.synchronize()
+    assert (g.info.cpu() == 0).all()
+    for i, (A, B) in enumerate(zip(host, rhs_host)):
+        ref = np.linalg.solve(A.astype(np.complex128), B.astype(np.complex128))
+        X = rhs[i].cpu().numpy()
+        err = np.linalg.norm(X - ref) / np.linalg.norm(ref)
+        assert err < tol, (i, err)
+    g.close()
