@@ -385,18 +385,25 @@ def run_ldos(args, dev):
 
 def run_ensemble(args, dev):
     """Batched ensemble: args.ensemble 3D vector configurations (N log-uniform
-    in [500, 3000], 16 directions x 2 polarisations), solved one at a time by
-    the FP64 path and concurrently (4 streams) by the FP64 and the BF16x3
-    mixed paths; errors of the batched results against the loop."""
+    in [500, 3000], 16 directions x 2 polarisations), each assembled and
+    solved: one at a time by the FP64 path (the loop), concurrently on 4
+    streams by the FP64 and BF16x3 paths, and by the grouped lockstep LU
+    (FP64 pivoted; BF16x3 factor + FP64 iterative refinement to
+    args.batch_tol).  Errors of every batched result against the loop.  The
+    grouped runs are preceded by one untimed run of the same ensemble, which
+    autotunes the group's tunables (cached per size class)."""
     from magreete_amd import workloads as W
     from magreete_amd.ensemble import Ensemble
     ens = W.cfg_ensemble(n_configs=args.ensemble, seed=args.seed)
     out = {"workload": "%d x 3d_vector, N log-uniform [500, 3000], T=32" % len(ens), "unit": "configs/s"}
     ref = None
-    for name, method, streams in (("fp64_loop", "fp64", 1), ("fp64_batched", "fp64", 4),
-                                  ("bf16x3_batched", "bf16x3", 4)):
-        e = Ensemble(dev, streams=streams, method=method, tol=args.batch_tol)
-        e.run(ens[:4])
+    runs = (("fp64_loop", "fp64", 1, False), ("fp64_streams4", "fp64", 4, False),
+            ("bf16x3_streams4", "bf16x3", 4, False), ("fp64_grouped", "fp64", 1, True),
+            ("bf16x3_grouped", "bf16x3", 1, True))
+    for name, method, streams, grouped in runs:
+        e = Ensemble(dev, streams=streams, method=method, tol=args.batch_tol, grouped=grouped)
+        res = e.run(ens if grouped else ens[:4])
+        del res
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = e.run(ens)
@@ -407,9 +414,15 @@ def run_ensemble(args, dev):
             ref = [r["X"] for r in res]
         else:
             out[name + "_max_err_vs_loop"] = max(colerr(r["X"], x) for r, x in zip(res, ref))
+        if grouped:
+            out[name + "_groups"] = len(set(id(r["X"].untyped_storage()) for r in res))
+            if method == "bf16x3":
+                out[name + "_fallbacks"] = sum(bool(r["stats"]["fallback"]) for r in res)
+                out[name + "_max_refine_iters"] = max(r["stats"]["iters"] for r in res)
         print("# ensemble %s: %.1f configs/s" % (name, out[name]), file=sys.stderr, flush=True)
         del res
         torch.cuda.empty_cache()
+    out["grouped_vs_loop"] = {m: out[m + "_grouped"] / out["fp64_loop"] for m in ("fp64", "bf16x3")}
     return out
 
 

@@ -263,6 +263,55 @@ int64_t mg_relaxed_product_work_bytes(int prec, int64_t m, int64_t n, int64_t k)
 int mg_relaxed_product(int prec, int64_t m, int64_t n, int64_t k, const void* AS, const void* B, int64_t ldb,
                        void* C, int64_t ldc, void* work, int64_t work_bytes, void* stream);
 
+/* Grouped LU: up to 256 independent matrices of uneven order factored in
+ * lockstep, every launch of the blocked algorithm serving every problem with
+ * work at that column, and their right-hand sides solved the same way.  It
+ * replaces magreete's per-configuration loop (magreete.py:346-452), whose
+ * every iteration factors one matrix in torch.linalg.solve
+ * (Transmission3D.py:205, Transmission2D.py:192).  Per problem the factors,
+ * pivots (0-based) and info (first zero pivot, 1-based) are getrf's, with
+ * the same pivoting rule.
+ * mg_group_create: prec MG_C128, MG_C64 or MG_BF16X3 (complex float, trailing
+ * updates on BF16x3 products); host arrays of `count` device pointers and
+ * sizes (A[i]: n[i] x n[i], lda[i]; ipiv[i]: n[i] ints; info[i]: one int;
+ * B[i]: n[i] x nrhs, ldb[i], or B = NULL and nrhs = 0).  The descriptors are
+ * copied to a device table owned by the group; the buffers stay the
+ * caller's and must outlive it.
+ * mg_group_configure: panel width nb (multiple of 16, <= 1024), leaf width
+ * (<= 16), trailing-update tile (0: 128 x 64, 1: 64 x 64), leaf workgroup
+ * threads (256, 512, 1024).  mg_group_tune: the same tunables chosen by
+ * timing candidates on copies of a sample of the group (call before the
+ * factorisation), cached per device, precision and size class; cfg (4 ints)
+ * receives them, best_ms the sample's time (0 for a cached choice).
+ * mg_group_getrf / mg_group_getrs: factor / solve on `stream`.
+ * mg_group_capture(op 0: factor, 1: solve) records that launch sequence in
+ * a HIP graph (nothing runs; no memset nodes) and mg_group_replay launches
+ * it on `stream`: the buffers are the group's, their contents may change
+ * between replays.
+ * Calls on one group are not reentrant. */
+typedef struct mg_group mg_group;
+int mg_group_create(int prec, int count, void* const* A, const int64_t* n, const int64_t* lda, int* const* ipiv,
+                    int* const* info, void* const* B, const int64_t* ldb, int64_t nrhs, mg_group** out);
+int mg_group_destroy(mg_group* g);
+int mg_group_configure(mg_group* g, int nb, int leaf, int tile, int leaf_threads);
+int mg_group_tune(mg_group* g, int* cfg, float* best_ms, void* stream);
+int mg_group_getrf(mg_group* g, void* stream);
+int mg_group_getrs(mg_group* g, void* stream);
+int mg_group_capture(mg_group* g, int op);
+int mg_group_replay(mg_group* g, int op, void* stream);
+/* Iterative refinement over a group: mg_group_set_aux registers an n[i] x
+ * nrhs block X[i] (ldx[i]) per problem (caller order); mg_group_residual
+ * then forms B[i] -= A[i] X[i] for every problem in one launch (products in
+ * the group's element type, so a complex128 group of the original matrices
+ * gives the FP64 residual b - M x when B holds b); mg_group_colnorm writes
+ * ||B[i][:, c]|| to norms[i nrhs + c] (fixed summation order).  No reference
+ * counterpart: torch.linalg.solve (Transmission3D.py:205) refines nothing. */
+int mg_group_set_aux(mg_group* g, void* const* X, const int64_t* ldx);
+int mg_group_residual(mg_group* g, void* stream);
+int mg_group_colnorm(mg_group* g, double* norms, void* stream);
+/* Launches issued by the group's calls so far (measurement hook). */
+int64_t mg_group_launches(const mg_group* g);
+
 #ifdef __cplusplus
 }
 #endif

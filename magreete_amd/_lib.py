@@ -75,10 +75,16 @@ SIGNATURES = {
                          _dbl, _vp, _i64, _vp, _i64, _vp],
     "mg_group_create": [_int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _c.POINTER(_vp)],
     "mg_group_destroy": [_vp],
-    "mg_group_configure": [_vp, _int, _int, _int],
+    "mg_group_configure": [_vp, _int, _int, _int, _int],
     "mg_group_getrf": [_vp, _vp],
     "mg_group_getrs": [_vp, _vp],
     "mg_group_launches": [_vp],
+    "mg_group_tune": [_vp, _vp, _vp, _vp],
+    "mg_group_capture": [_vp, _int],
+    "mg_group_replay": [_vp, _int, _vp],
+    "mg_group_set_aux": [_vp, _vp, _vp],
+    "mg_group_residual": [_vp, _vp],
+    "mg_group_colnorm": [_vp, _vp, _vp],
     "mg_propagate_noalloc_max_bytes": [],
     "mg_propagate_noalloc": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
                              _vp, _i64, _vp],

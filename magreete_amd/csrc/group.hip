@@ -46,14 +46,16 @@ namespace mg {
 constexpr int kMaxGroup = 256;   // problems of one group
 
 struct GrpProb {
-  void* A;
+  void* A;        // the matrix (factored in place)
   int64_t lda;
   int64_t n;
   int* ipiv;
   int* info;
-  void* B;
+  void* B;        // right-hand sides (n x nrhs)
   int64_t ldb;
-  int64_t pad;
+  void* X;        // auxiliary n x nrhs block: the solution of the residual product B -= A X
+  int64_t ldx;
+  int64_t idx;    // the caller's index of this problem
 };
 
 // Workgroups of one grouped launch: problem p < np owns [first[p], first[p+1]).
@@ -67,6 +69,7 @@ struct GrpCfg {
   int nb = 128;      // panel width (trailing-update depth)
   int leaf = 16;     // leaf width, <= kGLeaf
   int tile = 0;      // trailing-update tile: 0 = 128 x 64 (three-product), 1 = 64 x 64
+  int lthreads = 1024; // threads of a leaf workgroup (the leaf streams its rows: more lanes, more loads in flight)
 };
 
 }  // namespace mg
@@ -76,9 +79,7 @@ namespace {
 using mg::GrpMap;
 using mg::GrpProb;
 
-constexpr int kGLeaf = 16;                    // widest leaf
-constexpr int kGThreads = 1024;               // leaf workgroup: 16 lanes per row, 64 rows per sweep
-constexpr int kGRows = kGThreads / kGLeaf;
+constexpr int kGLeaf = 16;                    // widest leaf (16 lanes per row)
 constexpr int kGUnroll = 4;                   // rows in flight per lane
 constexpr int kGTri = 64;                     // widest triangular block of the grouped solves
 
@@ -136,18 +137,20 @@ __device__ inline int64_t clamp_extent(int64_t end, int64_t n, int64_t start, bo
 
 // ---- leaf: one workgroup per problem ---------------------------------------
 // Leaf LU of rows [k, n) x columns [k, k + w') of problem blockIdx.x, w' =
-// min(w, n - k).  Lane c of each 16-lane group holds column c of one row.
+// min(w, n - k), by one workgroup of NT threads.  Lane c of each 16-lane
+// group holds column c of one row.
 // Pass j (0 <= j <= w'): rows r >= j get column j-1 scaled by the inverse of
 // pivot j-1 and columns >= j updated by it, and column j's pivot search runs
 // on the updated values; then rows j and the pivot row swap within the leaf.
-template <typename T>
-__global__ __launch_bounds__(kGThreads) void k_grp_getf2(const GrpProb* __restrict__ tab, int64_t k, int w) {
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_grp_getf2(const GrpProb* __restrict__ tab, int64_t k, int w) {
   typedef typename C2<T>::t T2;
+  constexpr int kGRows = NT / kGLeaf;   // rows per sweep
   __shared__ T2 urow[kGLeaf];
   __shared__ T2 sinv;
   __shared__ int snz;
-  __shared__ double sv[kGThreads / 64];
-  __shared__ int sr[kGThreads / 64];
+  __shared__ double sv[NT / 64];
+  __shared__ int sr[NT / 64];
   const GrpProb pb = tab[blockIdx.x];
   if (pb.n <= k) return;
   const int m = (int)(pb.n - k);
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(kGThreads) void k_grp_getf2(const GrpProb* __restri
     best = sv[0];
     brow = sr[0];
 #pragma unroll
-    for (int q = 1; q < kGThreads / 64; ++q)
+    for (int q = 1; q < NT / 64; ++q)
       if (better(sv[q], sr[q], best, brow)) {
         best = sv[q];
         brow = sr[q];
@@ -371,6 +374,8 @@ __global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ ta
 // B = rows [k0, k1) x columns [n0, n1) of its matrix (or right-hand sides),
 // C = rows [m0, m1) x columns [n0, n1) of the same operand as B.  Ends are
 // clamped to the problem's order (columns of right-hand sides are not).
+// rhs: 0 = B and C from the matrix, 1 = from the right-hand sides,
+// 2 = B from the auxiliary block X and C from the right-hand sides.
 struct GrpGemm {
   int64_t m0, m1, n0, n1, k0, k1;
   int rhs;
@@ -378,7 +383,7 @@ struct GrpGemm {
 
 struct GrpDims {
   int64_t M, N, K;
-  int64_t ldb;
+  int64_t ldb, ldc;
   const void* A;
   const void* B;
   void* C;
@@ -391,11 +396,13 @@ __device__ inline GrpDims grp_dims(const GrpProb& pb, const GrpGemm& op) {
   d.N = clamp_extent(op.n1, pb.n, op.n0, !op.rhs);
   d.K = clamp_extent(op.k1, pb.n, op.k0, true);
   const T2* A = reinterpret_cast<const T2*>(pb.A);
-  T2* X = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A);
-  d.ldb = op.rhs ? pb.ldb : pb.lda;
+  T2* Cb = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A);
+  const T2* Bb = op.rhs == 2 ? reinterpret_cast<const T2*>(pb.X) : Cb;
+  d.ldc = op.rhs ? pb.ldb : pb.lda;
+  d.ldb = op.rhs == 2 ? pb.ldx : d.ldc;
   d.A = A + op.m0 * pb.lda + op.k0;
-  d.B = X + op.k0 * d.ldb + op.n0;
-  d.C = X + op.m0 * d.ldb + op.n0;
+  d.B = Bb + op.k0 * d.ldb + op.n0;
+  d.C = Cb + op.m0 * d.ldc + op.n0;
   return d;
 }
 
@@ -418,7 +425,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_grp_gemm(const GrpProb* __rest
   beta.x = 1;
   beta.y = 0;
   MemA a{d.A, pb.lda};
-  cgemm_body<T, BM, BN, BK, WM, WN, MemA, G3>(d.M, d.N, d.K, alpha, a, d.B, d.ldb, beta, d.C, d.ldb, tm, tn);
+  cgemm_body<T, BM, BN, BK, WM, WN, MemA, G3>(d.M, d.N, d.K, alpha, a, d.B, d.ldb, beta, d.C, d.ldc, tm, tn);
 }
 
 // The grouped trailing update of complex-float factors on split-bf16 MFMA
@@ -443,9 +450,31 @@ __global__ __launch_bounds__(64 * WM * WN) void k_grp_gemm_split(const GrpProb* 
   float2* C = reinterpret_cast<float2*>(d.C);
   Tile t;
   // C - A B = -(A B - C): accumulators start at -C, the epilogue scales by -1
-  t.init(C + i0 * d.ldb + j0, d.ldb, d.M - i0, d.N - j0, make_float2(-1.f, 0.f), true);
+  t.init(C + i0 * d.ldc + j0, d.ldc, d.M - i0, d.N - j0, make_float2(-1.f, 0.f), true);
   t.mac(sm, A + i0 * pb.lda, pb.lda, B + j0, d.ldb, d.M - i0, d.N - j0, d.K);
-  t.store(C + i0 * d.ldb + j0, d.ldb, d.M - i0, d.N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+  t.store(C + i0 * d.ldc + j0, d.ldc, d.M - i0, d.N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+}
+
+// out[idx nrhs + c] = ||B[:, c]|| of every problem (caller's index idx),
+// columns 64 at a time, the rows in 4 interleaved partial sums combined in a
+// fixed order.
+template <typename T2>
+__global__ __launch_bounds__(256) void k_grp_colnorm(const GrpProb* __restrict__ tab, int nrhs, double* __restrict__ out) {
+  __shared__ double part[4][64];
+  const GrpProb pb = tab[blockIdx.y];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+  const T2* B = reinterpret_cast<const T2*>(pb.B);
+  double acc = 0;
+  if (c < nrhs)
+    for (int64_t r = ph; r < pb.n; r += 4) {
+      const T2 v = B[r * pb.ldb + c];
+      acc += (double)v.x * v.x + (double)v.y * v.y;
+    }
+  part[ph][threadIdx.x & 63] = acc;
+  __syncthreads();
+  if (ph == 0 && c < nrhs)
+    out[pb.idx * nrhs + c] = sqrt(((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                                  part[3][threadIdx.x]);
 }
 
 __global__ void k_grp_zero_info(const GrpProb* __restrict__ tab, int P) {
@@ -476,10 +505,11 @@ struct Group {
   GrpProb* dtab = nullptr;
   int64_t nrhs = 0;
   GrpCfg cfg;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};        // captured factor (0) and solve (1) sequences
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   hipStream_t s = nullptr;    // the stream launches go to (the call's, or s2 for look-ahead panels)
   hipStream_t s2 = nullptr;   // look-ahead panel stream (default priority)
+  hipStream_t scap = nullptr; // capture stream (default priority)
   hipEvent_t evA = nullptr, evB = nullptr;
   int64_t launches = 0;
 
@@ -512,7 +542,11 @@ template <typename T>
 int grp_getf2(Group& g, int64_t k, int w) {
   const int np = g.active(k);
   if (np == 0) return MG_OK;
-  hipLaunchKernelGGL(k_grp_getf2<T>, dim3(np), dim3(kGThreads), 0, g.s, g.dtab, k, w);
+  switch (g.cfg.lthreads) {
+    case 1024: hipLaunchKernelGGL((k_grp_getf2<T, 1024>), dim3(np), dim3(1024), 0, g.s, g.dtab, k, w); break;
+    case 512: hipLaunchKernelGGL((k_grp_getf2<T, 512>), dim3(np), dim3(512), 0, g.s, g.dtab, k, w); break;
+    default: hipLaunchKernelGGL((k_grp_getf2<T, 256>), dim3(np), dim3(256), 0, g.s, g.dtab, k, w); break;
+  }
   MG_HIP(hipGetLastError());
   ++g.launches;
   return MG_OK;
@@ -557,13 +591,14 @@ int grp_trsm_leaf(Group& g, bool upper, int64_t r0, int w, int64_t c0, int64_t c
 }
 
 // C -= A B over the group; split: the complex-float trailing update on BF16x3
-// products; tile: 0 = 128 x 64 (three-product FP64 / FP32), 1 = 64 x 64.
+// products; tile: 0 = 128 x 64 (three-product FP64 / FP32), 1 = 64 x 64,
+// 2 = 128 x 32 (three-product; products with few columns, as the residuals).
 template <typename T>
 int grp_gemm(Group& g, const GrpGemm& op, bool split, int tile) {
   const int np = g.active(op.k0);   // the depth needs rows beyond k0
   if (np == 0) return MG_OK;
-  const int BM = split ? (tile == 0 ? 128 : 64) : (tile == 0 ? 128 : 64);
-  const int BN = split ? 128 : 64;
+  const int BM = tile == 1 ? 64 : 128;
+  const int BN = split ? 128 : (tile == 2 ? 32 : 64);
   auto ext = [&](int64_t e, int64_t s, int64_t n, bool clamp) {
     const int64_t x = clamp ? std::min(e, n) : e;
     return x > s ? x - s : 0;
@@ -582,6 +617,8 @@ int grp_gemm(Group& g, const GrpGemm& op, bool split, int tile) {
     else hipLaunchKernelGGL((k_grp_gemm_split<2, 64, 128, 2, 2>), dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
   } else if (tile == 0) {
     hipLaunchKernelGGL((k_grp_gemm<T, 128, 64, 16, 4, 2, true>), dim3(total), dim3(512), 0, g.s, g.dtab, m, op);
+  } else if (tile == 2) {
+    hipLaunchKernelGGL((k_grp_gemm<T, 128, 32, 16, 4, 2, true>), dim3(total), dim3(512), 0, g.s, g.dtab, m, op);
   } else {
     constexpr bool F64 = std::is_same<T, double>::value;
     hipLaunchKernelGGL((k_grp_gemm<T, 64, 64, 16, 2, 2, !F64>), dim3(total), dim3(256), 0, g.s, g.dtab, m, op);
@@ -659,13 +696,14 @@ int grp_getrs(Group& g) {
   if (nr == 0 || nmax == 0) return MG_OK;
   MG_TRY(grp_laswp<T>(g, 0, INT64_MAX, 0, nr, true));
   const int64_t sb = kGTri;
+  const int tile = nr <= 32 ? 2 : 1;
   for (int64_t r0 = 0; r0 < nmax; r0 += sb) {
     MG_TRY(grp_trsm_leaf<T>(g, false, r0, (int)sb, 0, nr, true));
-    MG_TRY(grp_gemm<T>(g, GrpGemm{r0 + sb, INT64_MAX, 0, nr, r0, r0 + sb, 1}, false, 1));
+    MG_TRY(grp_gemm<T>(g, GrpGemm{r0 + sb, INT64_MAX, 0, nr, r0, r0 + sb, 1}, false, tile));
   }
   for (int64_t r0 = ((nmax - 1) / sb) * sb; r0 >= 0; r0 -= sb) {
     MG_TRY(grp_trsm_leaf<T>(g, true, r0, (int)sb, 0, nr, true));
-    if (r0 > 0) MG_TRY(grp_gemm<T>(g, GrpGemm{0, r0, 0, nr, r0, r0 + sb, 1}, false, 1));
+    if (r0 > 0) MG_TRY(grp_gemm<T>(g, GrpGemm{0, r0, 0, nr, r0, r0 + sb, 1}, false, tile));
   }
   return MG_OK;
 }
@@ -681,6 +719,181 @@ int run_factor(Group& g) {
 
 int run_solve(Group& g) {
   return g.prec == MG_C128 ? grp_getrs<double>(g) : grp_getrs<float>(g);
+}
+
+// B -= A X for every problem (X: the auxiliary blocks), products in the
+// group's element type.
+int run_residual(Group& g) {
+  const GrpGemm op{0, INT64_MAX, 0, g.nrhs, 0, INT64_MAX, 2};
+  const int tile = g.nrhs <= 32 ? 2 : 0;
+  return g.prec == MG_C128 ? grp_gemm<double>(g, op, false, tile) : grp_gemm<float>(g, op, false, tile);
+}
+
+int run_colnorm(Group& g, double* out) {
+  const dim3 grid((unsigned)((g.nrhs + 63) / 64), (unsigned)g.P);
+  if (g.prec == MG_C128) hipLaunchKernelGGL(k_grp_colnorm<double2>, grid, dim3(256), 0, g.s, g.dtab, (int)g.nrhs, out);
+  else hipLaunchKernelGGL(k_grp_colnorm<float2>, grid, dim3(256), 0, g.s, g.dtab, (int)g.nrhs, out);
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+// ---- autotuning -------------------------------------------------------------
+// The tunables are timed on a sample of the group (every k-th problem, at
+// most kTuneProblems and kTuneBytes of copies, so the sample keeps the
+// group's size mix) and the fastest is cached per (device, precision,
+// ceil log2 of the largest order).
+constexpr int kTuneProblems = 32;
+constexpr int64_t kTuneBytes = 12ll << 30;
+
+struct TuneKey {
+  int dev, prec, nlog;
+  bool operator<(const TuneKey& o) const {
+    if (dev != o.dev) return dev < o.dev;
+    if (prec != o.prec) return prec < o.prec;
+    return nlog < o.nlog;
+  }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, GrpCfg> g_tune_cache;
+
+int ceil_log2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
+}
+
+TuneKey tune_key(const Group& g) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return TuneKey{dev, g.prec, ceil_log2(g.P ? g.n[0] : 1)};
+}
+
+// panel widths around the measured best (256 for the 3D ensemble's orders
+// 1500 .. 9000 in both precisions; the 1024-thread leaf and the 128 x 64
+// tile won there at every width)
+const GrpCfg kCandidates[] = {{256, 16, 0, 1024}, {128, 16, 0, 1024}, {384, 16, 0, 1024}};
+
+int tune(Group& g, hipStream_t s, float* best_ms) {
+  const size_t esz = g.prec == MG_C128 ? 16 : 8;
+  // the sample: every stride-th problem (sorted order), within the byte cap
+  int stride = (g.P + kTuneProblems - 1) / kTuneProblems;
+  std::vector<int> pick;
+  for (;;) {
+    pick.clear();
+    int64_t bytes = 0;
+    for (int i = 0; i < g.P; i += stride) {
+      pick.push_back(i);
+      bytes += g.n[i] * g.n[i] * (int64_t)esz;
+    }
+    if (bytes <= kTuneBytes || stride >= g.P) break;
+    stride *= 2;
+  }
+  Group t;
+  t.prec = g.prec;
+  t.P = (int)pick.size();
+  t.s2 = g.s2;
+  t.evA = g.evA;
+  t.evB = g.evB;
+  std::vector<void*> bufs;
+  int st = MG_OK;
+  int* ints = nullptr;
+  int64_t nint = 0;
+  for (int i : pick) nint += g.n[i] + 1;
+  if (hipMalloc(&ints, nint * sizeof(int)) != hipSuccess) {
+    set_error("group tuning: allocation failed");
+    return MG_ERR_NOMEM;
+  }
+  int64_t ioff = 0;
+  auto q_of = [&](int i) { return (int)(std::find(pick.begin(), pick.end(), i) - pick.begin()); };
+  for (int i : pick) {
+    void* a = nullptr;
+    if (hipMalloc(&a, g.n[i] * g.n[i] * esz) != hipSuccess) {
+      st = MG_ERR_NOMEM;
+      set_error("group tuning: allocation failed");
+      break;
+    }
+    bufs.push_back(a);
+    GrpProb e{a, g.n[i], g.n[i], ints + ioff, ints + ioff + g.n[i], nullptr, 0, nullptr, 0, (int64_t)q_of(i)};
+    ioff += g.n[i] + 1;
+    t.host.push_back(e);
+    t.n.push_back(g.n[i]);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == MG_OK && (hipMalloc(&t.dtab, sizeof(GrpProb) * t.P) != hipSuccess ||
+                      hipMemcpy(t.dtab, t.host.data(), sizeof(GrpProb) * t.P, hipMemcpyHostToDevice) != hipSuccess ||
+                      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+    st = MG_ERR_HIP;
+    set_error("group tuning: set-up failed");
+  }
+  GrpCfg best = g.cfg;
+  float bms = -1.f;
+  auto refill = [&]() -> int {
+    for (size_t q = 0; q < pick.size(); ++q) {
+      const GrpProb& src = g.host[pick[q]];
+      MG_HIP(hipMemcpy2DAsync(bufs[q], g.n[pick[q]] * esz, src.A, src.lda * esz, g.n[pick[q]] * esz,
+                              g.n[pick[q]], hipMemcpyDeviceToDevice, s));
+    }
+    return MG_OK;
+  };
+  t.s = s;
+  const int nc = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+  for (int c = -1; c < nc && st == MG_OK; ++c) {   // c = -1: untimed warm-up
+    t.cfg = kCandidates[c < 0 ? 0 : c];
+    if ((st = refill()) != MG_OK) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { st = MG_ERR_HIP; break; }
+    if ((st = run_factor(t)) != MG_OK) break;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { st = MG_ERR_HIP; break; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (c >= 0 && (bms < 0 || ms < bms)) {
+      bms = ms;
+      best = t.cfg;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipStreamSynchronize(g.s2);
+  for (void* b : bufs) (void)hipFree(b);
+  (void)hipFree(ints);
+  if (t.dtab) (void)hipFree(t.dtab);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st != MG_OK) return st;
+  g.cfg = best;
+  if (best_ms) *best_ms = bms;
+  return MG_OK;
+}
+
+// Record op (0: factor, 1: solve) into the group's graph slot: the launch
+// sequence is captured on the group's capture stream (nothing runs) and
+// instantiated; replay launches it on any stream.  The sequence holds no
+// memset nodes (info is cleared by a kernel).
+int capture(Group& g, int op) {
+  if (g.exec[op]) {
+    (void)hipGraphExecDestroy(g.exec[op]);
+    g.exec[op] = nullptr;
+  }
+  if (g.graph[op]) {
+    (void)hipGraphDestroy(g.graph[op]);
+    g.graph[op] = nullptr;
+  }
+  MG_HIP(hipStreamBeginCapture(g.scap, hipStreamCaptureModeThreadLocal));
+  g.s = g.scap;
+  const int st = op == 0 ? run_factor(g) : run_solve(g);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(g.scap, &graph);
+  if (st != MG_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) {
+    set_error("group capture: %s", hipGetErrorString(e));
+    return MG_ERR_HIP;
+  }
+  g.graph[op] = graph;
+  MG_HIP(hipGraphInstantiate(&g.exec[op], graph, nullptr, nullptr, 0));
+  return MG_OK;
 }
 
 }  // namespace
@@ -715,11 +928,13 @@ int mg_group_create(int prec, int count, void* const* A, const int64_t* n, const
   std::stable_sort(g.order.begin(), g.order.end(), [&](int a, int b) { return n[a] > n[b]; });
   for (int i = 0; i < count; ++i) {
     const int c = g.order[i];
-    mg::GrpProb e{A[c], lda[c], n[c], ipiv[c], info[c], nrhs > 0 ? B[c] : nullptr, nrhs > 0 ? ldb[c] : 0, 0};
+    mg::GrpProb e{A[c], lda[c], n[c], ipiv[c], info[c], nrhs > 0 ? B[c] : nullptr, nrhs > 0 ? ldb[c] : 0,
+                  nullptr, 0, c};
     g.host.push_back(e);
     g.n.push_back(n[c]);
   }
   if (hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&g.scap, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&g.evA, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&g.evB, hipEventDisableTiming) != hipSuccess) {
     mg_group_destroy(h);
@@ -746,8 +961,11 @@ int mg_group_create(int prec, int count, void* const* A, const int64_t* n, const
 int mg_group_destroy(mg_group* h) {
   if (!h) return MG_OK;
   if (h->g.s2) (void)hipStreamSynchronize(h->g.s2);
-  if (h->g.exec) (void)hipGraphExecDestroy(h->g.exec);
-  if (h->g.graph) (void)hipGraphDestroy(h->g.graph);
+  for (int op = 0; op < 2; ++op) {
+    if (h->g.exec[op]) (void)hipGraphExecDestroy(h->g.exec[op]);
+    if (h->g.graph[op]) (void)hipGraphDestroy(h->g.graph[op]);
+  }
+  if (h->g.scap) (void)hipStreamDestroy(h->g.scap);
   if (h->g.dtab) (void)hipFree(h->g.dtab);
   if (h->g.evA) (void)hipEventDestroy(h->g.evA);
   if (h->g.evB) (void)hipEventDestroy(h->g.evB);
@@ -756,13 +974,15 @@ int mg_group_destroy(mg_group* h) {
   return MG_OK;
 }
 
-int mg_group_configure(mg_group* h, int nb, int leaf, int tile) {
+int mg_group_configure(mg_group* h, int nb, int leaf, int tile, int leaf_threads) {
   if (!h) return MG_ERR_ARG;
-  if (nb < 16 || nb > 1024 || nb % 16 != 0 || leaf < 1 || leaf > kGLeaf || (tile != 0 && tile != 1))
+  if (nb < 16 || nb > 1024 || nb % 16 != 0 || leaf < 1 || leaf > kGLeaf || (tile != 0 && tile != 1) ||
+      (leaf_threads != 256 && leaf_threads != 512 && leaf_threads != 1024))
     return MG_ERR_ARG;
   h->g.cfg.nb = nb;
   h->g.cfg.leaf = leaf;
   h->g.cfg.tile = tile;
+  h->g.cfg.lthreads = leaf_threads;
   return MG_OK;
 }
 
@@ -779,5 +999,77 @@ int mg_group_getrs(mg_group* h, void* stream) {
 }
 
 int64_t mg_group_launches(const mg_group* h) { return h ? h->g.launches : -1; }
+
+int mg_group_tune(mg_group* h, int* cfg_out, float* best_ms, void* stream) {
+  if (!h) return MG_ERR_ARG;
+  mg::Group& g = h->g;
+  const mg::TuneKey key = mg::tune_key(g);
+  int st = MG_OK;
+  {
+    std::lock_guard<std::mutex> lock(mg::g_tune_mu);
+    auto it = mg::g_tune_cache.find(key);
+    if (it != mg::g_tune_cache.end()) {
+      g.cfg = it->second;
+      if (best_ms) *best_ms = 0.f;
+    } else {
+      st = mg::tune(g, reinterpret_cast<hipStream_t>(stream), best_ms);
+      if (st == MG_OK) mg::g_tune_cache[key] = g.cfg;
+    }
+  }
+  if (st == MG_OK && cfg_out) {
+    cfg_out[0] = g.cfg.nb;
+    cfg_out[1] = g.cfg.leaf;
+    cfg_out[2] = g.cfg.tile;
+    cfg_out[3] = g.cfg.lthreads;
+  }
+  return st;
+}
+
+int mg_group_set_aux(mg_group* h, void* const* X, const int64_t* ldx) {
+  if (!h || !X || !ldx) return MG_ERR_ARG;
+  mg::Group& g = h->g;
+  if (g.nrhs == 0) return MG_ERR_ARG;
+  for (int i = 0; i < g.P; ++i) {
+    const int c = g.order[i];
+    if (!X[c] || ldx[c] < g.nrhs) return MG_ERR_ARG;
+    g.host[i].X = X[c];
+    g.host[i].ldx = ldx[c];
+  }
+  if (g.P > 0) MG_HIP(hipMemcpy(g.dtab, g.host.data(), sizeof(mg::GrpProb) * g.P, hipMemcpyHostToDevice));
+  return MG_OK;
+}
+
+int mg_group_residual(mg_group* h, void* stream) {
+  if (!h || h->g.nrhs == 0) return MG_ERR_ARG;
+  for (const mg::GrpProb& e : h->g.host)
+    if (!e.X) {
+      mg::set_error("mg_group_residual: no auxiliary blocks (mg_group_set_aux)");
+      return MG_ERR_ARG;
+    }
+  h->g.s = reinterpret_cast<hipStream_t>(stream);
+  return mg::run_residual(h->g);
+}
+
+int mg_group_colnorm(mg_group* h, double* norms, void* stream) {
+  if (!h || !norms || h->g.nrhs == 0) return MG_ERR_ARG;
+  h->g.s = reinterpret_cast<hipStream_t>(stream);
+  return mg::run_colnorm(h->g, norms);
+}
+
+int mg_group_capture(mg_group* h, int op) {
+  if (!h || (op != 0 && op != 1)) return MG_ERR_ARG;
+  if (op == 1 && h->g.nrhs == 0) return MG_ERR_ARG;
+  return mg::capture(h->g, op);
+}
+
+int mg_group_replay(mg_group* h, int op, void* stream) {
+  if (!h || (op != 0 && op != 1)) return MG_ERR_ARG;
+  if (!h->g.exec[op]) {
+    mg::set_error("mg_group_replay: nothing captured for op %d", op);
+    return MG_ERR_ARG;
+  }
+  MG_HIP(hipGraphLaunch(h->g.exec[op], reinterpret_cast<hipStream_t>(stream)));
+  return MG_OK;
+}
 
 }  // extern "C"

@@ -459,7 +459,7 @@ class GroupLU:
 
     MAX = 256
 
-    def __init__(self, mats, rhs=None, prec="c128", nb=128, leaf=16, tile=0):
+    def __init__(self, mats, rhs=None, prec="c128", nb=256, leaf=16, tile=0, leaf_threads=1024):
         lib = L.load()
         if not 0 < len(mats) <= self.MAX:
             raise ValueError("a group holds 1 to %d problems" % self.MAX)
@@ -496,22 +496,79 @@ class GroupLU:
             L.check(lib.mg_group_create(PRECS[prec], P, *self._keep, nrhs if rhs is not None else 0,
                                         ctypes.byref(h)), "mg_group_create")
         self.h = h
-        L.check(lib.mg_group_configure(self.h, int(nb), int(leaf), int(tile)), "mg_group_configure")
+        self._captured = set()
+        L.check(lib.mg_group_configure(self.h, int(nb), int(leaf), int(tile), int(leaf_threads)),
+                "mg_group_configure")
 
     def ipiv(self, i):
         return self.ipiv_all[self.off[i]:self.off[i + 1]]
 
-    def factor(self):
+    def tune(self):
+        """Pick the tunables by timing candidates on a sample of this group's
+        matrices (copies; call before factor) unless a group of the same size
+        class was tuned before in this process.  Returns ((nb, leaf, tile,
+        leaf_threads), best ms or 0 for a cached choice)."""
+        cfg = (ctypes.c_int * 4)()
+        ms = ctypes.c_float()
         with torch.cuda.device(self.device):
-            L.check(L.load().mg_group_getrf(self.h, _stream(self.device)), "mg_group_getrf")
+            L.check(L.load().mg_group_tune(self.h, cfg, ctypes.byref(ms), _stream(self.device)), "mg_group_tune")
+        self._captured.clear()      # sequences captured before hold the previous tunables
+        return tuple(cfg), ms.value
+
+    def factor(self, graph=False):
+        """Factor in place (graph: replay the captured factor sequence,
+        capturing it on first use)."""
+        with torch.cuda.device(self.device):
+            if graph:
+                self._replay(0)
+            else:
+                L.check(L.load().mg_group_getrf(self.h, _stream(self.device)), "mg_group_getrf")
         return self
 
-    def solve(self):
+    def solve(self, graph=False):
+        """Solve the right-hand sides in place from the factors (graph: replay
+        the captured solve sequence, capturing it on first use)."""
         if self.rhs is None:
             raise ValueError("this group has no right-hand sides")
         with torch.cuda.device(self.device):
-            L.check(L.load().mg_group_getrs(self.h, _stream(self.device)), "mg_group_getrs")
+            if graph:
+                self._replay(1)
+            else:
+                L.check(L.load().mg_group_getrs(self.h, _stream(self.device)), "mg_group_getrs")
         return self.rhs
+
+    def set_aux(self, X):
+        """Register one (n_i, nrhs) block per problem (the matrices' dtype,
+        row-major) as the X of residual()."""
+        if self.rhs is None or len(X) != len(self.mats):
+            raise ValueError("one auxiliary block per problem of a group with right-hand sides")
+        for A, B, x in zip(self.mats, self.rhs, X):
+            if x.dtype != A.dtype or x.device != self.device or x.shape != B.shape or x.stride(1) != 1:
+                raise ValueError("auxiliary blocks: the right-hand sides' shape, the matrices' dtype")
+        P = len(X)
+        self._aux = ((ctypes.c_void_p * P)(*[_p(x) for x in X]), (ctypes.c_int64 * P)(*[_ld(x) for x in X]))
+        self._aux_t = list(X)
+        L.check(L.load().mg_group_set_aux(self.h, *self._aux), "mg_group_set_aux")
+
+    def residual(self):
+        """B_i -= A_i X_i for every problem in one launch (A_i unfactored)."""
+        with torch.cuda.device(self.device):
+            L.check(L.load().mg_group_residual(self.h, _stream(self.device)), "mg_group_residual")
+
+    def colnorms(self, out=None):
+        """(P, nrhs) float64: the column norms of every problem's right-hand sides."""
+        if out is None:
+            out = torch.empty((len(self.mats), self.rhs[0].shape[1]), dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            L.check(L.load().mg_group_colnorm(self.h, _p(out), _stream(self.device)), "mg_group_colnorm")
+        return out
+
+    def _replay(self, op):
+        lib = L.load()
+        if op not in self._captured:
+            L.check(lib.mg_group_capture(self.h, op), "mg_group_capture")
+            self._captured.add(op)
+        L.check(lib.mg_group_replay(self.h, op, _stream(self.device)), "mg_group_replay")
 
     @property
     def launches(self):

@@ -34,15 +34,15 @@ def _unpack(LU):
     return np.tril(LU, -1) + np.eye(n), np.triu(LU)
 
 
-@pytest.mark.parametrize("cfg", [(128, 16, 0), (64, 8, 1), (256, 16, 0), (32, 4, 1)])
+@pytest.mark.parametrize("cfg", [(128, 16, 0, 1024), (64, 8, 1, 512), (256, 16, 0, 256), (32, 4, 1, 256)])
 def test_group_factors_match_lapack(cfg):
     """Pivots equal LAPACK's for every problem; P A = L U to rounding; the
     factors agree with zgetrf's."""
-    nb, leaf, tile = cfg
+    nb, leaf, tile, lt = cfg
     say("group getrf vs lapack", cfg)
     host = _mats(SIZES, 1)
     mats = [torch.from_numpy(A).to(DEV) for A in host]
-    g = E.GroupLU(mats, prec="c128", nb=nb, leaf=leaf, tile=tile).factor()
+    g = E.GroupLU(mats, prec="c128", nb=nb, leaf=leaf, tile=tile, leaf_threads=lt).factor()
     torch.cuda.synchronize()
     assert (g.info.cpu() == 0).all()
     for i, A in enumerate(host):
@@ -129,3 +129,26 @@ def test_group_complex_float(prec, tol):
         err = np.linalg.norm(X - ref) / np.linalg.norm(ref)
         assert err < tol, (i, err)
     g.close()
+
+
+def test_group_residual_and_colnorms():
+    """B_i -= A_i X_i over the group in one launch and the column norms,
+    against numpy, complex128 and complex64."""
+    say("group residual")
+    for dt, tol in ((np.complex128, 1e-13), (np.complex64, 1e-5)):
+        host = _mats([300, 41, 1, 129], 7, dtype=dt, shrink=1.0)
+        rng = np.random.default_rng(8)
+        Bh = [(rng.normal(size=(A.shape[0], 5)) + 1j * rng.normal(size=(A.shape[0], 5))).astype(dt) for A in host]
+        Xh = [(rng.normal(size=(A.shape[0], 5)) + 1j * rng.normal(size=(A.shape[0], 5))).astype(dt) for A in host]
+        mats = [torch.from_numpy(A).to(DEV) for A in host]
+        rhs = [torch.from_numpy(B).to(DEV) for B in Bh]
+        g = E.GroupLU(mats, rhs, prec="c128" if dt == np.complex128 else "c64")
+        g.set_aux([torch.from_numpy(X).to(DEV) for X in Xh])
+        g.residual()
+        nrm = g.colnorms().cpu().numpy()
+        for i, (A, B, X) in enumerate(zip(host, Bh, Xh)):
+            ref = B.astype(np.complex128) - A.astype(np.complex128) @ X.astype(np.complex128)
+            got = rhs[i].cpu().numpy()
+            assert np.abs(got - ref).max() <= tol * np.abs(A).max() * A.shape[0], i
+            assert np.allclose(nrm[i], np.linalg.norm(got.astype(np.complex128), axis=0), rtol=1e-12), i
+        g.close()
