@@ -102,6 +102,14 @@ __device__ inline T2 crecip(T2 a) {
 }
 
 template <typename T2>
+__device__ inline T2 zero2() {
+  T2 z;
+  z.x = 0;
+  z.y = 0;
+  return z;
+}
+
+template <typename T2>
 __device__ inline T2 cmul(T2 a, T2 b) {
   T2 r;
   r.x = a.x * b.x - a.y * b.y;
@@ -138,15 +146,53 @@ __device__ inline int64_t clamp_extent(int64_t end, int64_t n, int64_t start, bo
 // ---- leaf: one workgroup per problem ---------------------------------------
 // Leaf LU of rows [k, n) x columns [k, k + w') of problem blockIdx.x, w' =
 // min(w, n - k), by one workgroup of NT threads.  Lane c of each 16-lane
-// group holds column c of one row.
-// Pass j (0 <= j <= w'): rows r >= j get column j-1 scaled by the inverse of
-// pivot j-1 and columns >= j updated by it, and column j's pivot search runs
-// on the updated values; then rows j and the pivot row swap within the leaf.
+// group holds column c of one row.  Pass j (0 <= j <= w'): rows r >= j get
+// column j-1 scaled by the inverse of pivot j-1 and columns >= j updated by
+// it, and column j's pivot search runs on the updated values; then rows j
+// and the pivot row swap within the leaf.
+// The first LR rows of the leaf stay in LDS and the next RS * QR in
+// registers (slot q of the lane group rr: row LR + rr + RS q) for all
+// passes; only the rows beyond stream through the caches once per pass.
+// (144 KiB of LDS; 32 VGPRs of slots at 1024 threads, where a lane has 128
+// VGPRs, 40 below)
+template <typename T, int NT> struct LeafCache;
+template <int NT> struct LeafCache<double, NT> { static constexpr int LR = 576, QR = NT == 1024 ? 8 : 10; };
+template <int NT> struct LeafCache<float, NT> { static constexpr int LR = 1152, QR = NT == 1024 ? 16 : 20; };
+
+// One row of a leaf pass, held by a 16-lane group (lane c: column c): the
+// update by pivot j-1 (multiplier from lane j-1, scaled by the pivot's
+// inverse) and the search of column j.
+template <typename T2>
+__device__ inline void leaf_row(T2& x, int r, bool valid, bool upd, bool nz, T2 inv, T2 u, int c, int j, int gbase,
+                                double& best, int& brow) {
+  if (upd) {
+    T2 l;
+    l.x = __shfl(x.x, gbase + j - 1);
+    l.y = __shfl(x.y, gbase + j - 1);
+    if (nz) l = cmul(l, inv);
+    if (c == j - 1) {
+      x = l;
+    } else if (c >= j) {
+      x.x -= l.x * u.x - l.y * u.y;
+      x.y -= l.x * u.y + l.y * u.x;
+    }
+  }
+  if (c == j && valid) {
+    const double a = fabs((double)x.x) + fabs((double)x.y);
+    if (a > best) {
+      best = a;
+      brow = r;
+    }
+  }
+}
+
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_grp_getf2(const GrpProb* __restrict__ tab, int64_t k, int w) {
   typedef typename C2<T>::t T2;
-  constexpr int kGRows = NT / kGLeaf;   // rows per sweep
-  __shared__ T2 urow[kGLeaf];
+  constexpr int RS = NT / kGLeaf;   // rows per sweep
+  constexpr int LR = LeafCache<T, NT>::LR, QR = LeafCache<T, NT>::QR, CR = LR + RS * QR;
+  __shared__ T2 cache[LR][kGLeaf];
+  __shared__ T2 urow[kGLeaf], rowa[kGLeaf], rowb[kGLeaf];
   __shared__ T2 sinv;
   __shared__ int snz;
   __shared__ double sv[NT / 64];
@@ -155,59 +201,65 @@ __global__ __launch_bounds__(NT) void k_grp_getf2(const GrpProb* __restrict__ ta
   if (pb.n <= k) return;
   const int m = (int)(pb.n - k);
   const int wp = (int)(w < pb.n - k ? w : pb.n - k);
+  const int mc = m < LR ? m : LR, mr = m < CR ? m : CR;
   const int64_t lda = pb.lda;
-  T2* A = reinterpret_cast<T2*>(pb.A) + k * lda + k;
+  gptr<T2> A = (gptr<T2>)(reinterpret_cast<T2*>(pb.A) + k * lda + k);
   const int tid = threadIdx.x, c = tid & (kGLeaf - 1), rr = tid / kGLeaf;
   const int lane = tid & 63, gbase = lane & ~(kGLeaf - 1), wave = tid >> 6;
+  // the on-chip rows
+  for (int e = tid; e < mc * kGLeaf; e += NT) {
+    const int r = e / kGLeaf;
+    cache[r][c] = c < wp ? A[(int64_t)r * lda + c] : zero2<T2>();
+  }
+  T2 reg[QR];
+#pragma unroll
+  for (int q = 0; q < QR; ++q) {
+    const int r = LR + rr + RS * q;
+    reg[q] = (r < mr && c < wp) ? A[(int64_t)r * lda + c] : zero2<T2>();
+  }
   if (tid == 0) {
     snz = 0;
-    sinv.x = 0;
-    sinv.y = 0;
+    sinv = zero2<T2>();
   }
   __syncthreads();
   for (int j = 0; j <= wp; ++j) {
     const bool upd = j > 0;
     const bool nz = snz != 0;
     const T2 inv = sinv;
-    T2 u;
-    u.x = 0;
-    u.y = 0;
+    T2 u = zero2<T2>();
     if (upd && c < wp) u = urow[c];
-    const bool lc = c < wp && (upd ? c >= j - 1 : c == 0);
+    const bool act = c < wp && (upd ? c >= j - 1 : c == 0);
     double best = -1.0;
     int brow = INT_MAX;
-    for (int base = j; base < m; base += kGRows * kGUnroll) {
+    for (int base = j; base < mc; base += RS) {            // LDS rows
+      const int r = base + rr;
+      const bool v = r < mc;
+      T2 x = (v && act) ? cache[r][c] : zero2<T2>();
+      leaf_row(x, r, v, upd, nz, inv, u, c, j, gbase, best, brow);
+      if (v && act) cache[r][c] = x;
+    }
+    if (mr > LR) {                                         // register rows
+#pragma unroll
+      for (int q = 0; q < QR; ++q) {
+        const int r = LR + rr + RS * q;
+        const bool v = r >= j && r < mr;
+        T2 x = reg[q];
+        leaf_row(x, r, v, upd, nz, inv, u, c, j, gbase, best, brow);
+        if (v && act) reg[q] = x;
+      }
+    }
+    for (int base = j > CR ? j : CR; base < m; base += RS * kGUnroll) {   // streamed rows
       T2 x[kGUnroll];
 #pragma unroll
       for (int q = 0; q < kGUnroll; ++q) {
-        const int r = base + rr + q * kGRows;
-        x[q].x = 0;
-        x[q].y = 0;
-        if (lc && r < m) x[q] = A[(int64_t)r * lda + c];
+        const int r = base + rr + q * RS;
+        x[q] = (act && r < m) ? A[(int64_t)r * lda + c] : zero2<T2>();
       }
 #pragma unroll
       for (int q = 0; q < kGUnroll; ++q) {
-        const int r = base + rr + q * kGRows;
-        if (upd) {
-          T2 l;
-          l.x = __shfl(x[q].x, gbase + j - 1);
-          l.y = __shfl(x[q].y, gbase + j - 1);
-          if (nz) l = cmul(l, inv);
-          if (c == j - 1) {
-            x[q] = l;
-          } else if (c >= j) {
-            x[q].x -= l.x * u.x - l.y * u.y;
-            x[q].y -= l.x * u.y + l.y * u.x;
-          }
-        }
-        if (c == j && r < m) {
-          const double a = fabs((double)x[q].x) + fabs((double)x[q].y);
-          if (a > best) {
-            best = a;
-            brow = r;
-          }
-        }
-        if (lc && r < m) A[(int64_t)r * lda + c] = x[q];
+        const int r = base + rr + q * RS;
+        leaf_row(x[q], r, r < m, upd, nz, inv, u, c, j, gbase, best, brow);
+        if (act && r < m) A[(int64_t)r * lda + c] = x[q];
       }
     }
     if (j == wp) break;
@@ -235,15 +287,48 @@ __global__ __launch_bounds__(NT) void k_grp_getf2(const GrpProb* __restrict__ ta
         brow = sr[q];
       }
     const int p = (brow == INT_MAX) ? j : brow;   // whole column NaN: keep row j
-    if (tid < wp) {
-      const T2 a = A[(int64_t)j * lda + tid];
-      const T2 b = A[(int64_t)p * lda + tid];
-      if (p != j) {
-        A[(int64_t)j * lda + tid] = b;
-        A[(int64_t)p * lda + tid] = a;
+    // rows j and p wherever they live (LDS, a lane group's register slot,
+    // memory): gather both, then write them swapped
+    const int rels[2] = {j - LR, p - LR};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int r = t == 0 ? j : p;
+      T2* dst = t == 0 ? rowa : rowb;
+      if (r < LR) {
+        if (tid < wp) dst[tid] = cache[r][tid];
+      } else if (r < CR) {
+        if (rr == rels[t] % RS && c < wp) {
+          const int qs = rels[t] / RS;
+#pragma unroll
+          for (int q = 0; q < QR; ++q)
+            if (q == qs) dst[c] = reg[q];
+        }
+      } else if (tid < wp) {
+        dst[tid] = A[(int64_t)r * lda + tid];
       }
-      urow[tid] = b;
     }
+    __syncthreads();
+    if (p != j) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int r = t == 0 ? j : p;
+        const T2* src = t == 0 ? rowb : rowa;
+        if (r < LR) {
+          if (tid < wp) cache[r][tid] = src[tid];
+        } else if (r < CR) {
+          if (rr == rels[t] % RS && c < wp) {
+            const int qs = rels[t] / RS;
+            const T2 v = src[c];
+#pragma unroll
+            for (int q = 0; q < QR; ++q)
+              if (q == qs) reg[q] = v;
+          }
+        } else if (tid < wp) {
+          A[(int64_t)r * lda + tid] = src[tid];
+        }
+      }
+    }
+    if (tid < wp) urow[tid] = rowb[tid];
     __syncthreads();
     if (tid == 0) {
       pb.ipiv[k + j] = (int)(k + p);
@@ -251,14 +336,25 @@ __global__ __launch_bounds__(NT) void k_grp_getf2(const GrpProb* __restrict__ ta
       const bool z = piv.x == 0 && piv.y == 0;
       snz = z ? 0 : 1;
       if (z) {
-        sinv.x = 0;
-        sinv.y = 0;
+        sinv = zero2<T2>();
         atomicCAS(pb.info, 0, (int)(k + j + 1));
       } else {
         sinv = crecip(piv);
       }
     }
     __syncthreads();
+  }
+  // the on-chip rows back to the matrix (LDS rows were last written by other
+  // lane groups in the final pass)
+  __syncthreads();
+  for (int e = tid; e < mc * kGLeaf; e += NT) {
+    const int r = e / kGLeaf;
+    if (c < wp) A[(int64_t)r * lda + c] = cache[r][c];
+  }
+#pragma unroll
+  for (int q = 0; q < QR; ++q) {
+    const int r = LR + rr + RS * q;
+    if (r < mr && c < wp) A[(int64_t)r * lda + c] = reg[q];
   }
 }
 

@@ -12,7 +12,9 @@ pytestmark = pytest.mark.gpu
 from magreete_amd import engine as E
 
 DEV = "cuda:0"
-SIZES = [700, 33, 1, 256, 1029, 130, 64, 65, 17, 128, 129, 300]
+# orders around the leaf's on-chip rows (576 in LDS, then register slots, then
+# streamed rows: 1088 complex128 / 2176 complex64 rows at 1024 threads)
+SIZES = [700, 33, 1, 256, 1029, 130, 64, 65, 17, 128, 129, 300, 1100, 2300]
 
 
 def say(*a):
@@ -51,7 +53,8 @@ def test_group_factors_match_lapack(cfg):
         ip = g.ipiv(i).cpu().numpy()
         assert np.array_equal(ip, piv), (i, A.shape)
         err = np.abs(mine - lu).max() / np.abs(lu).max()
-        assert err < 1e-12, (i, A.shape, err)
+        # both factorisations round differently (blocking, FMA): ~eps n (2e-12 at n = 2300)
+        assert err < 5e-15 * max(A.shape[0], 100), (i, A.shape, err)
         Lm, Um = _unpack(mine)
         PA = A.copy()
         for r, p in enumerate(ip):
