@@ -393,14 +393,15 @@ __global__ __launch_bounds__(256) void k_grp_laswp(const GrpProb* __restrict__ t
 // upper (backward).  X: rows [r0, r0 + w'), columns [c0, c1) of the matrix
 // (c1 clamped to n) or of the right-hand sides.  One row per lane; each wave
 // carries kTriCols columns through the substitution together, four waves per
-// workgroup.
+// chunk, kTriChunks chunks per workgroup (T is read into LDS once for them).
 struct GrpTri {
   int64_t r0, c0, c1;
   int w, rhs;
 };
 
 constexpr int kTriCols = 8;
-constexpr int kTriWgCols = 4 * kTriCols;
+constexpr int kTriChunks = 1;   // column chunks per workgroup (more chunks: fewer loads of T, longer chains; 8 measured slower)
+constexpr int kTriWgCols = 4 * kTriCols * kTriChunks;
 
 template <typename T, bool UPPER>
 __global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ tab, GrpMap map, GrpTri op) {
@@ -417,9 +418,12 @@ __global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ ta
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t c1 = op.rhs ? op.c1 : (op.c1 < pb.n ? op.c1 : pb.n);
-  const int64_t cb = op.c0 + (int64_t)(blockIdx.x - map.first[p]) * kTriWgCols + (threadIdx.x >> 6) * kTriCols;
   const int64_t ldx = op.rhs ? pb.ldb : pb.lda;
   T2* X = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A) + op.r0 * ldx;
+  const int64_t cw = op.c0 + (int64_t)(blockIdx.x - map.first[p]) * kTriWgCols;
+  for (int ch = 0; ch < kTriChunks; ++ch) {
+  const int64_t cb = cw + (int64_t)ch * 4 * kTriCols + (threadIdx.x >> 6) * kTriCols;
+  if (cw + (int64_t)ch * 4 * kTriCols >= c1) break;   // (uniform)
   T2 x[kTriCols];
 #pragma unroll
   for (int q = 0; q < kTriCols; ++q) {
@@ -463,6 +467,7 @@ __global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ ta
 #pragma unroll
   for (int q = 0; q < kTriCols; ++q)
     if (cb + q < c1 && lane < wp) X[lane * ldx + cb + q] = x[q];
+  }
 }
 
 // ---- products --------------------------------------------------------------
@@ -774,14 +779,18 @@ int grp_getrf(Group& g, bool split) {
       MG_HIP(hipEventRecord(g.evA, s));
       MG_HIP(hipStreamWaitEvent(g.s2, g.evA, 0));
       g.s = g.s2;
-      const int st = grp_panel<T>(g, kn, nb);
+      int st = grp_panel<T>(g, kn, nb);
+      // this step's swaps on the columns left of it (disjoint from the
+      // panel and from the rest of the update) ride on the panel's stream
+      if (st == MG_OK && k > 0) st = grp_laswp<T>(g, k, kn, 0, k, false);
       g.s = s;
       MG_TRY(st);
       MG_HIP(hipEventRecord(g.evB, g.s2));
       MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn + nb, INT64_MAX, k, kn, 0}, split, g.cfg.tile));
       MG_HIP(hipStreamWaitEvent(s, g.evB, 0));
+    } else if (k > 0) {
+      MG_TRY(grp_laswp<T>(g, k, kn, 0, k, false));
     }
-    if (k > 0) MG_TRY(grp_laswp<T>(g, k, kn, 0, k, false));
   }
   return MG_OK;
 }
