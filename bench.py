@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ensemble", type=int, default=256,
                     help="configurations of the batched-ensemble measurement (0: skip)")
+    ap.add_argument("--ensemble-runs", default="fp64_loop,fp64_streams4,bf16x3_streams4,fp64_grouped,bf16x3_grouped",
+                    help="which ensemble measurements to run (the first one is the reference of the errors)")
     ap.add_argument("--serving", type=int, default=40, help="requests per serving load level (0: skip)")
     ap.add_argument("--ldos-sites", type=int, default=20000,
                     help="2D TE perturbed lattice at about this many sites, for each k0 a in --k0a: LDOS at every "
@@ -400,7 +402,10 @@ def run_ensemble(args, dev):
     runs = (("fp64_loop", "fp64", 1, False), ("fp64_streams4", "fp64", 4, False),
             ("bf16x3_streams4", "bf16x3", 4, False), ("fp64_grouped", "fp64", 1, True),
             ("bf16x3_grouped", "bf16x3", 1, True))
+    wanted = set(args.ensemble_runs.split(","))
     for name, method, streams, grouped in runs:
+        if name not in wanted:
+            continue
         e = Ensemble(dev, streams=streams, method=method, tol=args.batch_tol, grouped=grouped)
         res = e.run(ens if grouped else ens[:4])
         del res
@@ -422,7 +427,9 @@ def run_ensemble(args, dev):
         print("# ensemble %s: %.1f configs/s" % (name, out[name]), file=sys.stderr, flush=True)
         del res
         torch.cuda.empty_cache()
-    out["grouped_vs_loop"] = {m: out[m + "_grouped"] / out["fp64_loop"] for m in ("fp64", "bf16x3")}
+    if "fp64_loop" in out:
+        out["grouped_vs_loop"] = {m: out[m + "_grouped"] / out["fp64_loop"] for m in ("fp64", "bf16x3")
+                                  if m + "_grouped" in out}
     return out
 
 
