@@ -32,7 +32,7 @@ KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 til
 # HBM bytes per trailing-update launch (average over one factorisation of this workload), measured
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the
 # counters' reading for that 2 GiB copy (fetch x2.000, write x1.000)
-TRAFFIC = {"fp64": 2.448e9, "c64": 4.410e8, "bf16x3": 1.806e8, "bf16x6": 2.709e8}
+TRAFFIC = {"fp64": 2.507e9, "c64": 4.410e8, "bf16x3": 1.658e8, "bf16x6": 2.709e8}
 
 
 def colerr(x, ref):

@@ -16,7 +16,12 @@
 
 namespace mg_sg {
 
-constexpr int BK = 32, KP = BK + 8;   // KP: padded k stride (bf16 elements)
+// KP: padded k stride (bf16 elements): 96-byte rows.  On gfx950 the 128-bit
+// reads of a fragment (16 rows, one 16-byte unit each) are free of bank
+// conflicts at a 96-byte row stride; 80 bytes (BK + 8) gave a two-way conflict
+// on every read (LDS probe: 232 us against 142 us, SQ_LDS_BANK_CONFLICT half
+// of the LDS-active cycles), as did 48, 112 and 144.
+constexpr int BK = 32, KP = BK + 16;
 
 // x -> NS bf16 parts, each the rounded remainder of the previous ones
 template <int NS>
@@ -29,7 +34,7 @@ __device__ inline void split_n(float x, unsigned short* p) {
   }
 }
 
-// One padding row per plane shifts consecutive planes by 20 LDS banks, so
+// One padding row per plane shifts consecutive planes by 24 LDS banks, so
 // that lanes storing the same (row, k) of different planes do not collide.
 template <int NS, int BM, int BN>
 struct alignas(16) Smem {
