@@ -522,6 +522,11 @@ __device__ inline bf16x8 unit(const u32x4* p) { return __builtin_bit_cast(bf16x8
 
 // acc += the k-tile in st (BF16x3: part pairs (p, q) with p + q <= 1 of every
 // real product, the imaginary parts of B negated for the real accumulator).
+// Term-major: for each half of the wave's columns, every product term goes
+// over that half's eight accumulators before the next term, so consecutive
+// MFMAs update different accumulators (the block-by-block order chained six
+// dependent MFMAs per accumulator; standalone 12032^2 x 256: 0.97 ms against
+// 1.18, the same bits: every accumulator sees its terms in the same order).
 __device__ inline void compute(const Stage& st, v4f (&cr)[TM][TN], v4f (&ci)[TM][TN]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave / WN, wn = wave % WN, li = lane & 15, lg = lane >> 4;
@@ -537,26 +542,39 @@ __device__ inline void compute(const Stage& st, v4f (&cr)[TM][TN], v4f (&ci)[TM]
     }
   }
 #pragma unroll
-  for (int b = 0; b < TN; ++b) {
-    const int col = wn * 64 + b * 16 + li;
-    bf16x8 br[2], bi[2], bn[2];
+  for (int h = 0; h < TN; h += 2) {
+    bf16x8 br[2][2], bi[2][2], bn[2][2];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      br[q] = unit(&st.b[col * UPR + ((lg * U + q) ^ sl)]);
-      bi[q] = unit(&st.b[col * UPR + ((lg * U + 2 + q) ^ sl)]);
-      bn[q] = mg_neg8(bi[q]);
+    for (int b = 0; b < 2; ++b) {
+      const int col = wn * 64 + (h + b) * 16 + li;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        br[b][q] = unit(&st.b[col * UPR + ((lg * U + q) ^ sl)]);
+        bi[b][q] = unit(&st.b[col * UPR + ((lg * U + 2 + q) ^ sl)]);
+        bn[b][q] = mg_neg8(bi[b][q]);
+      }
     }
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+    for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int q = 0; q + p < 2; ++q) {
 #pragma unroll
-        for (int q = 0; q + p < 2; ++q) {
-          cr[a][b] = mg_mfma_bf16(ar[a][p], br[q], cr[a][b]);
-          cr[a][b] = mg_mfma_bf16(ai[a][p], bn[q], cr[a][b]);
-          ci[a][b] = mg_mfma_bf16(ar[a][p], bi[q], ci[a][b]);
-          ci[a][b] = mg_mfma_bf16(ai[a][p], br[q], ci[a][b]);
-        }
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) cr[a][h + b] = mg_mfma_bf16(ar[a][p], br[b][q], cr[a][h + b]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) ci[a][h + b] = mg_mfma_bf16(ar[a][p], bi[b][q], ci[a][h + b]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) cr[a][h + b] = mg_mfma_bf16(ai[a][p], bn[b][q], cr[a][h + b]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) ci[a][h + b] = mg_mfma_bf16(ai[a][p], br[b][q], ci[a][h + b]);
+      }
   }
 }
 

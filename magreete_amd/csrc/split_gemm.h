@@ -228,17 +228,45 @@ struct Tile {
         afn[a][q] = mg_neg8(afi[a][q]);
       }
     }
+    // Term-major over pairs of column blocks (BF16x3; BF16x6 holds one block's
+    // six B fragments, pairs would spill): every product term goes over the
+    // group's accumulators before the next term, so consecutive MFMAs update
+    // different accumulators (each accumulator still takes its terms in
+    // cmac's order: the same bits as block by block).
+    constexpr int HB = (TN % 2 == 0 && NS == 2) ? 2 : 1;
 #pragma unroll
-    for (int b = 0; b < TN; ++b) {
-      const int col = wn * (BN / WN) + b * 16 + li;
-      bf16x8 bfr[NS], bfi[NS];
+    for (int h = 0; h < TN; h += HB) {
+      bf16x8 bfr[HB][NS], bfi[HB][NS];
 #pragma unroll
-      for (int q = 0; q < NS; ++q) {
-        bfr[q] = mg_ld8(&s.b[q][col][lg * 8]);
-        bfi[q] = mg_ld8(&s.b[NS + q][col][lg * 8]);
+      for (int b = 0; b < HB; ++b) {
+        const int col = wn * (BN / WN) + (h + b) * 16 + li;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          bfr[b][q] = mg_ld8(&s.b[q][col][lg * 8]);
+          bfi[b][q] = mg_ld8(&s.b[NS + q][col][lg * 8]);
+        }
       }
 #pragma unroll
-      for (int a = 0; a < TM; ++a) cmac<NS>(afr[a], afi[a], afn[a], bfr, bfi, cr[a][b], ci[a][b]);
+      for (int p = 0; p < NS; ++p)
+#pragma unroll
+        for (int q = 0; q + p < NS; ++q) {
+#pragma unroll
+          for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < HB; ++b) cr[a][h + b] = mg_mfma_bf16(afr[a][p], bfr[b][q], cr[a][h + b]);
+#pragma unroll
+          for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < HB; ++b) cr[a][h + b] = mg_mfma_bf16(afn[a][p], bfi[b][q], cr[a][h + b]);
+#pragma unroll
+          for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < HB; ++b) ci[a][h + b] = mg_mfma_bf16(afr[a][p], bfi[b][q], ci[a][h + b]);
+#pragma unroll
+          for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < HB; ++b) ci[a][h + b] = mg_mfma_bf16(afi[a][p], bfr[b][q], ci[a][h + b]);
+        }
     }
   }
 
