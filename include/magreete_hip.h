@@ -270,7 +270,9 @@ int mg_relaxed_product(int prec, int64_t m, int64_t n, int64_t k, const void* AS
  * every iteration factors one matrix in torch.linalg.solve
  * (Transmission3D.py:205, Transmission2D.py:192).  Per problem the factors,
  * pivots (0-based) and info (first zero pivot, 1-based) are getrf's, with
- * the same pivoting rule.
+ * the same pivoting rule (MG_BF16X3 groups with tournament leaves, the
+ * default: each leaf's pivots are those of partial pivoting among the rows
+ * that chunks of 256 rows offer, ipiv keeping getrf's meaning).
  * mg_group_create: prec MG_C128, MG_C64 or MG_BF16X3 (complex float, trailing
  * updates on BF16x3 products); host arrays of `count` device pointers and
  * sizes (A[i]: n[i] x n[i], lda[i]; ipiv[i]: n[i] ints; info[i]: one int;
@@ -279,7 +281,8 @@ int mg_relaxed_product(int prec, int64_t m, int64_t n, int64_t k, const void* AS
  * caller's and must outlive it.
  * mg_group_configure: panel width nb (multiple of 16, <= 1024), leaf width
  * (<= 16), trailing-update tile (0: 128 x 64, 1: 64 x 64), leaf workgroup
- * threads (256, 512, 1024).  mg_group_tune: the same tunables chosen by
+ * threads (256, 512, 1024), pivot (MG_BF16X3 leaves: 1 tournament, 0
+ * partial pivoting as getrf; other precisions always pivot as getrf).  mg_group_tune: the same tunables chosen by
  * timing candidates on copies of a sample of the group (call before the
  * factorisation), cached per device, precision and size class; cfg (4 ints)
  * receives them, best_ms the sample's time (0 for a cached choice).
@@ -293,7 +296,7 @@ typedef struct mg_group mg_group;
 int mg_group_create(int prec, int count, void* const* A, const int64_t* n, const int64_t* lda, int* const* ipiv,
                     int* const* info, void* const* B, const int64_t* ldb, int64_t nrhs, mg_group** out);
 int mg_group_destroy(mg_group* g);
-int mg_group_configure(mg_group* g, int nb, int leaf, int tile, int leaf_threads);
+int mg_group_configure(mg_group* g, int nb, int leaf, int tile, int leaf_threads, int pivot);
 int mg_group_tune(mg_group* g, int* cfg, float* best_ms, void* stream);
 int mg_group_getrf(mg_group* g, void* stream);
 int mg_group_getrs(mg_group* g, void* stream);

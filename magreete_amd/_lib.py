@@ -75,7 +75,7 @@ SIGNATURES = {
                          _dbl, _vp, _i64, _vp, _i64, _vp],
     "mg_group_create": [_int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _c.POINTER(_vp)],
     "mg_group_destroy": [_vp],
-    "mg_group_configure": [_vp, _int, _int, _int, _int],
+    "mg_group_configure": [_vp, _int, _int, _int, _int, _int],
     "mg_group_getrf": [_vp, _vp],
     "mg_group_getrs": [_vp, _vp],
     "mg_group_launches": [_vp],

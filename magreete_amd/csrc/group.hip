@@ -32,6 +32,8 @@
 #include "cgemm.h"
 #include "lu.h"
 #include "mg_internal.h"
+#include "dma_update.h"
+#include "psplit.h"
 #include "split_gemm.h"
 #include <algorithm>
 #include <climits>
@@ -56,6 +58,8 @@ struct GrpProb {
   void* X;        // auxiliary n x nrhs block: the solution of the residual product B -= A X
   int64_t ldx;
   int64_t idx;    // the caller's index of this problem
+  void* SA[2];    // BF16x3: the pre-split A operand (L21) of even / odd steps
+  void* SB;       // BF16x3: the pre-split B operand (U12) of the current step
 };
 
 // Workgroups of one grouped launch: problem p < np owns [first[p], first[p+1]).
@@ -70,6 +74,7 @@ struct GrpCfg {
   int leaf = 16;     // leaf width, <= kGLeaf
   int tile = 0;      // trailing-update tile: 0 = 128 x 64 (three-product), 1 = 64 x 64
   int lthreads = 1024; // threads of a leaf workgroup (the leaf streams its rows: more lanes, more loads in flight)
+  int pivot = 1;       // BF16x3 leaves: 1 = tournament pivoting, 0 = partial pivoting (LAPACK's pivots)
 };
 
 }  // namespace mg
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(256) void k_grp_laswp(const GrpProb* __restrict__ t
   const int64_t c = op.c0 + (int64_t)(blockIdx.x - map.first[p]) * 256 + threadIdx.x;
   const int64_t c1 = op.rhs ? op.c1 : (op.c1 < pb.n ? op.c1 : pb.n);
   if (c >= c1) return;
-  T2* A = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A);
+  gptr<T2> A = (gptr<T2>)reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A);
   const int64_t ld = op.rhs ? pb.ldb : pb.lda;
   const int64_t k2 = op.k2 < pb.n ? op.k2 : pb.n;
   for (int64_t i = op.k1; i < k2; ++i) {
@@ -411,7 +416,7 @@ __global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ ta
   const int p = grp_find(map, blockIdx.x);
   const GrpProb pb = tab[p];
   const int wp = (int)(op.w < pb.n - op.r0 ? op.w : pb.n - op.r0);
-  const T2* L = reinterpret_cast<const T2*>(pb.A) + op.r0 * pb.lda + op.r0;
+  gcptr<T2> L = (gcptr<T2>)(reinterpret_cast<const T2*>(pb.A) + op.r0 * pb.lda + op.r0);
   for (int e = threadIdx.x; e < wp * wp; e += blockDim.x) Ts[e / wp][e % wp] = L[(e / wp) * pb.lda + e % wp];
   __syncthreads();
   if (UPPER && threadIdx.x < wp) dinv[threadIdx.x] = crecip(Ts[threadIdx.x][threadIdx.x]);
@@ -419,7 +424,7 @@ __global__ __launch_bounds__(256) void k_grp_trsm(const GrpProb* __restrict__ ta
   const int lane = threadIdx.x & 63;
   const int64_t c1 = op.rhs ? op.c1 : (op.c1 < pb.n ? op.c1 : pb.n);
   const int64_t ldx = op.rhs ? pb.ldb : pb.lda;
-  T2* X = reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A) + op.r0 * ldx;
+  gptr<T2> X = (gptr<T2>)(reinterpret_cast<T2*>(op.rhs ? pb.B : pb.A) + op.r0 * ldx);
   const int64_t cw = op.c0 + (int64_t)(blockIdx.x - map.first[p]) * kTriWgCols;
   for (int ch = 0; ch < kTriChunks; ++ch) {
   const int64_t cb = cw + (int64_t)ch * 4 * kTriCols + (threadIdx.x >> 6) * kTriCols;
@@ -578,6 +583,343 @@ __global__ __launch_bounds__(256) void k_grp_colnorm(const GrpProb* __restrict__
                                   part[3][threadIdx.x]);
 }
 
+// ---- BF16x3 trailing update from pre-split operands -------------------------
+// Step k's update A22 -= L21 U12 (panel width nb = 32 T) reads its operands
+// pre-split (psplit.h's format, BF16x3: four 16-byte units per row or column
+// and group of 8 k): L21 (rows [k + nb, n) of columns [k, k + nb)) as A
+// operand in SA[parity of the step], U12 (columns [k + nb, n) of rows [k, k +
+// nb)) as B operand in SB.  Rows and columns past n - k - nb up to the next
+// multiple of 128 hold zeros, so the update's 128 x 128 tiles (dma_update.h)
+// read them unchecked.  L21 alternates between two buffers because step
+// k + nb's panel is split on the look-ahead stream while step k's update
+// still reads step k's.
+
+__device__ inline int64_t pad128(int64_t v) { return (v + 127) / 128 * 128; }
+
+// SA[par] of step k for every problem with rows below the step: thread t
+// splits group t mod nb/8 of row t / (nb/8) (8 consecutive elements).
+__global__ __launch_bounds__(256) void k_grp_pack_l(const GrpProb* __restrict__ tab, GrpMap map, int64_t k, int nb,
+                                                    int par) {
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int kg = nb / 8;
+  const int64_t r0 = k + nb, M = pb.n - r0;
+  const int64_t t = (int64_t)(blockIdx.x - map.first[p]) * 256 + threadIdx.x;
+  if (t >= pad128(M) * kg) return;
+  const int64_t r = t / kg, g = t - r * kg;
+  gcptr<float2> A = (gcptr<float2>)(reinterpret_cast<const float2*>(pb.A) + (r0 + r) * pb.lda + k + 8 * g);
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = r < M ? (float2)A[e] : make_float2(0.f, 0.f);
+  u32x4 u[4];
+  mg_ps::split8<2>(v, u);
+  gptr<u32x4> out = (gptr<u32x4>)(reinterpret_cast<u32x4*>(par ? pb.SA[1] : pb.SA[0]) + t * 4);   // (a select: no indexed copy of pb)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = u[q];
+}
+
+// SB of step k: thread t splits column t mod pad128(N) over group t / pad128(N)
+// (consecutive threads read consecutive columns).
+__global__ __launch_bounds__(256) void k_grp_pack_u(const GrpProb* __restrict__ tab, GrpMap map, int64_t k, int nb) {
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int kg = nb / 8;
+  const int64_t c0 = k + nb, N = pb.n - c0, Np = pad128(N);
+  const int64_t t = (int64_t)(blockIdx.x - map.first[p]) * 256 + threadIdx.x;
+  if (t >= Np * kg) return;
+  const int64_t g = t / Np, c = t - g * Np;
+  gcptr<float2> B = (gcptr<float2>)(reinterpret_cast<const float2*>(pb.A) + (k + 8 * g) * pb.lda + c0 + c);
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = c < N ? (float2)B[e * pb.lda] : make_float2(0.f, 0.f);
+  u32x4 u[4];
+  mg_ps::split8<2>(v, u);
+  gptr<u32x4> out = (gptr<u32x4>)(reinterpret_cast<u32x4*>(pb.SB) + (c * kg + g) * 4);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = u[q];
+}
+
+// A22 -= L21 U12 of step k over columns [c0, c1) of the trailing matrix
+// (counted from k + nb, c1 clamped to its order; c0 a multiple of 128):
+// problem p's 128 x 128 tiles walked in rows of 8 tiles, column by column, as
+// k_grp_gemm's.
+template <int T>
+__global__ __launch_bounds__(mg_dma::NT) void k_grp_update_dma(const GrpProb* __restrict__ tab, GrpMap map, int64_t k,
+                                                              int64_t c0, int64_t c1, int par) {
+  __shared__ mg_dma::Stage s0, s1;
+  constexpr int KG = 4 * T;   // groups of 8 k: nb = 32 T
+  const int id = xcd_local(blockIdx.x, gridDim.x);
+  const int p = grp_find(map, id);
+  const GrpProb pb = tab[p];
+  const int64_t r0 = k + 8 * KG, M = pb.n - r0;
+  const int64_t cend = c1 < M ? c1 : M;
+  const int nbm = (int)((M + 127) / 128), nbn = (int)((cend - c0 + 127) / 128);
+  const int local = id - map.first[p];
+  const int in_group = 8 * nbn, g = local / in_group, first_m = g * 8;
+  const int gsize = min(nbm - first_m, 8);
+  const int tm = first_m + (local % in_group) % gsize, tn = (local % in_group) / gsize;
+  const int64_t i0 = (int64_t)tm * 128, j0 = c0 + (int64_t)tn * 128;
+  const u32x4* A = reinterpret_cast<const u32x4*>(par ? pb.SA[1] : pb.SA[0]) + i0 * KG * mg_dma::U;
+  const u32x4* B = reinterpret_cast<const u32x4*>(pb.SB) + j0 * KG * mg_dma::U;
+  float2* C = reinterpret_cast<float2*>(pb.A) + (r0 + i0) * pb.lda + r0 + j0;
+  auto tile = [&](int t, mg_dma::Stage& st) { mg_dma::load_tile(st, A, B, KG, 4 * t); };
+  mg_dma::tile_update<T>(s0, s1, tile, C, pb.lda, M - i0, cend - j0);
+}
+
+// ---- tournament-pivoting leaf (BF16x3 groups) --------------------------------
+// The complex-float factors of a BF16x3 group only precondition an FP64
+// refinement, so their leaves may pivot by tournament (communication-avoiding
+// LU, Grigori, Demmel and Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)): the
+// leaf's rows [k, n) are cut into chunks of kTsRows; each chunk is eliminated
+// with partial pivoting on its own and offers the w rows it pivoted on
+// (k_grp_ts_local); one workgroup per problem eliminates the offered rows
+// (their original values) with partial pivoting, which selects the w pivot
+// rows in order and gives L11 \ U11, and moves the rows (k_grp_ts_final);
+// every other row is then solved against U11 (k_grp_ts_apply).  Three
+// launches of many short workgroups instead of one workgroup per problem that
+// passes over every row once per column.  The pivots are those of partial
+// pivoting on the offered rows, not LAPACK's; ipiv keeps LAPACK's meaning
+// (row k + j swapped with row ipiv[k + j], in order).
+constexpr int kTsRows = 256;     // rows per chunk
+constexpr int kTsChunks = 64;    // chunks per problem at most: leaves of at most 16384 rows
+
+// Workgroup arg-max of (v, r) over 256 threads (larger v first, then smaller
+// r); every thread gets the result.
+__device__ inline void ts_argmax(float& v, int& r, float* sv, int* sr) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(v, off);
+    const int orr = __shfl_xor(r, off);
+    if (ov > v || (ov == v && orr < r)) {
+      v = ov;
+      r = orr;
+    }
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sv[wave] = v;
+    sr[wave] = r;
+  }
+  __syncthreads();
+  v = sv[0];
+  r = sr[0];
+#pragma unroll
+  for (int q = 1; q < 4; ++q)
+    if (sv[q] > v || (sv[q] == v && sr[q] < r)) {
+      v = sv[q];
+      r = sr[q];
+    }
+}
+
+__device__ inline float cabs1(float2 a) { return fabsf(a.x) + fabsf(a.y); }
+
+// Partial-pivoting elimination of the first wp columns of rows [0, rows) of
+// t (row stride kGLeaf + 1), the rows' labels in lab swapped along; 256
+// threads, row i held by thread i mod 256.  On return rows [0, wp) hold the
+// pivot rows in order as L11 \ U11.
+template <int RPT>
+__device__ inline void ts_gepp(float2 (*t)[kGLeaf + 1], int* lab, int rows, int wp, float* sv, int* sr) {
+  const int tid = threadIdx.x;
+  for (int j = 0; j < wp; ++j) {
+    float v = -1.f;
+    int br = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = tid + 256 * q;
+      if (i >= j && i < rows && lab[i] >= 0) {
+        const float a = cabs1(t[i][j]);
+        if (a > v || (a == v && i < br)) {
+          v = a;
+          br = i;
+        }
+      }
+    }
+    ts_argmax(v, br, sv, sr);   // (its barrier also ends every thread's reads of column j)
+    const int p = br == INT_MAX ? j : br;
+    if (p != j) {
+      if (tid < wp) {
+        const float2 x = t[j][tid];
+        t[j][tid] = t[p][tid];
+        t[p][tid] = x;
+      }
+      if (tid == 0) {
+        const int x = lab[j];
+        lab[j] = lab[p];
+        lab[p] = x;
+      }
+    }
+    __syncthreads();
+    const float2 piv = t[j][j];
+    const bool nz = piv.x != 0.f || piv.y != 0.f;
+    const float2 inv = nz ? crecip(piv) : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = tid + 256 * q;
+      if (i > j && i < rows) {
+        float2 l = t[i][j];
+        if (nz) l = cmul(l, inv);
+        t[i][j] = l;
+        for (int c = j + 1; c < wp; ++c) {
+          const float2 u = t[j][c];
+          t[i][c].x -= l.x * u.x - l.y * u.y;
+          t[i][c].y -= l.x * u.y + l.y * u.x;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per chunk of kTsRows rows of the leaf rows [k, n) x columns
+// [k, k + wp): its wp pivot rows' indices (-1: past n) to cand[p][chunk][*]
+// (p: the problem's place in the group).
+__global__ __launch_bounds__(256) void k_grp_ts_local(const GrpProb* __restrict__ tab, GrpMap map, int64_t k, int w,
+                                                      int* __restrict__ cand) {
+  __shared__ float2 t[kTsRows][kGLeaf + 1];
+  __shared__ int lab[kTsRows];
+  __shared__ float sv[4];
+  __shared__ int sr[4];
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int chunk = blockIdx.x - map.first[p];
+  const int64_t m = pb.n - k, r0 = (int64_t)chunk * kTsRows;
+  const int wp = (int)(w < m ? w : m);
+  const int rows = (int)(m - r0 < kTsRows ? m - r0 : kTsRows);
+  gcptr<float2> A = (gcptr<float2>)(reinterpret_cast<const float2*>(pb.A) + (k + r0) * pb.lda + k);
+  for (int e = threadIdx.x; e < kTsRows * kGLeaf; e += 256) {
+    const int r = e / kGLeaf, c = e % kGLeaf;
+    t[r][c] = (r < rows && c < wp) ? (float2)A[(int64_t)r * pb.lda + c] : make_float2(0.f, 0.f);
+  }
+  lab[threadIdx.x] = threadIdx.x < rows ? (int)(k + r0) + threadIdx.x : -1;
+  __syncthreads();
+  ts_gepp<1>(t, lab, rows, wp, sv, sr);
+  if (threadIdx.x < wp) cand[((int64_t)p * kTsChunks + chunk) * kGLeaf + threadIdx.x] = lab[threadIdx.x];
+}
+
+// One workgroup per problem: partial-pivoting elimination of the offered
+// rows (original values), ipiv, info, the factored pivot rows to rows
+// [k, k + wp) and the rows they displace to the pivot rows' places, all in
+// columns [k, k + wp).
+__global__ __launch_bounds__(256) void k_grp_ts_final(const GrpProb* __restrict__ tab, int64_t k, int w,
+                                                      const int* __restrict__ cand) {
+  constexpr int kCand = kTsChunks * kGLeaf;
+  __shared__ float2 t[kCand][kGLeaf + 1];
+  __shared__ int lab[kCand];
+  __shared__ float2 top[kGLeaf][kGLeaf];
+  __shared__ int mpos[2 * kGLeaf], msrc[2 * kGLeaf];
+  __shared__ int nmove;
+  __shared__ float sv[4];
+  __shared__ int sr[4];
+  const int p = blockIdx.x;
+  const GrpProb pb = tab[p];
+  const int64_t m = pb.n - k;
+  const int wp = (int)(w < m ? w : m);
+  const int nch = (int)((m + kTsRows - 1) / kTsRows);
+  const int rows = nch * wp;
+  gptr<float2> A = (gptr<float2>)reinterpret_cast<float2*>(pb.A);
+  const int64_t lda = pb.lda;
+  for (int e = threadIdx.x; e < rows * kGLeaf; e += 256) {
+    const int r = e / kGLeaf, c = e % kGLeaf;
+    const int g = cand[((int64_t)p * kTsChunks + r / wp) * kGLeaf + r % wp];
+    t[r][c] = (g >= 0 && c < wp) ? A[(int64_t)g * lda + k + c] : make_float2(0.f, 0.f);
+    if (c == 0) lab[r] = g;
+  }
+  for (int e = threadIdx.x; e < wp * kGLeaf; e += 256) {
+    const int r = e / kGLeaf, c = e % kGLeaf;
+    top[r][c] = c < wp ? A[(k + r) * lda + k + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  ts_gepp<kCand / 256>(t, lab, rows, wp, sv, sr);
+  if (threadIdx.x == 0) {
+    // LAPACK's swap sequence that brings lab[0 .. wp) to rows k .. k + wp - 1:
+    // positions whose content changed, as (position, original row) pairs
+    int cnt = 0;
+    auto at = [&](int pos) {
+      for (int i = 0; i < cnt; ++i)
+        if (mpos[i] == pos) return msrc[i];
+      return pos;
+    };
+    auto put = [&](int pos, int row) {
+      for (int i = 0; i < cnt; ++i)
+        if (mpos[i] == pos) {
+          msrc[i] = row;
+          return;
+        }
+      mpos[cnt] = pos;
+      msrc[cnt] = row;
+      ++cnt;
+    };
+    bool zero = false;
+    for (int j = 0; j < wp; ++j) {
+      const int target = lab[j], kj = (int)k + j;
+      int q = target;
+      for (int i = 0; i < cnt; ++i)
+        if (msrc[i] == target) q = mpos[i];
+      pb.ipiv[kj] = q;
+      const int a = at(kj), b = at(q);
+      put(kj, b);
+      put(q, a);
+      if (!zero && t[j][j].x == 0.f && t[j][j].y == 0.f) {
+        zero = true;
+        atomicCAS(pb.info, 0, kj + 1);
+      }
+    }
+    nmove = cnt;
+  }
+  __syncthreads();
+  // the factored pivot rows, then the displaced original rows of [k, k + wp)
+  for (int e = threadIdx.x; e < wp * wp; e += 256) {
+    const int r = e / wp, c = e % wp;
+    A[(k + r) * lda + k + c] = t[r][c];
+  }
+  for (int e = threadIdx.x; e < nmove * wp; e += 256) {
+    const int i = e / wp, c = e % wp;
+    const int pos = mpos[i], src = msrc[i];
+    if (pos >= k + wp && src >= k && src < k + wp) A[(int64_t)pos * lda + k + c] = top[src - k][c];
+  }
+}
+
+// Rows [k + wp, n) x columns [k, k + wp) of every problem solved against
+// U11 (row by row: the eliminations of the pivot rows), one row per thread.
+__global__ __launch_bounds__(256) void k_grp_ts_apply(const GrpProb* __restrict__ tab, GrpMap map, int64_t k, int w) {
+  __shared__ float2 u[kGLeaf][kGLeaf + 1];
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int64_t m = pb.n - k;
+  const int wp = (int)(w < m ? w : m);
+  gptr<float2> A = (gptr<float2>)reinterpret_cast<float2*>(pb.A);
+  const int64_t lda = pb.lda;
+  for (int e = threadIdx.x; e < wp * kGLeaf; e += 256) {
+    const int r = e / kGLeaf, c = e % kGLeaf;
+    u[r][c] = c < wp ? A[(k + r) * lda + k + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int64_t r = k + wp + (int64_t)(blockIdx.x - map.first[p]) * 256 + threadIdx.x;
+  if (r >= pb.n) return;
+  float2 x[kGLeaf];
+  gptr<float2> row = A + r * lda + k;
+#pragma unroll
+  for (int c = 0; c < kGLeaf; ++c) x[c] = c < wp ? row[c] : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j = 0; j < kGLeaf; ++j) {
+    if (j < wp) {
+      const float2 piv = u[j][j];
+      float2 l = x[j];
+      if (piv.x != 0.f || piv.y != 0.f) l = cmul(l, crecip(piv));
+      x[j] = l;
+#pragma unroll
+      for (int c = j + 1; c < kGLeaf; ++c) {
+        const float2 v = u[j][c];
+        x[c].x -= l.x * v.x - l.y * v.y;
+        x[c].y -= l.x * v.y + l.y * v.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kGLeaf; ++c)
+    if (c < wp) row[c] = x[c];
+}
+
 __global__ void k_grp_zero_info(const GrpProb* __restrict__ tab, int P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < P) *tab[p].info = 0;
@@ -613,6 +955,9 @@ struct Group {
   hipStream_t scap = nullptr; // capture stream (default priority)
   hipEvent_t evA = nullptr, evB = nullptr;
   int64_t launches = 0;
+  void* split = nullptr;      // BF16x3: the pre-split operand buffers of every problem
+  int split_nb = 0;           // the panel width they were laid out for
+  int* cand = nullptr;        // BF16x3: the rows each chunk of a tournament leaf offers
 
   // problems with rows at or beyond row k (a prefix: orders decrease)
   int active(int64_t k) const {
@@ -647,6 +992,126 @@ int grp_getf2(Group& g, int64_t k, int w) {
     case 1024: hipLaunchKernelGGL((k_grp_getf2<T, 1024>), dim3(np), dim3(1024), 0, g.s, g.dtab, k, w); break;
     case 512: hipLaunchKernelGGL((k_grp_getf2<T, 512>), dim3(np), dim3(512), 0, g.s, g.dtab, k, w); break;
     default: hipLaunchKernelGGL((k_grp_getf2<T, 256>), dim3(np), dim3(256), 0, g.s, g.dtab, k, w); break;
+  }
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+// The BF16x3 trailing update from pre-split operands applies to panel widths
+// that are whole 128-column tiles with an even count of 32-deep k-tiles.
+bool dma_split(const Group& g) { return g.prec == MG_BF16X3 && (g.cfg.nb == 128 || g.cfg.nb == 256 || g.cfg.nb == 384); }
+
+// The pre-split operand buffers of a BF16x3 group: per problem SA[0], SA[1]
+// and SB of pad128(n) rows or columns of nb / 8 groups of four 16-byte units,
+// in one allocation, laid out for the current panel width and kept while it
+// is not wider.  Called before a factorisation or its capture (never inside
+// one); a new allocation waits for the device first, as work in flight may
+// still read the old one.
+int ensure_split(Group& g) {
+  if (g.prec != MG_BF16X3 || g.P == 0) return MG_OK;
+  if (!g.cand && hipMalloc(&g.cand, sizeof(int) * g.P * kTsChunks * kGLeaf) != hipSuccess) {
+    g.cand = nullptr;
+    set_error("grouped LU: tournament candidate table not allocated");
+    return MG_ERR_NOMEM;
+  }
+  if (!dma_split(g)) return MG_OK;
+  if (g.split && g.split_nb >= g.cfg.nb) return MG_OK;
+  if (g.split) {
+    MG_HIP(hipDeviceSynchronize());
+    MG_HIP(hipFree(g.split));
+    g.split = nullptr;
+    g.split_nb = 0;
+  }
+  const int64_t per = (int64_t)g.cfg.nb / 8 * 4 * 16;   // bytes per row or column
+  int64_t total = 0;
+  for (int p = 0; p < g.P; ++p) total += 3 * ((g.n[p] + 127) / 128 * 128) * per;
+  if (hipMalloc(&g.split, total) != hipSuccess) {
+    g.split = nullptr;
+    set_error("grouped LU: pre-split operand buffers (%lld bytes) not allocated", (long long)total);
+    return MG_ERR_NOMEM;
+  }
+  char* b = static_cast<char*>(g.split);
+  for (int p = 0; p < g.P; ++p) {
+    const int64_t sz = ((g.n[p] + 127) / 128 * 128) * per;
+    g.host[p].SA[0] = b;
+    g.host[p].SA[1] = b + sz;
+    g.host[p].SB = b + 2 * sz;
+    b += 3 * sz;
+  }
+  MG_HIP(hipMemcpy(g.dtab, g.host.data(), sizeof(GrpProb) * g.P, hipMemcpyHostToDevice));
+  g.split_nb = g.cfg.nb;
+  return MG_OK;
+}
+
+// Tournament-pivoting leaves apply to BF16x3 groups whose leaf has at most
+// kTsRows * kTsChunks rows.
+bool ts_leaf(const Group& g, int64_t k) {
+  return g.prec == MG_BF16X3 && g.cfg.pivot == 1 && g.cand && g.n[0] - k <= (int64_t)kTsRows * kTsChunks;
+}
+
+// Tournament-pivoting leaf over columns [k, k + w) of every problem.
+int grp_tslu(Group& g, int64_t k, int w) {
+  const int np = g.active(k);
+  if (np == 0) return MG_OK;
+  GrpMap m;
+  int total;
+  if (!g.make_map(np, [&](int p) -> int64_t { return (g.n[p] - k + kTsRows - 1) / kTsRows; }, m, total))
+    return MG_OK;
+  hipLaunchKernelGGL(k_grp_ts_local, dim3(total), dim3(256), 0, g.s, g.dtab, m, k, w, g.cand);
+  MG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_grp_ts_final, dim3(np), dim3(256), 0, g.s, g.dtab, k, w, g.cand);
+  MG_HIP(hipGetLastError());
+  g.launches += 2;
+  if (g.make_map(np, [&](int p) -> int64_t {
+        const int64_t m = g.n[p] - k, below = m - std::min<int64_t>(w, m);
+        return (below + 255) / 256;
+      }, m, total)) {
+    hipLaunchKernelGGL(k_grp_ts_apply, dim3(total), dim3(256), 0, g.s, g.dtab, m, k, w);
+    MG_HIP(hipGetLastError());
+    ++g.launches;
+  }
+  return MG_OK;
+}
+
+// Split L21 (lower) into SA[par] or U12 into SB for step k.
+int grp_pack(Group& g, int64_t k, bool lower, int par) {
+  const int nb = g.cfg.nb;
+  const int64_t kn = k + nb;
+  const int np = g.active(kn);
+  if (np == 0) return MG_OK;
+  GrpMap m;
+  int total;
+  if (!g.make_map(np, [&](int p) -> int64_t { return (((g.n[p] - kn) + 127) / 128 * 128 * (nb / 8) + 255) / 256; },
+                  m, total))
+    return MG_OK;
+  if (lower) hipLaunchKernelGGL(k_grp_pack_l, dim3(total), dim3(256), 0, g.s, g.dtab, m, k, nb, par);
+  else hipLaunchKernelGGL(k_grp_pack_u, dim3(total), dim3(256), 0, g.s, g.dtab, m, k, nb);
+  MG_HIP(hipGetLastError());
+  ++g.launches;
+  return MG_OK;
+}
+
+// A22 -= L21 U12 of step k from the pre-split operands, columns [c0, c1) of
+// the trailing matrix (counted from k + nb).
+int grp_update_dma(Group& g, int64_t k, int64_t c0, int64_t c1, int par) {
+  const int nb = g.cfg.nb;
+  const int64_t kn = k + nb;
+  const int np = g.active(kn);
+  if (np == 0) return MG_OK;
+  GrpMap m;
+  int total;
+  if (!g.make_map(np, [&](int p) -> int64_t {
+        const int64_t M = g.n[p] - kn, cend = std::min(c1, M);
+        if (M <= 0 || cend <= c0) return 0;
+        return ((M + 127) / 128) * ((cend - c0 + 127) / 128);
+      }, m, total))
+    return MG_OK;
+  switch (nb) {
+    case 128: hipLaunchKernelGGL(k_grp_update_dma<4>, dim3(total), dim3(mg_dma::NT), 0, g.s, g.dtab, m, k, c0, c1, par); break;
+    case 256: hipLaunchKernelGGL(k_grp_update_dma<8>, dim3(total), dim3(mg_dma::NT), 0, g.s, g.dtab, m, k, c0, c1, par); break;
+    case 384: hipLaunchKernelGGL(k_grp_update_dma<12>, dim3(total), dim3(mg_dma::NT), 0, g.s, g.dtab, m, k, c0, c1, par); break;
+    default: return MG_ERR_ARG;
   }
   MG_HIP(hipGetLastError());
   ++g.launches;
@@ -748,7 +1213,11 @@ int grp_trsm(Group& g, bool upper, int64_t r0, int64_t w, int64_t c0, int64_t c1
 // Panel: columns [k, k + w) of every problem, rows [k, n).
 template <typename T>
 int grp_panel(Group& g, int64_t k, int64_t w) {
-  if (w <= g.cfg.leaf) return grp_getf2<T>(g, k, (int)w);
+  if (w <= g.cfg.leaf) {
+    if constexpr (std::is_same<T, float>::value)
+      if (ts_leaf(g, k)) return grp_tslu(g, k, (int)w);
+    return grp_getf2<T>(g, k, (int)w);
+  }
   const int64_t w1 = split_half(w);
   MG_TRY(grp_panel<T>(g, k, w1));
   MG_TRY(grp_laswp<T>(g, k, k + w1, k + w1, k + w, false));
@@ -769,24 +1238,38 @@ int grp_getrf(Group& g, bool split) {
   hipLaunchKernelGGL(k_grp_zero_info, dim3((g.P + 255) / 256), dim3(256), 0, s, g.dtab, g.P);
   MG_HIP(hipGetLastError());
   const int64_t nb = g.cfg.nb;
+  // BF16x3 with whole 128-column tiles: the updates read L21 and U12 split
+  // once per step (the buffers are set up by the caller, outside any capture)
+  const bool pre = split && dma_split(g);
+  if (pre && !g.split) return MG_ERR_ARG;
   MG_TRY(grp_panel<T>(g, 0, nb));
+  if (pre) MG_TRY(grp_pack(g, 0, true, 0));
   for (int64_t k = 0; k < nmax; k += nb) {
     const int64_t kn = k + nb;
+    const int par = (int)((k / nb) & 1);
     if (kn < nmax) {
       MG_TRY(grp_laswp<T>(g, k, kn, kn, INT64_MAX, false));
       MG_TRY(grp_trsm<T>(g, false, k, nb, kn, INT64_MAX, false));
-      MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn, kn + nb, k, kn, 0}, split, g.cfg.tile));
+      if (pre) {
+        MG_TRY(grp_pack(g, k, false, par));
+        MG_TRY(grp_update_dma(g, k, 0, nb, par));
+      } else {
+        MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn, kn + nb, k, kn, 0}, split, g.cfg.tile));
+      }
       MG_HIP(hipEventRecord(g.evA, s));
       MG_HIP(hipStreamWaitEvent(g.s2, g.evA, 0));
       g.s = g.s2;
       int st = grp_panel<T>(g, kn, nb);
+      // the next step's L21, split while this step's update reads the other buffer
+      if (st == MG_OK && pre) st = grp_pack(g, kn, true, par ^ 1);
       // this step's swaps on the columns left of it (disjoint from the
       // panel and from the rest of the update) ride on the panel's stream
       if (st == MG_OK && k > 0) st = grp_laswp<T>(g, k, kn, 0, k, false);
       g.s = s;
       MG_TRY(st);
       MG_HIP(hipEventRecord(g.evB, g.s2));
-      MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn + nb, INT64_MAX, k, kn, 0}, split, g.cfg.tile));
+      if (pre) MG_TRY(grp_update_dma(g, k, nb, INT64_MAX, par));
+      else MG_TRY(grp_gemm<T>(g, GrpGemm{kn, INT64_MAX, kn + nb, INT64_MAX, k, kn, 0}, split, g.cfg.tile));
       MG_HIP(hipStreamWaitEvent(s, g.evB, 0));
     } else if (k > 0) {
       MG_TRY(grp_laswp<T>(g, k, kn, 0, k, false));
@@ -852,10 +1335,11 @@ constexpr int kTuneProblems = 32;
 constexpr int64_t kTuneBytes = 12ll << 30;
 
 struct TuneKey {
-  int dev, prec, nlog;
+  int dev, prec, nlog, pivot;
   bool operator<(const TuneKey& o) const {
     if (dev != o.dev) return dev < o.dev;
     if (prec != o.prec) return prec < o.prec;
+    if (pivot != o.pivot) return pivot < o.pivot;
     return nlog < o.nlog;
   }
 };
@@ -872,7 +1356,7 @@ int ceil_log2(int64_t v) {
 TuneKey tune_key(const Group& g) {
   int dev = 0;
   (void)hipGetDevice(&dev);
-  return TuneKey{dev, g.prec, ceil_log2(g.P ? g.n[0] : 1)};
+  return TuneKey{dev, g.prec, ceil_log2(g.P ? g.n[0] : 1), g.prec == MG_BF16X3 ? g.cfg.pivot : 0};
 }
 
 // panel widths around the measured best (256 for the 3D ensemble's orders
@@ -946,6 +1430,8 @@ int tune(Group& g, hipStream_t s, float* best_ms) {
   const int nc = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
   for (int c = -1; c < nc && st == MG_OK; ++c) {   // c = -1: untimed warm-up
     t.cfg = kCandidates[c < 0 ? 0 : c];
+    t.cfg.pivot = g.cfg.pivot;
+    if ((st = ensure_split(t)) != MG_OK) break;
     if ((st = refill()) != MG_OK) break;
     if (hipEventRecord(e0, s) != hipSuccess) { st = MG_ERR_HIP; break; }
     if ((st = run_factor(t)) != MG_OK) break;
@@ -960,6 +1446,8 @@ int tune(Group& g, hipStream_t s, float* best_ms) {
   (void)hipStreamSynchronize(s);
   (void)hipStreamSynchronize(g.s2);
   for (void* b : bufs) (void)hipFree(b);
+  if (t.split) (void)hipFree(t.split);
+  if (t.cand) (void)hipFree(t.cand);
   (void)hipFree(ints);
   if (t.dtab) (void)hipFree(t.dtab);
   if (e0) (void)hipEventDestroy(e0);
@@ -983,6 +1471,7 @@ int capture(Group& g, int op) {
     (void)hipGraphDestroy(g.graph[op]);
     g.graph[op] = nullptr;
   }
+  if (op == 0) MG_TRY(ensure_split(g));   // allocates: not inside the capture
   MG_HIP(hipStreamBeginCapture(g.scap, hipStreamCaptureModeThreadLocal));
   g.s = g.scap;
   const int st = op == 0 ? run_factor(g) : run_solve(g);
@@ -1072,6 +1561,8 @@ int mg_group_destroy(mg_group* h) {
   }
   if (h->g.scap) (void)hipStreamDestroy(h->g.scap);
   if (h->g.dtab) (void)hipFree(h->g.dtab);
+  if (h->g.split) (void)hipFree(h->g.split);
+  if (h->g.cand) (void)hipFree(h->g.cand);
   if (h->g.evA) (void)hipEventDestroy(h->g.evA);
   if (h->g.evB) (void)hipEventDestroy(h->g.evB);
   if (h->g.s2) (void)hipStreamDestroy(h->g.s2);
@@ -1079,20 +1570,22 @@ int mg_group_destroy(mg_group* h) {
   return MG_OK;
 }
 
-int mg_group_configure(mg_group* h, int nb, int leaf, int tile, int leaf_threads) {
+int mg_group_configure(mg_group* h, int nb, int leaf, int tile, int leaf_threads, int pivot) {
   if (!h) return MG_ERR_ARG;
   if (nb < 16 || nb > 1024 || nb % 16 != 0 || leaf < 1 || leaf > kGLeaf || (tile != 0 && tile != 1) ||
-      (leaf_threads != 256 && leaf_threads != 512 && leaf_threads != 1024))
+      (leaf_threads != 256 && leaf_threads != 512 && leaf_threads != 1024) || (pivot != 0 && pivot != 1))
     return MG_ERR_ARG;
   h->g.cfg.nb = nb;
   h->g.cfg.leaf = leaf;
   h->g.cfg.tile = tile;
   h->g.cfg.lthreads = leaf_threads;
+  h->g.cfg.pivot = pivot;
   return MG_OK;
 }
 
 int mg_group_getrf(mg_group* h, void* stream) {
   if (!h) return MG_ERR_ARG;
+  MG_TRY(mg::ensure_split(h->g));
   h->g.s = reinterpret_cast<hipStream_t>(stream);
   return mg::run_factor(h->g);
 }

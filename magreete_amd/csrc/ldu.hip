@@ -24,6 +24,7 @@
 // falls back to a pivoted factorisation when they are not good enough.
 #include "bf16x3.h"
 #include "cgemm.h"
+#include "dma_update.h"
 #include "lu.h"
 #include "mg_internal.h"
 #include "psplit.h"
@@ -474,182 +475,34 @@ int launch_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ld
 }
 
 // ---- BF16x3 trailing update for large C: operand tiles copied to LDS by DMA --
-// 128 x 128 tiles, 8 waves of 32 x 64 (the same per-wave MFMA work and
-// accumulation order as k_ldu_update, hence the same bits).  Each k-tile of
-// both operands (128 rows x 16 units per operand) goes from the pre-split
-// panels straight to LDS (global_load_lds_dwordx4: lane l writes 16 bytes at
-// the instruction's LDS base + 16 l), two stages, so no registers stage the
-// operands and no ds_write pass runs; in LDS the 16 units of a row are
-// XOR-swizzled by a permutation of the row's low bits (swz below) so that
-// the reads of one fragment hit distinct banks.  The accumulators start
-// from -C, loaded behind the first k-tile's operands.  The k-loop is fully
-// unrolled (NSRC steps of 8 k-tiles) so that the compiler can tell the two
-// stages' DMA apart and waits only where the loop asks for it.
-// Standalone, 12032^2 x 256: 1.03 ms against 1.19 for k_ldu_update; x 512:
-// 1.67 against 1.70 (the same bits).
+// 128 x 128 tiles (dma_update.h: the same per-wave MFMA work and accumulation
+// order as k_ldu_update, hence the same bits), the k-loop over NSRC steps of
+// 8 k-tiles.  Standalone, 12032^2 x 256: 1.03 ms against 1.19 for
+// k_ldu_update; x 512: 1.67 against 1.70 (the same bits).
 namespace dma {
-constexpr int BM = 128, BN = 128, WN = 2, NT = 512;
-constexpr int U = 4, UPR = 16, TM = 2, TN = 4;
-struct Stage {
-  u32x4 a[BM * UPR];
-  u32x4 b[BN * UPR];
-};
-typedef __attribute__((address_space(1))) const void* gvp;
-typedef __attribute__((address_space(3))) void* lvp;
-
-// LDS slot of 16-byte unit u in row r is u ^ swz(r).  The permutation
-// (2 r mod 16) ^ (r / 8 mod 2) of the row's low bits makes the 128-bit reads
-// of compute() (16 rows, one unit each, per 16-lane group) free of bank
-// conflicts on gfx950; the plain u ^ (r mod 16) gave a two-way conflict on
-// every read (LDS probe: SQ_LDS_BANK_CONFLICT equal to half the LDS-active
-// cycles, 237 us against 188 us for the same reads).
-__device__ inline int swz(int r) { return ((2 * r) & 15) ^ ((r >> 3) & 1); }
-
-// Rows [16 w, 16 w + 16) of both operands' k-tile (groups g0 .. g0 + 3) for
-// wave w: four DMA instructions per operand, each four whole rows.
-__device__ inline void load_tile(Stage& st, const u32x4* A, const u32x4* B, int64_t ldg, int64_t g0) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r0 = wave * 16 + i * 4;
-    const int r = r0 + (lane >> 4), u = (lane & 15) ^ swz(r);
-    __builtin_amdgcn_global_load_lds((gvp)(A + ((int64_t)r * ldg + g0) * U + u), (lvp)&st.a[r0 * UPR], 16, 0, 0);
-    __builtin_amdgcn_global_load_lds((gvp)(B + ((int64_t)r * ldg + g0) * U + u), (lvp)&st.b[r0 * UPR], 16, 0, 0);
-  }
-}
-
-__device__ inline bf16x8 unit(const u32x4* p) { return __builtin_bit_cast(bf16x8, *p); }
-
-// acc += the k-tile in st (BF16x3: part pairs (p, q) with p + q <= 1 of every
-// real product, the imaginary parts of B negated for the real accumulator).
-// Term-major: for each half of the wave's columns, every product term goes
-// over that half's eight accumulators before the next term, so consecutive
-// MFMAs update different accumulators (the block-by-block order chained six
-// dependent MFMAs per accumulator; standalone 12032^2 x 256: 0.97 ms against
-// 1.18, the same bits: every accumulator sees its terms in the same order).
-__device__ inline void compute(const Stage& st, v4f (&cr)[TM][TN], v4f (&ci)[TM][TN]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave / WN, wn = wave % WN, li = lane & 15, lg = lane >> 4;
-  const int sl = swz(li);   // rows and columns read here are 16 k + li
-  bf16x8 ar[TM][2], ai[TM][2];
-#pragma unroll
-  for (int a = 0; a < TM; ++a) {
-    const int row = wm * 32 + a * 16 + li;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      ar[a][q] = unit(&st.a[row * UPR + ((lg * U + q) ^ sl)]);
-      ai[a][q] = unit(&st.a[row * UPR + ((lg * U + 2 + q) ^ sl)]);
-    }
-  }
-#pragma unroll
-  for (int h = 0; h < TN; h += 2) {
-    bf16x8 br[2][2], bi[2][2], bn[2][2];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int col = wn * 64 + (h + b) * 16 + li;
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        br[b][q] = unit(&st.b[col * UPR + ((lg * U + q) ^ sl)]);
-        bi[b][q] = unit(&st.b[col * UPR + ((lg * U + 2 + q) ^ sl)]);
-        bn[b][q] = mg_neg8(bi[b][q]);
-      }
-    }
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int q = 0; q + p < 2; ++q) {
-#pragma unroll
-        for (int a = 0; a < TM; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) cr[a][h + b] = mg_mfma_bf16(ar[a][p], br[b][q], cr[a][h + b]);
-#pragma unroll
-        for (int a = 0; a < TM; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) ci[a][h + b] = mg_mfma_bf16(ar[a][p], bi[b][q], ci[a][h + b]);
-#pragma unroll
-        for (int a = 0; a < TM; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) cr[a][h + b] = mg_mfma_bf16(ai[a][p], bn[b][q], cr[a][h + b]);
-#pragma unroll
-        for (int a = 0; a < TM; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) ci[a][h + b] = mg_mfma_bf16(ai[a][p], br[b][q], ci[a][h + b]);
-      }
-  }
-}
-
-// s_waitcnt immediate that waits until at most v vector-memory operations
-// of this wave are outstanding (other counters left at their maximum).
-constexpr unsigned vm_wait(int v) { return (unsigned)((v & 0xF) | ((v >> 4) << 14) | (0x7 << 4) | (0xF << 8)); }
-constexpr unsigned kLdsWait = 0xC07F;   // lgkmcnt(0)
+using mg_dma::BM;
+using mg_dma::BN;
+using mg_dma::U;
 
 template <int NSRC>
-__global__ __launch_bounds__(NT) void k_ldu_update_dma(int64_t M, int64_t N, UpdSrc src, float2* __restrict__ C,
-                                                      int64_t ldc, int nbm, int nbn) {
-  __shared__ Stage s0, s1;
+__global__ __launch_bounds__(mg_dma::NT) void k_ldu_update_dma(int64_t M, int64_t N, UpdSrc src,
+                                                              float2* __restrict__ C, int64_t ldc, int nbm, int nbn) {
+  __shared__ mg_dma::Stage s0, s1;
   constexpr int KG = kIB / 8, T = NSRC * (KG / 4);
-  static_assert(T % 2 == 0, "k-tiles come in pairs (one per stage)");
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
   const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave / WN, wn = wave % WN;
-  auto tile = [&](int t, Stage& st) {
+  auto tile = [&](int t, mg_dma::Stage& st) {
     const int q = t / (KG / 4), g0 = (t % (KG / 4)) * 4;
-    load_tile(st, src.A[q] + i0 * KG * U, src.B[q] + j0 * KG * U, KG, g0);
+    mg_dma::load_tile(st, src.A[q] + i0 * KG * U, src.B[q] + j0 * KG * U, KG, g0);
   };
-  v4f cr[TM][TN], ci[TM][TN];
-  const bool whole = M - i0 >= BM && N - j0 >= BN;
-  const int64_t rbase = i0 + wm * 32 + (lane >> 4) * 4, cbase = j0 + wn * 64 + (lane & 15);
-  tile(0, s0);
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = rbase + a * 16 + r, col = cbase + b * 16;
-        float2 c = make_float2(0.f, 0.f);
-        if (whole || (row < M && col < N)) c = C[row * ldc + col];
-        cr[a][b][r] = -c.x;
-        ci[a][b][r] = -c.y;
-      }
-  tile(1, s1);
-  // bare s_barrier (no fence): the waits before it order the DMA and the reads
-#pragma unroll
-  for (int t = 0; t < T; t += 2) {
-    __builtin_amdgcn_s_waitcnt(vm_wait(8));     // k-tile t has landed (t + 1 may be in flight)
-    __builtin_amdgcn_s_barrier();
-    compute(s0, cr, ci);
-    __builtin_amdgcn_s_waitcnt(kLdsWait);       // this wave's reads of s0 are done
-    __builtin_amdgcn_s_barrier();
-    if (t + 2 < T) {
-      tile(t + 2, s0);
-      __builtin_amdgcn_s_waitcnt(vm_wait(8));   // k-tile t + 1 has landed
-    } else {
-      __builtin_amdgcn_s_waitcnt(vm_wait(0));
-    }
-    __builtin_amdgcn_s_barrier();
-    compute(s1, cr, ci);
-    __builtin_amdgcn_s_waitcnt(kLdsWait);
-    __builtin_amdgcn_s_barrier();
-    if (t + 3 < T) tile(t + 3, s1);
-  }
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = rbase + a * 16 + r, col = cbase + b * 16;
-        if (whole || (row < M && col < N)) C[row * ldc + col] = make_float2(-cr[a][b][r], -ci[a][b][r]);
-      }
+  mg_dma::tile_update<T>(s0, s1, tile, C + i0 * ldc + j0, ldc, M - i0, N - j0);
 }
 
 template <int NSRC>
 int launch_dma(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
   const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
-  hipLaunchKernelGGL(k_ldu_update_dma<NSRC>, dim3(nbm * nbn), dim3(NT), 0, s, M, N, src, C, ldc, nbm, nbn);
+  hipLaunchKernelGGL(k_ldu_update_dma<NSRC>, dim3(nbm * nbn), dim3(mg_dma::NT), 0, s, M, N, src, C, ldc, nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }

@@ -455,11 +455,15 @@ class GroupLU:
     ('c64', 'bf16x3': trailing updates on BF16x3 products), row-major,
     factored in place; rhs: optional (n_i, nrhs) tensors of the same dtype
     (one nrhs for all), solved in place by solve().  The factors, pivots and
-    info per problem follow getrf."""
+    info per problem follow getrf; pivot ('bf16x3' only): 'tournament' (the
+    default) picks each leaf's pivots by partial pivoting among the rows
+    that chunks of 256 rows offer, 'partial' as getrf."""
 
     MAX = 256
+    PIVOTS = {"partial": 0, "tournament": 1}
 
-    def __init__(self, mats, rhs=None, prec="c128", nb=256, leaf=16, tile=0, leaf_threads=1024):
+    def __init__(self, mats, rhs=None, prec="c128", nb=256, leaf=16, tile=0, leaf_threads=1024,
+                 pivot="tournament"):
         lib = L.load()
         if not 0 < len(mats) <= self.MAX:
             raise ValueError("a group holds 1 to %d problems" % self.MAX)
@@ -497,8 +501,10 @@ class GroupLU:
                                         ctypes.byref(h)), "mg_group_create")
         self.h = h
         self._captured = set()
-        L.check(lib.mg_group_configure(self.h, int(nb), int(leaf), int(tile), int(leaf_threads)),
-                "mg_group_configure")
+        if pivot not in self.PIVOTS:
+            raise ValueError("pivot: 'partial' or 'tournament'")
+        L.check(lib.mg_group_configure(self.h, int(nb), int(leaf), int(tile), int(leaf_threads),
+                                       self.PIVOTS[pivot]), "mg_group_configure")
 
     def ipiv(self, i):
         return self.ipiv_all[self.off[i]:self.off[i + 1]]

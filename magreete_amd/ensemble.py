@@ -130,9 +130,12 @@ class Ensemble:
     # -- grouped ------------------------------------------------------------
     def _groups(self, workloads):
         """Largest first, at most GroupLU.MAX problems and group_bytes of
-        matrices (complex128, plus the complex64 copies for bf16x3) per group."""
+        matrices (complex128, plus for bf16x3 the complex64 copies and the
+        pre-split update operands, three buffers of n x nb x 8 bytes at panel
+        widths nb up to 384) per group."""
         order = sorted(range(len(workloads)), key=lambda i: -cost(workloads[i]))
-        per = 16 + (8 if self.method == "bf16x3" else 0)
+        bf = self.method == "bf16x3"
+        per = 16 + (8 if bf else 0)
         cap = self.group_bytes
         if cap is None:
             cap = 0.5 * torch.cuda.get_device_properties(self.dev).total_memory
@@ -140,7 +143,7 @@ class Ensemble:
         for i in order:
             w = workloads[i]
             n = w.d * w.points.shape[0]
-            b = per * n * n
+            b = per * n * n + (3 * 384 * 8 * ((n + 127) // 128 * 128) if bf else 0)
             if cur and (len(cur) >= E.GroupLU.MAX or used + b > cap):
                 groups.append(cur)
                 cur, used = [], 0

@@ -108,11 +108,17 @@ def test_group_zero_pivot_info():
     g.close()
 
 
-@pytest.mark.parametrize("prec,tol", [("c64", 2e-4), ("bf16x3", 2e-3)])
-def test_group_complex_float(prec, tol):
-    """complex64 factors (the trailing updates on FP32 or BF16x3 products):
-    solutions within the precision of the factor of the complex128 ones."""
-    say("group", prec)
+@pytest.mark.parametrize("prec,tol,nb,pivot", [("c64", 2e-4, 256, "partial"), ("bf16x3", 2e-3, 256, "tournament"),
+                                               ("bf16x3", 2e-3, 128, "tournament"),
+                                               ("bf16x3", 2e-3, 384, "tournament"), ("bf16x3", 2e-3, 64, "tournament"),
+                                               ("bf16x3", 2e-3, 256, "partial")])
+def test_group_complex_float(prec, tol, nb, pivot):
+    """complex64 factors (the trailing updates on FP32 or BF16x3 products;
+    BF16x3 at panel widths 128, 256 and 384 from operands split once per
+    step, at 64 split in every tile; tournament or partial pivoting in the
+    leaves): solutions within the precision of the factor of the complex128
+    ones."""
+    say("group", prec, nb, pivot)
     host = _mats(SIZES, 5, dtype=np.complex64, shrink=1.0)
     rng = np.random.default_rng(6)
     rhs_host = [(rng.normal(size=(A.shape[0], 8)) + 1j * rng.normal(size=(A.shape[0], 8))).astype(np.complex64)
@@ -121,7 +127,7 @@ def test_group_complex_float(prec, tol):
         host[i] = (host[i] + 4 * np.sqrt(host[i].shape[0]) * np.eye(host[i].shape[0])).astype(np.complex64)
     mats = [torch.from_numpy(A).to(DEV) for A in host]
     rhs = [torch.from_numpy(B).to(DEV) for B in rhs_host]
-    g = E.GroupLU(mats, rhs, prec=prec)
+    g = E.GroupLU(mats, rhs, prec=prec, nb=nb, pivot=pivot)
     g.factor()
     g.solve()
     torch.cuda.synchronize()
@@ -155,3 +161,79 @@ def test_group_residual_and_colnorms():
             assert np.abs(got - ref).max() <= tol * np.abs(A).max() * A.shape[0], i
             assert np.allclose(nrm[i], np.linalg.norm(got.astype(np.complex128), axis=0), rtol=1e-12), i
         g.close()
+
+
+def _gepp_labels(M, labels, w):
+    """Partial pivoting (largest |re| + |im|, first on ties) on the first w
+    columns of M (complex128): the labels of the pivot rows, in order."""
+    M = M[:, :w].astype(np.complex128).copy()
+    labels = list(labels)
+    for j in range(min(w, M.shape[0])):
+        a = np.abs(M[j:, j].real) + np.abs(M[j:, j].imag)
+        p = j + int(np.argmax(a))
+        M[[j, p]] = M[[p, j]]
+        labels[j], labels[p] = labels[p], labels[j]
+        if M[j, j] != 0:
+            M[j + 1:, j] /= M[j, j]
+            M[j + 1:, j + 1:] -= np.outer(M[j + 1:, j], M[j, j + 1:])
+    return labels[:w]
+
+
+def _tournament_first_leaf(A, w=16, rows=256):
+    """The tournament rule on columns [0, w): every chunk of `rows` rows
+    offers the rows partial pivoting picks in it, then partial pivoting among
+    the offered rows (original values) picks the leaf's pivot rows in order;
+    returned as getrf's swap sequence."""
+    n = A.shape[0]
+    cand = []
+    for r0 in range(0, n, rows):
+        cand += _gepp_labels(A[r0:r0 + rows], range(r0, min(n, r0 + rows)), w)
+    sel = _gepp_labels(A[cand], cand, w)
+    cur = list(range(n))
+    ipiv = []
+    for j, row in enumerate(sel):
+        q = cur.index(row)
+        ipiv.append(q)
+        cur[j], cur[q] = cur[q], cur[j]
+    return np.array(ipiv)
+
+
+def test_group_tournament_leaf():
+    """BF16x3 tournament leaves: the first leaf's pivots equal the rule's
+    (restated in numpy on the same complex64 data), every pivot is a valid
+    swap, P A = L U to the factor's precision, and a zero column gives
+    getrf's info."""
+    say("group tournament")
+    host = _mats([2300, 700, 1029, 300, 17], 9, dtype=np.complex64, shrink=0.05)
+    errs = {}
+    for pivot in ("partial", "tournament"):
+        mats = [torch.from_numpy(A).to(DEV) for A in host]
+        g = E.GroupLU(mats, prec="bf16x3", pivot=pivot).factor()
+        torch.cuda.synchronize()
+        assert (g.info.cpu() == 0).all()
+        for i, A in enumerate(host):
+            n = A.shape[0]
+            ip = g.ipiv(i).cpu().numpy()
+            assert ((ip >= np.arange(n)) & (ip < n)).all(), i
+            if pivot == "tournament":
+                w = min(16, n)
+                assert np.array_equal(ip[:w], _tournament_first_leaf(A, w)[:w]), i
+            Lm, Um = _unpack(mats[i].cpu().numpy().astype(np.complex128))
+            PA = A.astype(np.complex128).copy()
+            for r, p in enumerate(ip):
+                PA[[r, p]] = PA[[p, r]]
+            errs[pivot, i] = np.abs(Lm @ Um - PA).max() / np.abs(A).max()
+            say(pivot, i, n, "|L| max %.2f" % np.abs(Lm).max(), "P A - L U %.2e" % errs[pivot, i])
+        g.close()
+    # BF16x3 products: ~2e-5 per product, ~sqrt(n) of them per entry; the
+    # tournament's factors as accurate as partial pivoting's
+    for i in range(len(host)):
+        assert errs["tournament", i] < 5e-3 and errs["tournament", i] <= 3 * errs["partial", i] + 1e-5, i
+    host = _mats([90, 200, 40], 4, dtype=np.complex64, shrink=1.0)
+    host[1][:, 77] = 0
+    mats = [torch.from_numpy(A).to(DEV) for A in host]
+    g = E.GroupLU(mats, prec="bf16x3", pivot="tournament").factor()
+    torch.cuda.synchronize()
+    info = g.info.cpu().numpy()
+    assert info[0] == 0 and info[2] == 0 and info[1] == 78, info
+    g.close()
