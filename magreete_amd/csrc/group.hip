@@ -60,6 +60,10 @@ struct GrpProb {
   int64_t idx;    // the caller's index of this problem
   void* SA[2];    // BF16x3: the pre-split A operand (L21) of even / odd steps
   void* SB;       // BF16x3: the pre-split B operand (U12) of the current step
+  int* perm;      // BF16x3 solves: position r of P A holds row perm[r] of A
+  void* Dinv;     // BF16x3 solves: the inverted 64 x 64 diagonal blocks of L and U
+  void* W1;       // BF16x3 solves: n x nrhs, the permuted right-hand sides (forward sweep in place)
+  void* W2;       // BF16x3 solves: n x nrhs, the forward sweep's result (backward sweep in place)
 };
 
 // Workgroups of one grouped launch: problem p < np owns [first[p], first[p+1]).
@@ -920,6 +924,364 @@ __global__ __launch_bounds__(256) void k_grp_ts_apply(const GrpProb* __restrict_
     if (c < wp) row[c] = x[c];
 }
 
+// ---- BF16x3 groups: solves from inverted diagonal blocks ---------------------
+// The factors of a BF16x3 group precondition an FP64 refinement, so their
+// solves may run on GEMMs only: the 64 x 64 diagonal blocks of L (unit) and U
+// are inverted once per factorisation (Dinv: per block, L^-1 then U^-1, 64 x
+// 64 row-major, zero past the block's order), and each sweep is one dataflow
+// launch over the blocks of every problem (k_grp_sweep): the workgroup of
+// block k (64 rows, 32 right-hand sides) forms W_k - sum_j T_kj X_j over the
+// blocks j it depends on (forward: j < k with T = L; backward: j > k with T
+// = U) as soon as each X_j is published, then X_k = D_k^-1 (that), publishes
+// it and signals.  Products on split-bf16 MFMA (BF16x3), 64 x 32 tiles of
+// 128 threads.
+constexpr int kSB = 64;   // solve block
+typedef mg_sg::Tile<2, kSB, 32, 2, 1> SolveTile;
+
+// perm from getrf's swap sequence, one workgroup per problem (the swaps
+// composed by one thread in LDS for n <= kPermLds, in memory beyond).
+constexpr int kPermLds = 16384;
+__global__ __launch_bounds__(256) void k_grp_perm(const GrpProb* __restrict__ tab) {
+  __shared__ int sp[kPermLds], sv[kPermLds];
+  const GrpProb pb = tab[blockIdx.x];
+  const int n = (int)pb.n;
+  gptr<int> perm = (gptr<int>)pb.perm;
+  gcptr<int> ipiv = (gcptr<int>)pb.ipiv;
+  if (n <= kPermLds) {
+    for (int r = threadIdx.x; r < n; r += 256) {
+      sp[r] = r;
+      sv[r] = ipiv[r];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < n; ++i) {
+        const int j = sv[i], t = sp[i];
+        sp[i] = sp[j];
+        sp[j] = t;
+      }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += 256) perm[r] = sp[r];
+  } else {
+    for (int r = threadIdx.x; r < n; r += 256) perm[r] = r;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < n; ++i) {
+        const int j = ipiv[i], t = perm[i];
+        perm[i] = perm[j];
+        perm[j] = t;
+      }
+  }
+}
+
+// Inverse of diagonal block b (order h = min(64, n - 64 b)) of L (unit
+// lower; workgroup 2 b of the problem) or U (upper; 2 b + 1), by columns:
+// lane c substitutes column c of the identity.  One wave per workgroup.
+__global__ __launch_bounds__(64) void k_grp_trinv(const GrpProb* __restrict__ tab, GrpMap map) {
+  __shared__ float2 t[kSB][kSB + 1], x[kSB][kSB + 1];
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int local = blockIdx.x - map.first[p], b = local >> 1;
+  const bool upper = local & 1;
+  const int64_t r0 = (int64_t)b * kSB;
+  const int h = (int)(pb.n - r0 < kSB ? pb.n - r0 : kSB);
+  gcptr<float2> A = (gcptr<float2>)(reinterpret_cast<const float2*>(pb.A) + r0 * pb.lda + r0);
+  const int c = threadIdx.x;
+  for (int r = 0; r < h; ++r) t[r][c] = c < h ? (float2)A[r * pb.lda + c] : make_float2(0.f, 0.f);
+  __syncthreads();
+  for (int r = 0; r < kSB; ++r) x[r][c] = make_float2(0.f, 0.f);
+  if (c < h) {
+    if (!upper) {
+      x[c][c] = make_float2(1.f, 0.f);
+      for (int r = c + 1; r < h; ++r) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int j = c; j < r; ++j) {
+          const float2 l = t[r][j], v = x[j][c];
+          acc.x -= l.x * v.x - l.y * v.y;
+          acc.y -= l.x * v.y + l.y * v.x;
+        }
+        x[r][c] = acc;
+      }
+    } else {
+      x[c][c] = crecip(t[c][c]);
+      for (int r = c - 1; r >= 0; --r) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int j = r + 1; j <= c; ++j) {
+          const float2 u = t[r][j], v = x[j][c];
+          acc.x -= u.x * v.x - u.y * v.y;
+          acc.y -= u.x * v.y + u.y * v.x;
+        }
+        x[r][c] = cmul(acc, crecip(t[r][r]));
+      }
+    }
+  }
+  __syncthreads();
+  gptr<float2> D = (gptr<float2>)(reinterpret_cast<float2*>(pb.Dinv) + ((int64_t)2 * b + (upper ? 1 : 0)) * kSB * kSB);
+  for (int r = 0; r < kSB; ++r) D[r * kSB + c] = x[r][c];
+}
+
+// W1 = P B (rows gathered through perm), 256 entries per workgroup.
+__global__ __launch_bounds__(256) void k_grp_gather(const GrpProb* __restrict__ tab, GrpMap map, int nrhs) {
+  const int p = grp_find(map, blockIdx.x);
+  const GrpProb pb = tab[p];
+  const int64_t e = (int64_t)(blockIdx.x - map.first[p]) * 256 + threadIdx.x;
+  if (e >= pb.n * nrhs) return;
+  const int64_t r = e / nrhs, c = e - r * nrhs;
+  gcptr<float2> B = (gcptr<float2>)reinterpret_cast<const float2*>(pb.B);
+  gptr<float2> W = (gptr<float2>)reinterpret_cast<float2*>(pb.W1);
+  W[r * nrhs + c] = B[(int64_t)((gcptr<int>)pb.perm)[r] * pb.ldb + c];
+}
+
+// Tickets of a sweep in dependency order: level s holds block s (forward) or
+// block nblk - 1 - s (backward) of every problem with more than s blocks
+// (a prefix of the group: orders decrease), nct column tiles each; level s
+// starts at ticket lev[s].  A workgroup only waits for blocks of lower levels,
+// whose tickets were taken before its own, so the sweep needs no
+// co-residency.
+// ctr (zero on entry): ctr[0] hands out tickets, ctr[1 + coff_p + k] counts
+// the finished column tiles of problem p's block k (coff_p = blocks of the
+// problems before p); next: the following sweep's counters, cleared here.
+// Published blocks are written with agent-scope stores and read with
+// agent-scope loads (coherent across XCDs), each wave's stores complete
+// before the barrier that precedes the signal.  A wait past the spin bound
+// publishes NaN, so every workgroup ends and the refinement sees the failure.
+constexpr long long kGrpSweepSpin = 1LL << 22;
+
+template <bool FWD>
+__global__ __launch_bounds__(128) void k_grp_sweep(const GrpProb* __restrict__ tab, const int* __restrict__ lev,
+                                                   int nlev, const int* __restrict__ coff, int nrhs, int* ctr,
+                                                   int* next, int cap) {
+  __shared__ SolveTile::S sm;
+  __shared__ int s_ticket, s_ok;
+  if (blockIdx.x == 0)
+    for (int e = threadIdx.x; e < cap; e += blockDim.x)
+      __hip_atomic_store(next + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int ticket = s_ticket;
+  int lo = 0, hi = nlev;   // the level: the last s with lev[s] <= ticket
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (lev[mid] <= ticket) lo = mid;
+    else hi = mid;
+  }
+  const int lvl = lo;
+  const int nct = (nrhs + 31) / 32;
+  const int p = (ticket - lev[lvl]) / nct, ct = (ticket - lev[lvl]) % nct;
+  const GrpProb pb = tab[p];
+  const int nblk = (int)((pb.n + kSB - 1) / kSB);
+  const int k = FWD ? lvl : nblk - 1 - lvl;
+  int* done = ctr + 1 + coff[p];
+  const int64_t c0 = (int64_t)ct * 32, N = nrhs - c0 < 32 ? nrhs - c0 : 32;
+  const int64_t rk = (int64_t)k * kSB, hk = pb.n - rk < kSB ? pb.n - rk : kSB;
+  float2* Wk = reinterpret_cast<float2*>(FWD ? pb.W1 : pb.W2) + rk * nrhs + c0;   // this block's input, in place
+  float2* X = reinterpret_cast<float2*>(FWD ? pb.W2 : pb.B);                       // the published blocks
+  const int64_t ldx = FWD ? nrhs : pb.ldb;
+  SolveTile t;
+  t.init(Wk, nrhs, hk, N, make_float2(-1.f, 0.f), true);   // -W_k
+  bool ok = true;
+  for (int q = 0; q < (FWD ? k : nblk - 1 - k); ++q) {
+    const int j = FWD ? q : nblk - 1 - q;
+    if (threadIdx.x == 0) {
+      long long spins = 0;
+      int st = 1;
+      while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nct) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kGrpSweepSpin) {
+          st = 0;
+          break;
+        }
+      }
+      s_ok = st;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (!s_ok) {
+      ok = false;
+      break;
+    }
+    // += T_kj X_j (rows of block k, columns of block j; X_j agent-scope loads)
+    const int64_t rj = (int64_t)j * kSB, hj = pb.n - rj < kSB ? pb.n - rj : kSB;
+    const float2* T = reinterpret_cast<const float2*>(pb.A) + rk * pb.lda + rj;
+    const float2* Xj = X + rj * ldx + c0;
+    t.template fetch<true>(T, pb.lda, Xj, ldx, hk, N, hj, 0);
+    for (int64_t k0 = 0; k0 < hj; k0 += mg_sg::BK) {
+      __syncthreads();
+      t.stash(sm);
+      __syncthreads();
+      if (k0 + mg_sg::BK < hj) t.template fetch<true>(T, pb.lda, Xj, ldx, hk, N, hj, k0 + mg_sg::BK);
+      t.compute(sm);
+    }
+  }
+  float2* Xk = X + rk * ldx + c0;
+  if (ok) {
+    // W_k' = -acc to this block's input rows (read back below by this workgroup only)
+    t.store(Wk, nrhs, hk, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const float2* Dk = reinterpret_cast<const float2*>(pb.Dinv) + ((int64_t)2 * k + (FWD ? 0 : 1)) * kSB * kSB;
+    SolveTile d;
+    d.init(nullptr, 0, 0, 0, make_float2(0.f, 0.f), false);
+    d.mac(sm, Dk, kSB, Wk, nrhs, hk, N, hk);
+    d.template store<true>(Xk, ldx, hk, N, make_float2(1.f, 0.f), make_float2(0.f, 0.f), false);
+  } else {
+    const float nan = __builtin_nanf("");
+    for (int e = threadIdx.x; e < hk * N; e += blockDim.x)
+      mg_sg::coherent_store(&Xk[(e / N) * ldx + e % N], make_float2(nan, nan));
+  }
+  // every wave's stores complete before the barrier after which the block's
+  // counter is bumped (a workgroup fence emits no such wait on gfx950)
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_grp_clear(int* c, int cap) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < cap; e += gridDim.x * blockDim.x)
+    __hip_atomic_store(c + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- FP64 residual of the refinement: B -= A X ------------------------------
+// Complex128 A (n x n) and X (n x nrhs, the auxiliary block), B the
+// right-hand sides.  One pass over A at streaming rate: a workgroup owns 128
+// rows (32 per wave: two 16-row MFMA fragments) and 32 right-hand sides and
+// walks k in chunks of 8 (two 16x16x4 FP64 MFMA steps; chunks of 16 spill
+// registers at two waves per SIMD); each lane's A and X fragments of the next
+// chunk are loaded into registers while the current chunk is multiplied,
+// straight from memory (every element of A is read once; X, n x 32, through
+// the caches).  Complex products in the
+// three-product form (t1 = Ar Xr, t2 = Ai Xi, t3 = (Ar + Ai)(Xr + Xi)).
+// Rows, k and columns past the problem's extents are loaded from clamped
+// addresses and zeroed or not stored.
+constexpr int kResRows = 128, kResCols = 32, kResSteps = 2;
+
+struct ResFrag {
+  double2 a[2][kResSteps];   // [row fragment][k-step]
+  double2 x[2][kResSteps];   // [column fragment][k-step]
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_grp_residual(const GrpProb* __restrict__ tab, GrpMap map, int nrhs) {
+  const int id = xcd_local(blockIdx.x, gridDim.x);
+  const int p = grp_find(map, id);
+  const GrpProb pb = tab[p];
+  const int64_t n = pb.n;
+  const int local = id - map.first[p];
+  const int nct = (nrhs + kResCols - 1) / kResCols;
+  const int rb = local / nct, ct = local - rb * nct;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int64_t r0 = (int64_t)rb * kResRows + wave * 32;
+  const int c0 = ct * kResCols;
+  gcptr<double2> A = (gcptr<double2>)reinterpret_cast<const double2*>(pb.A);
+  gcptr<double2> X = (gcptr<double2>)reinterpret_cast<const double2*>(pb.X);
+  // this lane's A entries: rows r0 + 16 f + li (clamped), k = k0 + 4 s + lk;
+  // X entries: k = k0 + 4 s + lk, columns c0 + 16 g + li (clamped)
+  gcptr<double2> pa[2];
+  gcptr<double2> px[2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    const int64_t r = r0 + 16 * f + li;
+    pa[f] = A + (r < n ? r : n - 1) * pb.lda + lk;
+    const int64_t c = c0 + 16 * f + li;
+    px[f] = X + (int64_t)lk * pb.ldx + (c < nrhs ? c : nrhs - 1);
+  }
+  const int64_t xstep = 4 * pb.ldx;
+  // full chunks: unpredicated loads at k0 (pointers advanced by the caller)
+  auto fetch = [&](ResFrag& fr) {
+#pragma unroll
+    for (int s = 0; s < kResSteps; ++s)
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        fr.a[f][s] = pa[f][4 * s];
+        fr.x[f][s] = px[f][s * xstep];
+      }
+  };
+  constexpr int64_t KC = 4 * kResSteps;
+  auto advance = [&]() {
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      pa[f] += KC;
+      px[f] += KC * pb.ldx;
+    }
+  };
+  v4d t1[2][2], t2[2][2], t3[2][2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) t1[f][g][r] = t2[f][g][r] = t3[f][g][r] = 0.0;
+  auto step = [&](const ResFrag& fr) {
+#pragma unroll
+    for (int s = 0; s < kResSteps; ++s) {
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        const double ar = fr.a[f][s].x, ai = fr.a[f][s].y, as = ar + ai;
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          const double xr = fr.x[g][s].x, xi = fr.x[g][s].y, xs = xr + xi;
+          t1[f][g] = mg_mfma(ar, xr, t1[f][g]);
+          t2[f][g] = mg_mfma(ai, xi, t2[f][g]);
+          t3[f][g] = mg_mfma(as, xs, t3[f][g]);
+        }
+      }
+    }
+  };
+  const int64_t nfull = n / KC;   // whole chunks; the rest (n mod KC) below
+  ResFrag b0, b1;
+  if (nfull > 0) {
+    fetch(b0);
+    advance();
+  }
+  // steady state with unconditional fetches (the compiler's waits then only
+  // cover the chunk being multiplied), then the last one or two chunks
+  int64_t c = 0;
+  for (; c + 2 < nfull; c += 2) {
+    fetch(b1);
+    advance();
+    step(b0);
+    fetch(b0);
+    advance();
+    step(b1);
+  }
+  if (c + 1 < nfull) {
+    fetch(b1);
+    advance();
+    step(b0);
+    step(b1);
+  } else if (c < nfull) {
+    step(b0);
+  }
+  if (nfull * KC < n) {   // the last partial chunk, k past n zeroed
+#pragma unroll
+    for (int s = 0; s < kResSteps; ++s) {
+      const bool ok = nfull * KC + 4 * s + lk < n;
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        b0.a[f][s] = ok ? (double2)pa[f][4 * s] : make_double2(0.0, 0.0);
+        b0.x[f][s] = ok ? (double2)px[f][s * xstep] : make_double2(0.0, 0.0);
+      }
+    }
+    step(b0);
+  }
+  // accumulator item r of lane l: row l / 16 + 4 r, column l % 16 (FP64 MFMA)
+  gptr<double2> B = (gptr<double2>)reinterpret_cast<double2*>(pb.B);
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = r0 + 16 * f + lk + 4 * r;
+        const int c = c0 + 16 * g + li;
+        if (row < n && c < nrhs) {
+          double2 v = B[row * pb.ldb + c];
+          v.x -= t1[f][g][r] - t2[f][g][r];
+          v.y -= t3[f][g][r] - t1[f][g][r] - t2[f][g][r];
+          B[row * pb.ldb + c] = v;
+        }
+      }
+}
+
 __global__ void k_grp_zero_info(const GrpProb* __restrict__ tab, int P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < P) *tab[p].info = 0;
@@ -958,6 +1320,11 @@ struct Group {
   void* split = nullptr;      // BF16x3: the pre-split operand buffers of every problem
   int split_nb = 0;           // the panel width they were laid out for
   int* cand = nullptr;        // BF16x3: the rows each chunk of a tournament leaf offers
+  void* solvebuf = nullptr;   // BF16x3 with right-hand sides: perm, Dinv, W1, W2 of every problem
+  int* sweep_lev = nullptr;   // BF16x3 solves: first ticket of each sweep level (device, in solvebuf)
+  int* sweep_coff = nullptr;  // BF16x3 solves: each problem's first block counter (device, in solvebuf)
+  int* sweep_ctr = nullptr;   // BF16x3 solves: two counter sets of sweep_cap ints (device, in solvebuf)
+  int sweep_nlev = 0, sweep_cap = 0, sweep_tickets = 0;
 
   // problems with rows at or beyond row k (a prefix: orders decrease)
   int active(int64_t k) const {
@@ -1014,6 +1381,65 @@ int ensure_split(Group& g) {
     g.cand = nullptr;
     set_error("grouped LU: tournament candidate table not allocated");
     return MG_ERR_NOMEM;
+  }
+  if (g.nrhs > 0 && !g.solvebuf) {   // the solve's buffers (sizes fixed at creation)
+    int64_t total = 0;
+    auto sizes = [&](int p, int64_t& perm, int64_t& dinv, int64_t& w) {
+      const int64_t nblk = (g.n[p] + kSB - 1) / kSB;
+      perm = (g.n[p] * (int64_t)sizeof(int) + 255) / 256 * 256;
+      dinv = 2 * nblk * kSB * kSB * (int64_t)sizeof(float2);
+      w = (g.n[p] * g.nrhs * (int64_t)sizeof(float2) + 255) / 256 * 256;
+    };
+    for (int p = 0; p < g.P; ++p) {
+      int64_t a, b, c;
+      sizes(p, a, b, c);
+      total += a + b + 2 * c;
+    }
+    // the sweeps' ticket levels, counter offsets and counters
+    const int nct = (int)((g.nrhs + 31) / 32);
+    const int nlev = (int)((g.n[0] + kSB - 1) / kSB);
+    std::vector<int> lev(nlev + 1, 0), coff(g.P, 0);
+    int64_t blocks = 0;
+    for (int p = 0; p < g.P; ++p) {
+      coff[p] = (int)blocks;
+      blocks += (g.n[p] + kSB - 1) / kSB;
+    }
+    for (int l = 0, np = g.P; l < nlev; ++l) {
+      while (np > 0 && (g.n[np - 1] + kSB - 1) / kSB <= l) --np;
+      lev[l + 1] = lev[l] + np * nct;
+    }
+    if (blocks + 1 > INT_MAX / 4 || (int64_t)lev[nlev] > INT_MAX / 2) {
+      set_error("grouped LU: too many solve blocks");
+      return MG_ERR_ARG;
+    }
+    const int cap = (int)blocks + 1;
+    const int64_t tabs = ((int64_t)(nlev + 1 + g.P + 2 * cap) * (int64_t)sizeof(int) + 255) / 256 * 256;
+    total += tabs;
+    if (hipMalloc(&g.solvebuf, total) != hipSuccess) {
+      g.solvebuf = nullptr;
+      set_error("grouped LU: solve buffers (%lld bytes) not allocated", (long long)total);
+      return MG_ERR_NOMEM;
+    }
+    char* q = static_cast<char*>(g.solvebuf);
+    for (int p = 0; p < g.P; ++p) {
+      int64_t a, b, c;
+      sizes(p, a, b, c);
+      g.host[p].perm = reinterpret_cast<int*>(q);
+      g.host[p].Dinv = q + a;
+      g.host[p].W1 = q + a + b;
+      g.host[p].W2 = q + a + b + c;
+      q += a + b + 2 * c;
+    }
+    g.sweep_lev = reinterpret_cast<int*>(q);
+    g.sweep_coff = g.sweep_lev + nlev + 1;
+    g.sweep_ctr = g.sweep_coff + g.P;
+    g.sweep_nlev = nlev;
+    g.sweep_cap = cap;
+    MG_HIP(hipMemcpy(g.sweep_lev, lev.data(), sizeof(int) * (nlev + 1), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(g.sweep_coff, coff.data(), sizeof(int) * g.P, hipMemcpyHostToDevice));
+    MG_HIP(hipMemset(g.sweep_ctr, 0, sizeof(int) * 2 * cap));
+    MG_HIP(hipMemcpy(g.dtab, g.host.data(), sizeof(GrpProb) * g.P, hipMemcpyHostToDevice));
+    g.sweep_tickets = lev[nlev];
   }
   if (!dma_split(g)) return MG_OK;
   if (g.split && g.split_nb >= g.cfg.nb) return MG_OK;
@@ -1296,25 +1722,82 @@ int grp_getrs(Group& g) {
   return MG_OK;
 }
 
+// BF16x3 groups with right-hand sides solve from inverted diagonal blocks.
+bool inv_solve(const Group& g) { return g.prec == MG_BF16X3 && g.nrhs > 0 && g.solvebuf; }
+
+int64_t sblocks(int64_t n) { return (n + kSB - 1) / kSB; }
+
+// After the factorisation: perm and the inverted diagonal blocks.
+int grp_solve_prep(Group& g) {
+  hipLaunchKernelGGL(k_grp_perm, dim3(g.P), dim3(256), 0, g.s, g.dtab);
+  MG_HIP(hipGetLastError());
+  GrpMap m;
+  int total;
+  if (g.make_map(g.P, [&](int p) -> int64_t { return 2 * sblocks(g.n[p]); }, m, total)) {
+    hipLaunchKernelGGL(k_grp_trinv, dim3(total), dim3(64), 0, g.s, g.dtab, m);
+    MG_HIP(hipGetLastError());
+  }
+  g.launches += 2;
+  return MG_OK;
+}
+
+int grp_solve_inv(Group& g) {
+  const int nr = (int)g.nrhs, nct = (nr + 31) / 32;
+  GrpMap m;
+  int total;
+  if (g.make_map(g.P, [&](int p) -> int64_t { return (g.n[p] * nr + 255) / 256; }, m, total)) {
+    hipLaunchKernelGGL(k_grp_gather, dim3(total), dim3(256), 0, g.s, g.dtab, m, nr);
+    MG_HIP(hipGetLastError());
+    ++g.launches;
+  }
+  (void)nct;
+  // counter set 0 cleared (it is clear after any completed solve: the
+  // backward sweep clears it), the forward sweep on set 0 clears set 1 for
+  // the backward sweep
+  int* c0 = g.sweep_ctr;
+  int* c1 = g.sweep_ctr + g.sweep_cap;
+  hipLaunchKernelGGL(k_grp_clear, dim3(std::min(64, (g.sweep_cap + 255) / 256)), dim3(256), 0, g.s, c0, g.sweep_cap);
+  hipLaunchKernelGGL(k_grp_sweep<true>, dim3(g.sweep_tickets), dim3(128), 0, g.s, g.dtab, g.sweep_lev, g.sweep_nlev,
+                     g.sweep_coff, nr, c0, c1, g.sweep_cap);
+  hipLaunchKernelGGL(k_grp_sweep<false>, dim3(g.sweep_tickets), dim3(128), 0, g.s, g.dtab, g.sweep_lev, g.sweep_nlev,
+                     g.sweep_coff, nr, c1, c0, g.sweep_cap);
+  MG_HIP(hipGetLastError());
+  g.launches += 3;
+  return MG_OK;
+}
+
 int run_factor(Group& g) {
   switch (g.prec) {
     case MG_C128: return grp_getrf<double>(g, false);
     case MG_C64: return grp_getrf<float>(g, false);
-    case MG_BF16X3: return grp_getrf<float>(g, true);
+    case MG_BF16X3:
+      MG_TRY(grp_getrf<float>(g, true));
+      return inv_solve(g) ? grp_solve_prep(g) : MG_OK;
     default: return MG_ERR_ARG;
   }
 }
 
 int run_solve(Group& g) {
-  return g.prec == MG_C128 ? grp_getrs<double>(g) : grp_getrs<float>(g);
+  if (g.prec == MG_C128) return grp_getrs<double>(g);
+  return inv_solve(g) ? grp_solve_inv(g) : grp_getrs<float>(g);
 }
 
 // B -= A X for every problem (X: the auxiliary blocks), products in the
 // group's element type.
 int run_residual(Group& g) {
+  if (g.prec == MG_C128) {   // the refinement's FP64 residual: one streaming pass over every matrix
+    GrpMap m;
+    int total;
+    const int64_t nct = (g.nrhs + kResCols - 1) / kResCols;
+    if (!g.make_map(g.P, [&](int p) -> int64_t { return (g.n[p] + kResRows - 1) / kResRows * nct; }, m, total))
+      return MG_OK;
+    hipLaunchKernelGGL(k_grp_residual, dim3(total), dim3(256), 0, g.s, g.dtab, m, (int)g.nrhs);
+    MG_HIP(hipGetLastError());
+    ++g.launches;
+    return MG_OK;
+  }
   const GrpGemm op{0, INT64_MAX, 0, g.nrhs, 0, INT64_MAX, 2};
-  const int tile = g.nrhs <= 32 ? 2 : 0;
-  return g.prec == MG_C128 ? grp_gemm<double>(g, op, false, tile) : grp_gemm<float>(g, op, false, tile);
+  return grp_gemm<float>(g, op, false, g.nrhs <= 32 ? 2 : 0);
 }
 
 int run_colnorm(Group& g, double* out) {
@@ -1361,8 +1844,11 @@ TuneKey tune_key(const Group& g) {
 
 // panel widths around the measured best (256 for the 3D ensemble's orders
 // 1500 .. 9000 in both precisions; the 1024-thread leaf and the 128 x 64
-// tile won there at every width)
+// tile won there at every width).  BF16x3 groups leave out 128: on a
+// sample of 32 of the 256 problems it timed fastest, on the whole group
+// slowest (1164 ms against 730 at 256 and 718 at 384).
 const GrpCfg kCandidates[] = {{256, 16, 0, 1024}, {128, 16, 0, 1024}, {384, 16, 0, 1024}};
+const GrpCfg kCandidatesSplit[] = {{256, 16, 0, 1024}, {384, 16, 0, 1024}};
 
 int tune(Group& g, hipStream_t s, float* best_ms) {
   const size_t esz = g.prec == MG_C128 ? 16 : 8;
@@ -1427,9 +1913,12 @@ int tune(Group& g, hipStream_t s, float* best_ms) {
     return MG_OK;
   };
   t.s = s;
-  const int nc = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+  const bool sp = g.prec == MG_BF16X3;
+  const GrpCfg* cands = sp ? kCandidatesSplit : kCandidates;
+  const int nc = sp ? (int)(sizeof(kCandidatesSplit) / sizeof(kCandidatesSplit[0]))
+                    : (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
   for (int c = -1; c < nc && st == MG_OK; ++c) {   // c = -1: untimed warm-up
-    t.cfg = kCandidates[c < 0 ? 0 : c];
+    t.cfg = cands[c < 0 ? 0 : c];
     t.cfg.pivot = g.cfg.pivot;
     if ((st = ensure_split(t)) != MG_OK) break;
     if ((st = refill()) != MG_OK) break;
@@ -1448,6 +1937,7 @@ int tune(Group& g, hipStream_t s, float* best_ms) {
   for (void* b : bufs) (void)hipFree(b);
   if (t.split) (void)hipFree(t.split);
   if (t.cand) (void)hipFree(t.cand);
+  if (t.solvebuf) (void)hipFree(t.solvebuf);
   (void)hipFree(ints);
   if (t.dtab) (void)hipFree(t.dtab);
   if (e0) (void)hipEventDestroy(e0);
@@ -1563,6 +2053,7 @@ int mg_group_destroy(mg_group* h) {
   if (h->g.dtab) (void)hipFree(h->g.dtab);
   if (h->g.split) (void)hipFree(h->g.split);
   if (h->g.cand) (void)hipFree(h->g.cand);
+  if (h->g.solvebuf) (void)hipFree(h->g.solvebuf);
   if (h->g.evA) (void)hipEventDestroy(h->g.evA);
   if (h->g.evB) (void)hipEventDestroy(h->g.evB);
   if (h->g.s2) (void)hipStreamDestroy(h->g.s2);

@@ -58,6 +58,19 @@ __device__ inline void cmac(const bf16x8* ar, const bf16x8* ai, const bf16x8* na
     }
 }
 
+// Operands and accumulators live in device memory: the tile reads and writes
+// them through the global address space, also when a pointer was read from
+// memory (a grouped launch's descriptor table), where the compiler would
+// otherwise emit flat accesses, which also count against the LDS counter (the
+// waits for a k-tile's LDS reads would then wait for the next k-tile's loads).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(1))) float2* gcf2;
+typedef __attribute__((address_space(1))) float2* gf2;
+#else
+typedef const float2* gcf2;   // host pass: never executed
+typedef float2* gf2;
+#endif
+
 // Agent-scope (cross-XCD coherent) element access, for data that other
 // workgroups of the same launch write.
 __device__ inline float2 coherent_load(const float2* p) {
@@ -102,7 +115,8 @@ struct Tile {
   // elsewhere and everywhere when !load.
   // All loads are issued before any is used (a whole tile inside C takes the
   // branch-free path), so that they overlap.
-  __device__ void init(const float2* C, int64_t ldc, int64_t M, int64_t N, float2 q, bool load) {
+  __device__ void init(const float2* Cp, int64_t ldc, int64_t M, int64_t N, float2 q, bool load) {
+    const gcf2 C = (gcf2)Cp;
     float2 c[TM][TN][4];
     const bool whole = load && M >= BM && N >= BN;
 #pragma unroll
@@ -139,9 +153,10 @@ struct Tile {
   // the tile origin; entries outside M, N, K read as 0.  COHERENT_B: B with
   // agent-scope loads (B written by other workgroups of this launch).
   template <bool COHERENT_B = false>
-  __device__ void fetch(const float2* __restrict__ A, int64_t lda, const float2* B, int64_t ldb, int64_t M, int64_t N,
-                        int64_t K, int64_t k0) {
+  __device__ void fetch(const float2* __restrict__ Ap, int64_t lda, const float2* B, int64_t ldb, int64_t M,
+                        int64_t N, int64_t K, int64_t k0) {
     static_assert(kFloatStaging, "staging layout");
+    const gcf2 A = (gcf2)Ap, Bg = (gcf2)B;
     const int tid = ltid();
     const int ar = tid / (BK / 8), ak = (tid % (BK / 8)) * 8;
     const int bc = tid % BN, bk = (tid / BN) * 8;
@@ -166,7 +181,7 @@ struct Tile {
         const int64_t k = k0 + bk + 8 * BGRP * p + e, col = bc;
         if (bin || (k < K && col < N)) {
           if constexpr (COHERENT_B) rb[p][e] = coherent_load(&B[k * ldb + col]);
-          else rb[p][e] = B[k * ldb + col];
+          else rb[p][e] = Bg[k * ldb + col];
         } else {
           rb[p][e] = make_float2(0.f, 0.f);
         }
@@ -289,6 +304,7 @@ struct Tile {
   // COHERENT: agent-scope stores (read by other workgroups of this launch).
   template <bool COHERENT = false>
   __device__ void store(float2* C, int64_t ldc, int64_t M, int64_t N, float2 alpha, float2 beta, bool alpha0) const {
+    const gf2 Cg = (gf2)C;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -301,11 +317,11 @@ struct Tile {
             const float xr = cr[a][b][r], xi = ci[a][b][r];
             float2 o = make_float2(alpha.x * xr - alpha.y * xi, alpha.x * xi + alpha.y * xr);
             if (alpha0) {
-              const float2 c = C[row * ldc + col];
+              const float2 c = Cg[row * ldc + col];
               o = make_float2(beta.x * c.x - beta.y * c.y, beta.x * c.y + beta.y * c.x);
             }
             if constexpr (COHERENT) coherent_store(&C[row * ldc + col], o);
-            else C[row * ldc + col] = o;
+            else Cg[row * ldc + col] = o;
           }
         }
       }

@@ -54,7 +54,7 @@ class Ensemble:
 
     def __init__(self, device="cuda:0", streams=4, method="fp64", tol=1e-12, measure=False, grouped=False,
                  group_bytes=None, tune=True, max_refine=12):
-        # group_bytes: matrix bytes per group (default half the device's memory)
+        # group_bytes: matrix bytes per group (default 0.6 of the device's memory)
         self.dev = torch.device(device)
         if self.dev.index is None:
             self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -138,7 +138,7 @@ class Ensemble:
         per = 16 + (8 if bf else 0)
         cap = self.group_bytes
         if cap is None:
-            cap = 0.5 * torch.cuda.get_device_properties(self.dev).total_memory
+            cap = 0.6 * torch.cuda.get_device_properties(self.dev).total_memory
         groups, cur, used = [], [], 0
         for i in order:
             w = workloads[i]

@@ -108,20 +108,23 @@ def test_group_zero_pivot_info():
     g.close()
 
 
-@pytest.mark.parametrize("prec,tol,nb,pivot", [("c64", 2e-4, 256, "partial"), ("bf16x3", 2e-3, 256, "tournament"),
-                                               ("bf16x3", 2e-3, 128, "tournament"),
-                                               ("bf16x3", 2e-3, 384, "tournament"), ("bf16x3", 2e-3, 64, "tournament"),
-                                               ("bf16x3", 2e-3, 256, "partial")])
-def test_group_complex_float(prec, tol, nb, pivot):
+@pytest.mark.parametrize("prec,tol,nb,pivot,nrhs", [("c64", 2e-4, 256, "partial", 8),
+                                                    ("bf16x3", 2e-3, 256, "tournament", 8),
+                                                    ("bf16x3", 2e-3, 128, "tournament", 8),
+                                                    ("bf16x3", 2e-3, 384, "tournament", 40),
+                                                    ("bf16x3", 2e-3, 64, "tournament", 8),
+                                                    ("bf16x3", 2e-3, 256, "partial", 40)])
+def test_group_complex_float(prec, tol, nb, pivot, nrhs):
     """complex64 factors (the trailing updates on FP32 or BF16x3 products;
     BF16x3 at panel widths 128, 256 and 384 from operands split once per
     step, at 64 split in every tile; tournament or partial pivoting in the
-    leaves): solutions within the precision of the factor of the complex128
-    ones."""
-    say("group", prec, nb, pivot)
+    leaves; BF16x3 solves from inverted diagonal blocks, one and two
+    32-column tiles): solutions within the precision of the factor of the
+    complex128 ones."""
+    say("group", prec, nb, pivot, nrhs)
     host = _mats(SIZES, 5, dtype=np.complex64, shrink=1.0)
     rng = np.random.default_rng(6)
-    rhs_host = [(rng.normal(size=(A.shape[0], 8)) + 1j * rng.normal(size=(A.shape[0], 8))).astype(np.complex64)
+    rhs_host = [(rng.normal(size=(A.shape[0], nrhs)) + 1j * rng.normal(size=(A.shape[0], nrhs))).astype(np.complex64)
                 for A in host]
     for i in range(len(host)):
         host[i] = (host[i] + 4 * np.sqrt(host[i].shape[0]) * np.eye(host[i].shape[0])).astype(np.complex64)
@@ -140,15 +143,19 @@ def test_group_complex_float(prec, tol, nb, pivot):
     g.close()
 
 
-def test_group_residual_and_colnorms():
+@pytest.mark.parametrize("nrhs", [5, 40])
+def test_group_residual_and_colnorms(nrhs):
     """B_i -= A_i X_i over the group in one launch and the column norms,
-    against numpy, complex128 and complex64."""
-    say("group residual")
+    against numpy, complex128 (orders off the 16-deep k chunks and 128-row
+    blocks, one and two 32-column tiles) and complex64."""
+    say("group residual", nrhs)
     for dt, tol in ((np.complex128, 1e-13), (np.complex64, 1e-5)):
-        host = _mats([300, 41, 1, 129], 7, dtype=dt, shrink=1.0)
+        host = _mats([300, 41, 1, 129, 1040], 7, dtype=dt, shrink=1.0)
         rng = np.random.default_rng(8)
-        Bh = [(rng.normal(size=(A.shape[0], 5)) + 1j * rng.normal(size=(A.shape[0], 5))).astype(dt) for A in host]
-        Xh = [(rng.normal(size=(A.shape[0], 5)) + 1j * rng.normal(size=(A.shape[0], 5))).astype(dt) for A in host]
+        Bh = [(rng.normal(size=(A.shape[0], nrhs)) + 1j * rng.normal(size=(A.shape[0], nrhs))).astype(dt)
+              for A in host]
+        Xh = [(rng.normal(size=(A.shape[0], nrhs)) + 1j * rng.normal(size=(A.shape[0], nrhs))).astype(dt)
+              for A in host]
         mats = [torch.from_numpy(A).to(DEV) for A in host]
         rhs = [torch.from_numpy(B).to(DEV) for B in Bh]
         g = E.GroupLU(mats, rhs, prec="c128" if dt == np.complex128 else "c64")
