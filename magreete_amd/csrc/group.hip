@@ -1143,24 +1143,30 @@ __global__ void k_grp_clear(int* c, int cap) {
 
 // ---- FP64 residual of the refinement: B -= A X ------------------------------
 // Complex128 A (n x n) and X (n x nrhs, the auxiliary block), B the
-// right-hand sides.  One pass over A at streaming rate: a workgroup owns 128
-// rows (32 per wave: two 16-row MFMA fragments) and 32 right-hand sides and
-// walks k in chunks of 8 (two 16x16x4 FP64 MFMA steps; chunks of 16 spill
-// registers at two waves per SIMD); each lane's A and X fragments of the next
-// chunk are loaded into registers while the current chunk is multiplied,
-// straight from memory (every element of A is read once; X, n x 32, through
-// the caches).  Complex products in the
-// three-product form (t1 = Ar Xr, t2 = Ai Xi, t3 = (Ar + Ai)(Xr + Xi)).
-// Rows, k and columns past the problem's extents are loaded from clamped
-// addresses and zeroed or not stored.
-constexpr int kResRows = 128, kResCols = 32, kResSteps = 2;
+// right-hand sides.  One pass over A at streaming rate: a workgroup of four
+// waves owns 64 RF rows (16 RF per wave: RF 16-row MFMA fragments) and 32
+// right-hand sides and walks k in chunks of 4 KS (KS 16x16x4 FP64 MFMA
+// steps); each lane's A and X fragments of the next chunk are loaded into
+// registers while the current chunk is multiplied, straight from memory
+// (every element of A is read once; X, n x 32, through the caches).
+// Complex products in the three-product form (t1 = Ar Xr, t2 = Ai Xi, t3 =
+// (Ar + Ai)(Xr + Xi)).  Rows, k and columns past the problem's extents are
+// loaded from clamped addresses and zeroed or not stored.
+// Measured on the 256-matrix ensemble (96 GB of complex128, 32 columns):
+// RF 2, KS 2 at two waves per SIMD 36 ms; RF 1 with KS 4 (three waves per
+// SIMD) 41 ms, with KS 2 44 ms; the rows staged through LDS by DMA in
+// 512-byte segments (three stages, one workgroup per CU) 59 ms.
+constexpr int kResCols = 32;
 
+template <int RF, int KS>
 struct ResFrag {
-  double2 a[2][kResSteps];   // [row fragment][k-step]
-  double2 x[2][kResSteps];   // [column fragment][k-step]
+  double2 a[RF][KS];   // [row fragment][k-step]
+  double2 x[2][KS];    // [column fragment][k-step]
 };
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_grp_residual(const GrpProb* __restrict__ tab, GrpMap map, int nrhs) {
+template <int RF, int KS, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_grp_residual(
+    const GrpProb* __restrict__ tab, GrpMap map, int nrhs) {
   const int id = xcd_local(blockIdx.x, gridDim.x);
   const int p = grp_find(map, id);
   const GrpProb pb = tab[p];
@@ -1169,52 +1175,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int nct = (nrhs + kResCols - 1) / kResCols;
   const int rb = local / nct, ct = local - rb * nct;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-  const int64_t r0 = (int64_t)rb * kResRows + wave * 32;
+  const int64_t r0 = (int64_t)rb * (64 * RF) + wave * (16 * RF);
   const int c0 = ct * kResCols;
   gcptr<double2> A = (gcptr<double2>)reinterpret_cast<const double2*>(pb.A);
   gcptr<double2> X = (gcptr<double2>)reinterpret_cast<const double2*>(pb.X);
   // this lane's A entries: rows r0 + 16 f + li (clamped), k = k0 + 4 s + lk;
   // X entries: k = k0 + 4 s + lk, columns c0 + 16 g + li (clamped)
-  gcptr<double2> pa[2];
+  gcptr<double2> pa[RF];
   gcptr<double2> px[2];
 #pragma unroll
-  for (int f = 0; f < 2; ++f) {
+  for (int f = 0; f < RF; ++f) {
     const int64_t r = r0 + 16 * f + li;
     pa[f] = A + (r < n ? r : n - 1) * pb.lda + lk;
-    const int64_t c = c0 + 16 * f + li;
-    px[f] = X + (int64_t)lk * pb.ldx + (c < nrhs ? c : nrhs - 1);
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int64_t c = c0 + 16 * g + li;
+    px[g] = X + (int64_t)lk * pb.ldx + (c < nrhs ? c : nrhs - 1);
   }
   const int64_t xstep = 4 * pb.ldx;
+  typedef ResFrag<RF, KS> Frag;
   // full chunks: unpredicated loads at k0 (pointers advanced by the caller)
-  auto fetch = [&](ResFrag& fr) {
+  auto fetch = [&](Frag& fr) {
 #pragma unroll
-    for (int s = 0; s < kResSteps; ++s)
+    for (int s = 0; s < KS; ++s) {
 #pragma unroll
-      for (int f = 0; f < 2; ++f) {
-        fr.a[f][s] = pa[f][4 * s];
-        fr.x[f][s] = px[f][s * xstep];
-      }
-  };
-  constexpr int64_t KC = 4 * kResSteps;
-  auto advance = [&]() {
+      for (int f = 0; f < RF; ++f) fr.a[f][s] = pa[f][4 * s];
 #pragma unroll
-    for (int f = 0; f < 2; ++f) {
-      pa[f] += KC;
-      px[f] += KC * pb.ldx;
+      for (int g = 0; g < 2; ++g) fr.x[g][s] = px[g][s * xstep];
     }
   };
-  v4d t1[2][2], t2[2][2], t3[2][2];
+  constexpr int64_t KC = 4 * KS;
+  auto advance = [&]() {
 #pragma unroll
-  for (int f = 0; f < 2; ++f)
+    for (int f = 0; f < RF; ++f) pa[f] += KC;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) px[g] += KC * pb.ldx;
+  };
+  v4d t1[RF][2], t2[RF][2], t3[RF][2];
+#pragma unroll
+  for (int f = 0; f < RF; ++f)
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int r = 0; r < 4; ++r) t1[f][g][r] = t2[f][g][r] = t3[f][g][r] = 0.0;
-  auto step = [&](const ResFrag& fr) {
+  auto step = [&](const Frag& fr) {
 #pragma unroll
-    for (int s = 0; s < kResSteps; ++s) {
+    for (int s = 0; s < KS; ++s) {
 #pragma unroll
-      for (int f = 0; f < 2; ++f) {
+      for (int f = 0; f < RF; ++f) {
         const double ar = fr.a[f][s].x, ai = fr.a[f][s].y, as = ar + ai;
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
@@ -1227,7 +1236,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
   };
   const int64_t nfull = n / KC;   // whole chunks; the rest (n mod KC) below
-  ResFrag b0, b1;
+  Frag b0, b1;
   if (nfull > 0) {
     fetch(b0);
     advance();
@@ -1253,31 +1262,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
   if (nfull * KC < n) {   // the last partial chunk, k past n zeroed
 #pragma unroll
-    for (int s = 0; s < kResSteps; ++s) {
+    for (int s = 0; s < KS; ++s) {
       const bool ok = nfull * KC + 4 * s + lk < n;
 #pragma unroll
-      for (int f = 0; f < 2; ++f) {
-        b0.a[f][s] = ok ? (double2)pa[f][4 * s] : make_double2(0.0, 0.0);
-        b0.x[f][s] = ok ? (double2)px[f][s * xstep] : make_double2(0.0, 0.0);
-      }
+      for (int f = 0; f < RF; ++f) b0.a[f][s] = ok ? (double2)pa[f][4 * s] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int g = 0; g < 2; ++g) b0.x[g][s] = ok ? (double2)px[g][s * xstep] : make_double2(0.0, 0.0);
     }
     step(b0);
   }
   // accumulator item r of lane l: row l / 16 + 4 r, column l % 16 (FP64 MFMA)
   gptr<double2> B = (gptr<double2>)reinterpret_cast<double2*>(pb.B);
 #pragma unroll
-  for (int f = 0; f < 2; ++f)
+  for (int f = 0; f < RF; ++f)
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = r0 + 16 * f + lk + 4 * r;
-        const int c = c0 + 16 * g + li;
-        if (row < n && c < nrhs) {
-          double2 v = B[row * pb.ldb + c];
+        const int col = c0 + 16 * g + li;
+        if (row < n && col < nrhs) {
+          double2 v = B[row * pb.ldb + col];
           v.x -= t1[f][g][r] - t2[f][g][r];
           v.y -= t3[f][g][r] - t1[f][g][r] - t2[f][g][r];
-          B[row * pb.ldb + c] = v;
+          B[row * pb.ldb + col] = v;
         }
       }
 }
@@ -1789,9 +1797,10 @@ int run_residual(Group& g) {
     GrpMap m;
     int total;
     const int64_t nct = (g.nrhs + kResCols - 1) / kResCols;
-    if (!g.make_map(g.P, [&](int p) -> int64_t { return (g.n[p] + kResRows - 1) / kResRows * nct; }, m, total))
+    constexpr int RF = 2, KS = 2, WPE = 2;
+    if (!g.make_map(g.P, [&](int p) -> int64_t { return (g.n[p] + 64 * RF - 1) / (64 * RF) * nct; }, m, total))
       return MG_OK;
-    hipLaunchKernelGGL(k_grp_residual, dim3(total), dim3(256), 0, g.s, g.dtab, m, (int)g.nrhs);
+    hipLaunchKernelGGL((k_grp_residual<RF, KS, WPE>), dim3(total), dim3(256), 0, g.s, g.dtab, m, (int)g.nrhs);
     MG_HIP(hipGetLastError());
     ++g.launches;
     return MG_OK;
