@@ -21,20 +21,34 @@ sys.path.insert(0, ROOT)
 
 METRIC = "3D N=4096 solves/s: mixed-prec+IR vs FP64 @≤1e-9 err; serving p50/p99 latency"
 PEAK = {"fp64": 78.6, "c64": 157.3, "bf16x3": 2500.0 / 3.0, "bf16x6": 2500.0 / 6.0}   # TFLOP/s per product form
-# the factorisation's trailing-update launches (timed with HIP events on their stream)
-KERNEL = {"fp64": "k_cgemm_trail<double> (pivoted LU trailing update; 128x64 tiles, 3-product complex MFMA)",
-          "c64": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)",
-          "bf16x3": "k_ldu_update_dma<1|2> (block-LDU trailing update over pre-split panels, one or two steps per "
-                    "launch; BF16x3 MFMA, 128x128 tiles, operands DMA'd to LDS) for updates with a tile per CU, "
-                    "k_ldu_update<2, 64, 128, 2, 2> for the smaller ones",
-          "bf16x6": "k_ldu_update<3, 64, 64, 2, 2> (block-LDU trailing update over pre-split panels, one or two "
-                    "steps per launch; BF16x6 MFMA, 64x64 tiles)"}
-# HBM bytes per trailing-update launch (average over one factorisation of this workload), measured
-# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the
-# counters' reading for that 2 GiB copy (fetch x2.000, write x1.000)
-TRAFFIC = {"fp64": 2.507e9, "c64": 4.410e8, "bf16x3": 1.658e8, "bf16x6": 2.709e8}
-
-
+# The FP64 kernels multiply complex numbers with three real MFMA products (Gauss) where the algorithmic
+# count is four (8 m n k flops): the form's peak in algorithmic flops is 4/3 of the FP64 MFMA peak.
+PEAK_Z3 = 78.6 * 4.0 / 3.0
+# peak of each trailing-update kernel class's product form (the FP64 64x64 register-staged tile multiplies
+# with four real products, every other FP64 / FP32 kernel here with three)
+PEAK_CLASS = {("fp64", "dma"): PEAK_Z3, ("fp64", "reg"): 78.6, ("c64", "reg"): 157.3 * 4.0 / 3.0,
+              ("bf16x3", "dma"): PEAK["bf16x3"], ("bf16x3", "reg"): PEAK["bf16x3"], ("bf16x6", "reg"): PEAK["bf16x6"]}
+# the factorisation's trailing-update launches (timed with HIP events on their stream), per kernel class
+KERNEL = {"fp64": {"dma": "k_zgemm_dma_trail<64> (pivoted LU trailing update; 128x64 tiles, operands DMA'd to LDS "
+                          "through three stages, 3-product complex MFMA)",
+                   "reg": "k_cgemm_trail<double, 64, 64> (the updates too small for a tile per CU)"},
+          "c64": {"reg": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)"},
+          "bf16x3": {"dma": "k_ldu_update_dma<1|2> (block-LDU trailing update over pre-split panels, one or two "
+                            "steps per launch; BF16x3 MFMA, 128x128 tiles, operands DMA'd to LDS)",
+                     "reg": "k_ldu_update<2, 64, 128, 2, 2> (the chain-side updates too small for a 128x128 tile "
+                            "per CU: next diagonal blocks, block rows and columns of the next super-step)"},
+          "bf16x6": {"reg": "k_ldu_update<3, 64, 64, 2, 2> (block-LDU trailing update over pre-split panels, one or "
+                            "two steps per launch; BF16x6 MFMA, 64x64 tiles)"}}
+PRODUCT_KERNEL = ("k_zgemm_dma<64> + k_splitk_reduce<double> (the refinement's FP64 Krylov product M Z, "
+                  "12288 x 256 x 12288 split over K in two; 128x64 tiles, operands DMA'd to LDS, 3-product "
+                  "complex MFMA)")
+# HBM bytes per launch (average over one solve of this workload), measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the counters' reading for
+# that 2 GiB copy (fetch x2.000, write x1.000); None where not measured
+TRAFFIC = {("fp64", "dma"): None, ("fp64", "reg"): None, ("c64", "reg"): 4.410e8,
+           ("bf16x3", "dma"): None, ("bf16x3", "reg"): None, ("bf16x6", "reg"): 2.709e8, "product": None}
+# algorithmic HBM bytes per launch, for comparison with TRAFFIC: the FP64 product reads M (16 n^2 B) and
+# Z and writes the two K-split partials and their sum (16 n T B each way, ~3 times)
 def colerr(x, ref):
     x = x.reshape(-1, x.shape[-1])
     ref = ref.reshape(-1, ref.shape[-1])
@@ -125,14 +139,19 @@ def main():
                 its.append(dict(s.stats))
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
-            launches, flops, ms = E.read_update_profile(dev)
+            cls = E.read_update_profile_classes(dev)
             E.profile_updates(dev, False)
             E.check_barriers(dev)
             info = int(s.info.item())
+            launches = sum(v[0] for v in cls.values())
+            flops = sum(v[1] for v in cls.values())
+            ms = sum(v[2] for v in cls.values())
             results[method] = {
                 "solves_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps, "info": info,
                 "update_launches": launches, "update_tflops": (flops / (ms * 1e-3) / 1e12) if ms > 0 else None,
                 "update_ms": ms / args.steps, "refine": its[-1],
+                "updates": {k: {"launches_per_step": v[0] / args.steps, "ms_per_step": v[2] / args.steps,
+                                "tflops": v[1] / (v[2] * 1e-3) / 1e12} for k, v in cls.items() if v[0] > 0},
             }
             outputs[method] = (s.X.clone(), s.out.clone())
             if method == best_candidate(args):
@@ -200,15 +219,32 @@ def main():
         sys.exit(1)
     rb = results[best]
 
-    def roofline(method):
+    def kernel_rooflines(method):
+        """Per-kernel rooflines of a method: its trailing-update classes and, for the mixed methods, the
+        FP64 Krylov product (timed in phase_rooflines), each with its time per solve."""
         r = results.get(method, {})
-        if not r.get("update_tflops"):
-            return None
         base = method.split("-")[0]
-        return {"bound": "mfma", "achieved": r["update_tflops"], "peak": PEAK[base], "unit": "TFLOP/s",
-                "frac": r["update_tflops"] / PEAK[base], "traffic": TRAFFIC.get(base),
-                "kernel": KERNEL[base], "launches_per_step": r["update_launches"] / args.steps,
-                "ms_per_step": r["update_ms"]}
+        out = {}
+        for k, u in (r.get("updates") or {}).items():
+            pk = PEAK_CLASS[(base, k)]
+            out["update_" + k] = {
+                "bound": "mfma", "achieved": u["tflops"], "peak": pk, "unit": "TFLOP/s",
+                "frac": u["tflops"] / pk, "traffic": TRAFFIC.get((base, k)), "kernel": KERNEL[base][k],
+                "launches_per_step": u["launches_per_step"], "ms_per_step": u["ms_per_step"]}
+        ph = r.get("phases") or (results.get(best_candidate(args), {}).get("phases") or {})
+        prod = ph.get("fp64_product")
+        nprod = sum(1 for p in (r.get("refine") or {}).get("products", []) if p.startswith("fp64"))
+        if prod and nprod and base != "fp64":
+            out["fp64_product"] = dict(prod, launches_per_step=nprod, ms_per_step=nprod * prod["ms"])
+        return out
+
+    def roofline(method):
+        """The dominant kernel of the method: the one with the most time per solve."""
+        ks = kernel_rooflines(method)
+        if not ks:
+            return None
+        name = max(ks, key=lambda k: ks[k]["ms_per_step"])
+        return dict(ks[name], which=name)
 
     roof = roofline(best)
     speedup = {}
@@ -233,6 +269,7 @@ def main():
             "seed": args.seed, "tol": args.tol,
             "relaxed_products": not args.no_relax and not best.endswith("-norelax")},
         "roofline": roof, "roofline_fp64": roofline("fp64") if best != "fp64" else None,
+        "rooflines_kernels": kernel_rooflines(best),
         "speedup_vs_fp64": speedup,
         "rooflines_other": dict(method=best_candidate(args),
                                 **(results.get(best_candidate(args), {}).get("phases") or {})),
@@ -259,7 +296,10 @@ def host_info():
 def best_candidate(args):
     """The method whose other kernels get rooflines (the expected headline method)."""
     ms = [m for m in args.methods.split(",") if m]
-    return "bf16x3" if "bf16x3" in ms else (ms[0] if ms else None)
+    for m in ("bf16x3-norelax", "bf16x3"):
+        if m in ms:
+            return m
+    return ms[0] if ms else None
 
 
 def phase_rooflines(s, prob, args):
@@ -287,6 +327,27 @@ def phase_rooflines(s, prob, args):
         out["propagation"] = {"bound": "mfma", "achieved": fl / t / 1e12, "peak": 78.6, "unit": "TFLOP/s",
                               "frac": fl / t / 78.6e12, "ms": ph["propagate"], "flops": fl,
                               "kernel": "k_propagate<3D_VECTOR> (+ k_sum_parts_src: partial sums plus the incident field)"}
+    if s.method != "fp64":
+        # the refinement's FP64 Krylov product M Z (the same call as refine.py's), on the current stream
+        st = torch.cuda.current_stream()
+        Z = torch.randn((n, T), dtype=torch.complex128, device=s.M.device)
+        Wz = torch.empty_like(Z)
+        E.cgemm(s.M, Z, Wz, split=True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record(st)
+        for _ in range(reps):
+            E.cgemm(s.M, Z, Wz, split=True)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        fl = 8.0 * n * n * T
+        out["fp64_product"] = {"bound": "mfma", "achieved": fl / (ms * 1e-3) / 1e12, "peak": PEAK_Z3,
+                               "unit": "TFLOP/s", "frac": fl / (ms * 1e-3) / (PEAK_Z3 * 1e12), "ms": ms,
+                               "flops": fl, "traffic": TRAFFIC.get("product"),
+                               "algorithmic_bytes": 16.0 * n * n + 16.0 * n * T * 4,
+                               "kernel": PRODUCT_KERNEL}
+        del Z, Wz
     if s.method in ("bf16x3", "bf16x6"):
         M32 = torch.empty((n, n), dtype=torch.complex64, device=s.M.device)
         E.convert(s.M, M32)

@@ -136,11 +136,15 @@ int mg_set_leaf_blocks(mg_handle* h, int g);
 /* Waits for `stream`; MG_ERR_SYNC if a grid barrier of an LU leaf failed. */
 int mg_check(mg_handle* h, void* stream);
 /* Measurement hooks with no reference counterpart: HIP-event timing of the
- * LU trailing-update launches (kernel k_cgemm_trail) on the launching stream.
+ * LU trailing-update launches (pivoted LU: k_zgemm_dma_trail / k_cgemm_trail;
+ * block LDU: k_ldu_update_dma / k_ldu_update) on the launching stream.
  * read returns launches, algorithmic flops (8 m n k each) and summed ms since
- * the last read, and synchronises the stream. */
+ * the last read, and synchronises the stream; read_classes returns the same
+ * per kernel class in arrays of two (element 0: operands staged through
+ * registers, element 1: operands DMA'd to LDS). */
 int mg_profile_enable(mg_handle* h, int on);
 int mg_profile_read(mg_handle* h, void* stream, long long* launches, double* flops, double* ms);
+int mg_profile_read_classes(mg_handle* h, void* stream, long long* launches, double* flops, double* ms);
 /* Element conversion of a rows x cols block; dir 0: c128 -> c64, 1: c64 -> c128. */
 int mg_convert(int dir, const void* src, int64_t lds, void* dst, int64_t ldd, int64_t rows, int64_t cols,
                void* stream);

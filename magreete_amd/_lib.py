@@ -44,6 +44,7 @@ SIGNATURES = {
     "mg_getrs_dinv": [_vp, _int, _vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp],
     "mg_profile_enable": [_vp, _int],
     "mg_profile_read": [_vp, _vp, _c.POINTER(_c.c_longlong), _c.POINTER(_dbl), _c.POINTER(_dbl)],
+    "mg_profile_read_classes": [_vp, _vp, _c.POINTER(_c.c_longlong), _c.POINTER(_dbl), _c.POINTER(_dbl)],
     "mg_convert": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _vp],
     "mg_propagate": [_int, _vp, _i64, _vp, _i64, _dbl, _dbl, _dbl, _int, _dbl, _vp, _i64, _i64, _dbl, _dbl,
                      _vp, _i64, _vp],

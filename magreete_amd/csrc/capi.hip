@@ -240,6 +240,14 @@ int mg_profile_read(mg_handle* h, void* stream, long long* launches, double* flo
   return mg::profile_read(h->ws, reinterpret_cast<hipStream_t>(stream), launches, flops, ms);
 }
 
+// As mg_profile_read, per kernel class: element 0 the launches whose
+// operands are staged through registers, element 1 those with operands DMA'd
+// to LDS (arrays of two).
+int mg_profile_read_classes(mg_handle* h, void* stream, long long* launches, double* flops, double* ms) {
+  if (!h || !launches || !flops || !ms) return MG_ERR_ARG;
+  return mg::profile_read(h->ws, reinterpret_cast<hipStream_t>(stream), launches, flops, ms, true);
+}
+
 // dst = (precision of dst) src for a rows x cols block; dir 0: c128 -> c64, 1: c64 -> c128.
 int mg_convert(int dir, const void* src, int64_t lds, void* dst, int64_t ldd, int64_t rows, int64_t cols,
                void* stream) {

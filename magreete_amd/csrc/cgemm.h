@@ -261,7 +261,7 @@ __device__ inline void b_store(GemmSmem<T, BM, BN, BK, NP>& s, const BRegs<T, BN
 // (Gauss: t1 = Ar Br, t2 = Ai Bi, t3 = (Ar + Ai)(Br + Bi); Cr = t1 - t2,
 // Ci = t3 - t1 - t2) instead of four, at the cost of a third accumulator.
 template <typename T, int BM, int BN, int BK, int WM, int WN, typename ASrc, bool G3 = false>
-__device__ inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx<T>::t alpha, const ASrc& asrc,
+__device__ __attribute__((always_inline)) inline void cgemm_body(int64_t M, int64_t N, int64_t K, typename Cplx<T>::t alpha, const ASrc& asrc,
                                   const void* Bp, int64_t ldb, typename Cplx<T>::t beta, void* Cp, int64_t ldc,
                                   int tm, int tn) {
   typedef typename Cplx<T>::t T2;

@@ -1,6 +1,8 @@
 // Complex GEMM launchers (FP64 and FP32 MFMA) and their C-ABI entry.
 #include "cgemm.h"
+#include "zgemm_dma.h"
 #include "mg_internal.h"
+#include <cstdlib>
 
 namespace {
 
@@ -31,6 +33,29 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_trail(int64_t M, int64_t
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
   cgemm_body<T, BM, BN, BK, WM, WN, MemA, G3>(M, N, K, alpha, a, B, ldb, beta, C, ldc, tm, tn);
+}
+
+// FP64 GEMM with DMA-staged operands (zgemm_dma.h); blockIdx.y walks the
+// batch as in k_cgemm.  The _trail symbol serves the LU trailing update.
+template <int BN>
+__global__ __launch_bounds__(mg_zdma::NT) void k_zgemm_dma(int64_t M, int64_t N, int64_t K, double2 alpha,
+                                                          const double2* A, int64_t lda, int64_t sA,
+                                                          const double2* B, int64_t ldb, int64_t sB, double2 beta,
+                                                          double2* C, int64_t ldc, int64_t sC, int nbm, int nbn) {
+  int tm, tn;
+  mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
+  const int64_t b = blockIdx.y;
+  mg_zdma::zgemm_tile<BN>(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
+}
+
+template <int BN>
+__global__ __launch_bounds__(mg_zdma::NT) void k_zgemm_dma_trail(int64_t M, int64_t N, int64_t K, double2 alpha,
+                                                                const double2* A, int64_t lda, const double2* B,
+                                                                int64_t ldb, double2 beta, double2* C, int64_t ldc,
+                                                                int nbm, int nbn) {
+  int tm, tn;
+  mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
+  mg_zdma::zgemm_tile<BN>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
 }
 
 template <typename T>
@@ -66,11 +91,52 @@ int launch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda
   return MG_OK;
 }
 
+// MG_ZGEMM_DMA=0 selects the register-staged FP64 kernels instead (A/B timing)
+inline bool zdma_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("MG_ZGEMM_DMA");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
+
+// FP64 with DMA-staged operands: 128 x 64 tiles, three stages (one workgroup per CU)
+template <bool TRAIL>
+int launch_zdma(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+                int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
+  constexpr int BN = 64;
+  const int nbm = (int)((M + mg_zdma::BM - 1) / mg_zdma::BM), nbn = (int)((N + BN - 1) / BN);
+  const double2 al = to_t2<double>(alpha), be = to_t2<double>(beta);
+  const double2* a = reinterpret_cast<const double2*>(A);
+  const double2* b = reinterpret_cast<const double2*>(B);
+  double2* c = reinterpret_cast<double2*>(C);
+  if (TRAIL)
+    hipLaunchKernelGGL((k_zgemm_dma_trail<BN>), dim3(nbm * nbn), dim3(mg_zdma::NT), 0, s, M, N, K, al, a, lda, b, ldb,
+                       be, c, ldc, nbm, nbn);
+  else
+    hipLaunchKernelGGL((k_zgemm_dma<BN>), dim3(nbm * nbn, (unsigned)batch), dim3(mg_zdma::NT), 0, s, M, N, K, al, a,
+                       lda, sA, b, ldb, sB, be, c, ldc, sC, nbm, nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+// The DMA-staged FP64 kernel takes products whose 128 x 64 tiles give every
+// CU one (it holds one workgroup per CU) and whose depth is a whole number
+// of k-tiles.
+bool zdma_shape(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64) * batch;
+  return zdma_enabled() && tiles >= (int64_t)mg::cu_count() && K > 0 && K % mg_zdma::BK == 0 && M < (1ll << 31) &&
+         N < (1ll << 31);
+}
+
 template <typename T, bool TRAIL>
 int dispatch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
              int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
   constexpr bool F64 = std::is_same<T, double>::value;
   const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64) * batch;
+  if (F64 && zdma_shape(M, N, K, batch) && !(alpha.re == 0 && alpha.im == 0))
+    return launch_zdma<TRAIL>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
   if (tiles >= (F64 ? 1 : 2) * (int64_t)mg::cu_count())
     return launch<T, TRAIL, 128, 64, 16, 4, 2, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
                                                      s);
@@ -127,6 +193,8 @@ int splitk_parts(int64_t M, int64_t N, int64_t K) {
 }  // namespace
 
 namespace mg {
+
+bool zgemm_uses_dma(int64_t M, int64_t N, int64_t K, int64_t batch) { return zdma_shape(M, N, K, batch); }
 
 int cu_count() {
   static int cache[64] = {0};

@@ -513,14 +513,18 @@ int launch_dma(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, 
 // 12032^2 x 256 measured 1.07 ms against 1.21 ms with the register-staged
 // 128 x 128 tile); BF16x6: 64 x 64 (its 64 x 128 tile's LDS would leave one
 // workgroup per CU, and its 128 x 128 tile spills registers).
+// whether a BF16x3 update of M x N takes the DMA-staged kernel
+inline bool ldu_update_uses_dma(int64_t M, int64_t N) {
+  return ((M + dma::BM - 1) / dma::BM) * ((N + dma::BN - 1) / dma::BN) >= (int64_t)mg::cu_count();
+}
+
 template <int NS>
 int ldu_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return MG_OK;
   if (src.n < 1 || src.n > kMaxSrc) return MG_ERR_ARG;
   if constexpr (NS == 2) {
     static_assert(kIB % dma::BM == 0 && kIB % dma::BN == 0, "tiles within the padded 256-row blocks");
-    const int64_t big = ((M + dma::BM - 1) / dma::BM) * ((N + dma::BN - 1) / dma::BN);
-    if (big >= (int64_t)mg::cu_count()) {
+    if (ldu_update_uses_dma(M, N)) {
       switch (src.n) {
         case 1: return dma::launch_dma<1>(M, N, src, C, ldc, s);
         case 2: return dma::launch_dma<2>(M, N, src, C, ldc, s);
@@ -559,13 +563,13 @@ constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
 template <int NS, int BM, int BN, int WM, int WN, bool FWD>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2))) void k_ldu_sweep(
     const u32x4* __restrict__ SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X, int64_t ldx, u32x4* XS,
-    int64_t n, int64_t nrhs, int nblk, int ct, int* ctr, int* next) {
+    int64_t n, int64_t nrhs, int nblk, int ct, int* ctr, int* next, int nnext) {
   typedef mg_ps::PTile<NS, BM, BN, WM, WN> Tile;
   constexpr int U = 2 * NS, KG = kIB / 8, KT = KG / 4;
   __shared__ typename Tile::S sm[2];
   __shared__ int s_ticket, s_ok;
-  if (blockIdx.x == 0)   // the next sweep's counters (that launch follows this one on the stream)
-    for (int e = threadIdx.x; e <= nblk; e += blockDim.x)
+  if (blockIdx.x == 0)   // the next sweep's whole counter set (that launch follows this one on the stream)
+    for (int e = threadIdx.x; e < nnext; e += blockDim.x)
       __hip_atomic_store(next + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
@@ -730,7 +734,7 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
   if (grid > 0x7fffffffLL) return MG_ERR_ARG;
   int* ctr = sc.take();
   hipLaunchKernelGGL((k_ldu_sweep<NS, BM, BN, WM, WN, FWD>), dim3((unsigned)grid), dim3(64 * WM * WN), 0, s, SP, np,
-                     Bin, ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr, sc.other());
+                     Bin, ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr, sc.other(), (int)sc.cap);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
@@ -908,10 +912,13 @@ int getrf_ldu(Workspace& ws, void* A, int64_t n, int64_t lda, void* dinv, void* 
       src.B[q] = spa12(kq) + (j0 - r1) * KG * U;
     }
     const double flops = 8.0 * (double)mr * (double)nc * (double)(cnt * nb);
-    return timed_launch(ws, flops, st, [&] {
-      return split_ns == 2 ? ldu_update<2>(mr, nc, src, Ap + i0 * lda + j0, lda, st)
-                           : ldu_update<3>(mr, nc, src, Ap + i0 * lda + j0, lda, st);
-    });
+    return timed_launch(
+        ws, flops, st,
+        [&] {
+          return split_ns == 2 ? ldu_update<2>(mr, nc, src, Ap + i0 * lda + j0, lda, st)
+                               : ldu_update<3>(mr, nc, src, Ap + i0 * lda + j0, lda, st);
+        },
+        (split_ns == 2 && ldu_update_uses_dma(mr, nc)) ? 1 : 0);
   };
   // split precisions: the D inverses pre-split as A operands (DS, after
   // SPUr) for the solves' block-diagonal products, once all are final on s

@@ -5,10 +5,15 @@
 
 namespace mg {
 
-constexpr int kLeaf = 32;              // widest leaf factored by the cooperative kernel
+constexpr int kLeaf = 32;              // widest leaf of the triangular solves
+constexpr int kGetf2MaxW = 256;        // widest panel factored by one cooperative leaf launch
 constexpr int kMaxGetf2Blocks = 256;   // workgroups of one leaf launch (<= one per CU)
 constexpr int kGetf2MinRows = 64;      // aim for at least this many rows per workgroup
-constexpr int kGetf2Lds = 48 * 1024;   // dynamic LDS per leaf workgroup (static use stays under 64 KB total)
+// dynamic LDS per leaf workgroup: small enough that a leaf workgroup shares its
+// CU with a trailing-update workgroup (the look-ahead runs both at once);
+// measured on the 12288-row factorisation: 138 ms at 48 KB, 158 ms at 96 KB,
+// 161 ms at 144 KB
+constexpr int kGetf2Lds = 48 * 1024;
 
 // Cross-workgroup exchange area of the leaf kernel (double-buffered by
 // column parity).  The barrier counters come in two sets used by alternate
@@ -19,8 +24,8 @@ constexpr int kGetf2Lds = 48 * 1024;   // dynamic LDS per leaf workgroup (static
 struct Getf2Work {
   double* cand_val = nullptr;   // [2][G] best |re|+|im| per workgroup
   int* cand_row = nullptr;      // [2][G] its row
-  void* cand_data = nullptr;    // [2][G][kLeaf] the candidate rows
-  void* rowj = nullptr;         // [2][kLeaf] row j before the swap
+  void* cand_data = nullptr;    // [2][G][kGetf2MaxW] the candidate rows
+  void* rowj = nullptr;         // [2][kGetf2MaxW] row j before the swap
   unsigned* bar = nullptr;      // the set this launch counts in (kernel argument); in the handle: both sets
   unsigned* bar_next = nullptr; // the set this launch clears (kernel argument)
   int* err = nullptr;           // sticky barrier-failure flag
@@ -33,6 +38,7 @@ struct UpdateProfile {
   bool enabled = false;
   std::vector<hipEvent_t> ev;   // start/stop pairs, reused
   std::vector<double> flops;    // per recorded launch
+  std::vector<int> cls;         // per recorded launch: 1 DMA-staged operands, 0 register-staged
   size_t used = 0;
 };
 
@@ -65,8 +71,10 @@ int timed_update(Workspace& ws, GemmFn fn, int64_t M, int64_t N, int64_t K, cd a
                  const void* B, int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s);
 // launch() on s, timed with HIP events on s when ws.prof is enabled (flops:
 // the launch's algorithmic count).
+// cls: the kernel class recorded with the launch (1: operands DMA'd to LDS,
+// 0: staged through registers).
 template <typename F>
-int timed_launch(Workspace& ws, double flops, hipStream_t s, F&& launch) {
+int timed_launch(Workspace& ws, double flops, hipStream_t s, F&& launch, int cls = 0) {
   UpdateProfile& pr = ws.prof;
   if (pr.enabled) {
     if (pr.used * 2 + 2 > pr.ev.size()) {
@@ -82,13 +90,20 @@ int timed_launch(Workspace& ws, double flops, hipStream_t s, F&& launch) {
   if (pr.enabled) {
     MG_HIP(hipEventRecord(pr.ev[pr.used * 2 + 1], s));
     if (pr.flops.size() <= pr.used) pr.flops.resize(pr.used + 1);
+    if (pr.cls.size() <= pr.used) pr.cls.resize(pr.used + 1);
     pr.flops[pr.used] = flops;
+    pr.cls[pr.used] = cls;
     ++pr.used;
   }
   return MG_OK;
 }
-// Sum of recorded update launches since the last read (synchronises the stream).
-int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms);
+// Sum of recorded update launches since the last read (synchronises the
+// stream); with per_class, arrays of two: class 0 (register-staged) and
+// class 1 (DMA-staged) separately.
+int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms,
+                 bool per_class = false);
+// Whether the FP64 GEMM of this shape runs the DMA-staged kernel (gemm.hip).
+bool zgemm_uses_dma(int64_t M, int64_t N, int64_t K, int64_t batch = 1);
 
 int workspace_init(Workspace& ws);
 void workspace_free(Workspace& ws);

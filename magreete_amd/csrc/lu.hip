@@ -122,7 +122,7 @@ __device__ inline void grid_barrier(unsigned* bar, unsigned it, int G, int* err,
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Leaf LU of an m x w panel (w <= kLeaf) at A (row-major, lda).  Workgroup g
+// Leaf LU of an m x w panel (w <= kGetf2MaxW) at A (row-major, lda).  Workgroup g
 // owns panel rows [g R, g R + R).  ipiv and info use global indices (row0 is
 // the panel's global row, which equals its global column).  The launch
 // counts its barriers in wk.bar and clears wk.bar_next for the next leaf.
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
   typedef typename C2<T>::t T2;
   extern __shared__ unsigned char smem_raw[];
   T2* tile = reinterpret_cast<T2*>(smem_raw);
-  __shared__ T2 prow[mg::kLeaf], orow[mg::kLeaf];
+  __shared__ T2 prow[mg::kGetf2MaxW], orow[mg::kGetf2MaxW];
   __shared__ double sv[kThreads / 64];
   __shared__ int sr[kThreads / 64];
 
@@ -167,8 +167,9 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
       pub_store(&wk.cand_val[par * G + g], best);
       pub_store(&wk.cand_row[par * G + g], brow);
     }
-    if (brow != INT_MAX && tid < w) pub_store2(&cdata[(par * G + g) * mg::kLeaf + tid], tile[(brow - rb) * w + tid]);
-    if (j >= rb && j < re && tid < w) pub_store2(&rowj[par * mg::kLeaf + tid], tile[(j - rb) * w + tid]);
+    if (brow != INT_MAX && tid < w)
+      pub_store2(&cdata[(par * G + g) * mg::kGetf2MaxW + tid], tile[(brow - rb) * w + tid]);
+    if (j >= rb && j < re && tid < w) pub_store2(&rowj[par * mg::kGetf2MaxW + tid], tile[(j - rb) * w + tid]);
     if (G > 1) grid_barrier(wk.bar, (unsigned)j, G, lerr, wk.err);
     else __syncthreads();
 
@@ -179,8 +180,8 @@ __global__ __launch_bounds__(kThreads) void k_getf2(typename C2<T>::t* __restric
     if (p == INT_MAX) p = j;   // whole column NaN: keep row j, NaN propagates
     const int pwg = p / R;
     if (tid < w) {
-      orow[tid] = pub_load2(&rowj[par * mg::kLeaf + tid]);
-      prow[tid] = (p == j) ? orow[tid] : pub_load2(&cdata[(par * G + pwg) * mg::kLeaf + tid]);
+      orow[tid] = pub_load2(&rowj[par * mg::kGetf2MaxW + tid]);
+      prow[tid] = (p == j) ? orow[tid] : pub_load2(&cdata[(par * G + pwg) * mg::kGetf2MaxW + tid]);
     }
     __syncthreads();
     if (tid < w) {
@@ -481,9 +482,10 @@ template <typename T>
 static int panel(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int64_t w, int* ipiv, int* info,
                  hipStream_t s) {
   typedef typename C2<T>::t T2;
-  // leaf width: the widest power of two up to kLeaf whose rows fit the
-  // LDS of the workgroups a leaf may use
-  int64_t lw = kLeaf;
+  // leaf width: the widest power of two up to kGetf2MaxW whose rows fit the
+  // LDS of the workgroups a leaf may use (the narrower the leaf, the more
+  // launches of swaps, triangular solves and GEMMs the recursion adds)
+  int64_t lw = kGetf2MaxW;
   while (lw > 1 && (int64_t)ws.coop_blocks * (kGetf2Lds / (lw * (int64_t)sizeof(T2))) < m) lw /= 2;
   if (w <= lw) return getf2<T>(ws, A, lda, k, m, w, ipiv, info, s);
   T2* Ap = reinterpret_cast<T2*>(A);
@@ -573,8 +575,8 @@ int workspace_init(Workspace& ws) {
   const int G = kMaxGetf2Blocks;
   MG_HIP(hipMalloc(&ws.getf2.cand_val, sizeof(double) * 2 * G));
   MG_HIP(hipMalloc(&ws.getf2.cand_row, sizeof(int) * 2 * G));
-  MG_HIP(hipMalloc(&ws.getf2.cand_data, sizeof(double2) * 2 * G * kLeaf));
-  MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kLeaf));
+  MG_HIP(hipMalloc(&ws.getf2.cand_data, sizeof(double2) * 2 * G * kGetf2MaxW));
+  MG_HIP(hipMalloc(&ws.getf2.rowj, sizeof(double2) * 2 * kGetf2MaxW));
   MG_HIP(hipMalloc(&ws.getf2.bar, sizeof(unsigned) * 2 * kBarWords));
   MG_HIP(hipMemset(ws.getf2.bar, 0, sizeof(unsigned) * 2 * kBarWords));
   MG_HIP(hipMalloc(&ws.inv_exch, sizeof(float2) * 2 * 256 * 32));
@@ -598,18 +600,24 @@ int workspace_init(Workspace& ws) {
   return MG_OK;
 }
 
-int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms) {
+int profile_read(Workspace& ws, hipStream_t s, long long* launches, double* flops, double* ms, bool per_class) {
   MG_HIP(hipStreamSynchronize(s));
-  double f = 0.0, t = 0.0;
+  double f[2] = {0.0, 0.0}, t[2] = {0.0, 0.0};
+  long long c[2] = {0, 0};
   for (size_t i = 0; i < ws.prof.used; ++i) {
     float e = 0.f;
     MG_HIP(hipEventElapsedTime(&e, ws.prof.ev[2 * i], ws.prof.ev[2 * i + 1]));
-    t += e;
-    f += ws.prof.flops[i];
+    const int k = ws.prof.cls[i] ? 1 : 0;
+    t[k] += e;
+    f[k] += ws.prof.flops[i];
+    ++c[k];
   }
-  if (launches) *launches = (long long)ws.prof.used;
-  if (flops) *flops = f;
-  if (ms) *ms = t;
+  for (int k = 0; k < (per_class ? 2 : 1); ++k) {
+    const int q = per_class ? k : 0;
+    if (launches) launches[q] = per_class ? c[k] : c[0] + c[1];
+    if (flops) flops[q] = per_class ? f[k] : f[0] + f[1];
+    if (ms) ms[q] = per_class ? t[k] : t[0] + t[1];
+  }
   ws.prof.used = 0;
   return MG_OK;
 }
@@ -643,8 +651,10 @@ void workspace_free(Workspace& ws) {
 
 int timed_update(Workspace& ws, GemmFn fn, int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda,
                  const void* B, int64_t ldb, cd beta, void* C, int64_t ldc, hipStream_t s) {
-  return timed_launch(ws, 8.0 * (double)M * (double)N * (double)K, s,
-                      [&] { return fn(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); });
+  const int cls = (fn == &cgemm_trail<double> && zgemm_uses_dma(M, N, K)) ? 1 : 0;
+  return timed_launch(
+      ws, 8.0 * (double)M * (double)N * (double)K, s, [&] { return fn(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s); },
+      cls);
 }
 
 int workspace_leaf_blocks(Workspace& ws, int g) {

@@ -303,6 +303,18 @@ def read_update_profile(device):
     return n.value, f.value, ms.value
 
 
+def read_update_profile_classes(device):
+    """As read_update_profile, per kernel class: {"reg": (launches, flops, ms),
+    "dma": (...)} -- operands staged through registers, or DMA'd to LDS."""
+    n = (ctypes.c_longlong * 2)()
+    f = (ctypes.c_double * 2)()
+    ms = (ctypes.c_double * 2)()
+    with torch.cuda.device(device):
+        L.check(L.load().mg_profile_read_classes(handle(device), _stream(device), n, f, ms),
+                "mg_profile_read_classes")
+    return {"reg": (n[0], f[0], ms[0]), "dma": (n[1], f[1], ms[1])}
+
+
 def check_barriers(device):
     """Synchronises the current stream and raises if a grid barrier of the
     LU leaf on this thread's handle timed out since the last check (the

@@ -40,6 +40,32 @@ def test_cgemm(prec, mnk):
     assert err < {"c128": 1e-14, "c64": 2e-6, "bf16x3": 5e-5, "bf16x6": 2e-6}[prec], err
 
 
+@pytest.mark.parametrize("mnk", [(4200, 4300, 96), (4099, 4113, 64), (12288, 256, 512), (2100, 4113, 1024)])
+@pytest.mark.parametrize("ld_pad", [0, 24])
+def test_cgemm_fp64_dma_shapes(mnk, ld_pad):
+    """Shapes that take the FP64 kernel with DMA-staged operands (tiles for
+    every CU, depth a multiple of 16), with ragged edges and, with ld_pad,
+    operands and C as device views whose leading dimension exceeds their
+    columns."""
+    m, n, k = mnk
+    say("cgemm dma", mnk, ld_pad)
+    rng = np.random.default_rng(sum(mnk) + ld_pad)
+    Af = torch.from_numpy(crand((m, k + ld_pad), rng)).to(DEV)
+    Bf = torch.from_numpy(crand((k, n + ld_pad), rng)).to(DEV)
+    Cf = torch.from_numpy(crand((m, n + ld_pad), rng)).to(DEV)
+    o = ld_pad // 2
+    Ad, Bd, Cd = Af[:, o:o + k], Bf[:, :n], Cf[:, ld_pad:]
+    A, B, C = Ad.cpu().numpy(), Bd.cpu().numpy(), Cd.cpu().numpy()
+    outside = Cf[:, :ld_pad].clone()
+    alpha, beta = 0.7 - 0.3j, -1.1 + 0.4j
+    ref = alpha * (A @ B) + beta * C
+    E.cgemm(Ad, Bd, Cd, alpha, beta, "c128")
+    got = Cd.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-14, err
+    assert torch.equal(Cf[:, :ld_pad], outside)   # nothing written outside the view
+
+
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 @pytest.mark.parametrize("mnk", [(3000, 40, 4096), (2000, 70, 1500), (12288 // 4, 256, 3072)])
 def test_cgemm_split_k(prec, mnk):
@@ -155,7 +181,7 @@ def test_system_matrix_vs_oracle(kind):
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-@pytest.mark.parametrize("n", [1, 7, 33, 100, 257, 600, 1100])
+@pytest.mark.parametrize("n", [1, 7, 33, 100, 257, 600, 1100, 4100])
 def test_getrf_getrs(prec, n):
     say("lu", prec, n)
     rng = np.random.default_rng(n)
