@@ -44,11 +44,15 @@ PRODUCT_KERNEL = ("k_zgemm_dma<64> + k_splitk_reduce<double> (the refinement's F
                   "complex MFMA)")
 # HBM bytes per launch (average over one solve of this workload), measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE on this script with --copy-probe 2 and scaled by the counters' reading for
-# that 2 GiB copy (fetch x2.000, write x1.000); None where not measured
-TRAFFIC = {("fp64", "dma"): None, ("fp64", "reg"): None, ("c64", "reg"): 4.410e8,
-           ("bf16x3", "dma"): None, ("bf16x3", "reg"): None, ("bf16x6", "reg"): 2.709e8, "product": None}
-# algorithmic HBM bytes per launch, for comparison with TRAFFIC: the FP64 product reads M (16 n^2 B) and
-# Z and writes the two K-split partials and their sum (16 n T B each way, ~3 times)
+# that 2 GiB copy (fetch x2.000, write x1.000), beside the algorithmic bytes of the same launches: C read
+# and written once plus each operand panel read once (pivoted LU: 16 B per complex128 element each way;
+# block LDU: 8 B per complex64 element each way, 8 B per pre-split BF16x3 panel element and k); the
+# product: M and Z read once, the two K-split partials written and read once, the result written once.
+TRAFFIC = {("fp64", "dma"): 2.192e9, ("fp64", "reg"): 6.44e7, ("c64", "reg"): 4.410e8,
+           ("bf16x3", "dma"): 1.171e9, ("bf16x3", "reg"): 3.85e7, ("bf16x6", "reg"): 2.709e8, "product": 4.179e9}
+ALGO_BYTES = {("fp64", "dma"): 1.345e9, ("fp64", "reg"): 4.93e7, ("bf16x3", "dma"): 6.7e8, "product": 2.718e9}
+
+
 def colerr(x, ref):
     x = x.reshape(-1, x.shape[-1])
     ref = ref.reshape(-1, ref.shape[-1])
@@ -229,7 +233,8 @@ def main():
             pk = PEAK_CLASS[(base, k)]
             out["update_" + k] = {
                 "bound": "mfma", "achieved": u["tflops"], "peak": pk, "unit": "TFLOP/s",
-                "frac": u["tflops"] / pk, "traffic": TRAFFIC.get((base, k)), "kernel": KERNEL[base][k],
+                "frac": u["tflops"] / pk, "traffic": TRAFFIC.get((base, k)),
+                "algorithmic_bytes": ALGO_BYTES.get((base, k)), "kernel": KERNEL[base][k],
                 "launches_per_step": u["launches_per_step"], "ms_per_step": u["ms_per_step"]}
         ph = r.get("phases") or (results.get(best_candidate(args), {}).get("phases") or {})
         prod = ph.get("fp64_product")
@@ -345,7 +350,7 @@ def phase_rooflines(s, prob, args):
         out["fp64_product"] = {"bound": "mfma", "achieved": fl / (ms * 1e-3) / 1e12, "peak": PEAK_Z3,
                                "unit": "TFLOP/s", "frac": fl / (ms * 1e-3) / (PEAK_Z3 * 1e12), "ms": ms,
                                "flops": fl, "traffic": TRAFFIC.get("product"),
-                               "algorithmic_bytes": 16.0 * n * n + 16.0 * n * T * 4,
+                               "algorithmic_bytes": 16.0 * n * n + 16.0 * n * T * 6,
                                "kernel": PRODUCT_KERNEL}
         del Z, Wz
     if s.method in ("bf16x3", "bf16x6"):

@@ -39,17 +39,28 @@ __device__ inline v4f mg_mfma_nega(float a, float b, v4f c) {
 // works on are contiguous groups of rows: ids are dealt round-robin over the
 // 8 XCDs, so undo that first (for the leading multiple of 8 ids; the few
 // left over keep their place), then walk the tile grid in row groups of 8.
+// When an XCD's share is a whole number of tile rows, the XCD takes exactly
+// those rows (every tile of a row block on one L2: a 12288 x 256 product's
+// 96 x 4 tiles read M once from HBM instead of twice).
 __device__ inline void mg_tile_order(int bid, int nbm, int nbn, int& tm, int& tn) {
   const int total = nbm * nbn;
-  int id = bid;
   const int per = total / 8;
-  if (total >= 64 && bid < 8 * per) id = (bid % 8) * per + bid / 8;
+  int id = bid, row0 = 0, rows = nbm;
+  if (total >= 64 && bid < 8 * per) {
+    if (per % nbn == 0) {
+      rows = per / nbn;
+      row0 = (bid % 8) * rows;
+      id = bid / 8;
+    } else {
+      id = (bid % 8) * per + bid / 8;
+    }
+  }
   const int group = 8;
   const int in_group = group * nbn;
   const int g = id / in_group;
   const int first_m = g * group;
-  const int gsize = min(nbm - first_m, group);
-  tm = first_m + (id % in_group) % gsize;
+  const int gsize = min(rows - first_m, group);
+  tm = row0 + first_m + (id % in_group) % gsize;
   tn = (id % in_group) / gsize;
 }
 

@@ -6,7 +6,12 @@
 namespace mg {
 
 constexpr int kLeaf = 32;              // widest leaf of the triangular solves
-constexpr int kGetf2MaxW = 256;        // widest panel factored by one cooperative leaf launch
+// widest panel factored by one cooperative leaf launch; leaves of 64 to 256
+// columns (fewer swap / triangular-solve / GEMM launches in the panel) were
+// measured on the 12288-row factorisation: the same 138 ms factor, and the
+// longer leaf launches slowed the trailing update they share CUs with
+// (solve 160 ms against 158 ms)
+constexpr int kGetf2MaxW = 32;
 constexpr int kMaxGetf2Blocks = 256;   // workgroups of one leaf launch (<= one per CU)
 constexpr int kGetf2MinRows = 64;      // aim for at least this many rows per workgroup
 // dynamic LDS per leaf workgroup: small enough that a leaf workgroup shares its
