@@ -514,9 +514,17 @@ def run_serving(args, dev):
     cap = args.serving / (time.perf_counter() - t0)
     out = {"requests_per_level": args.serving, "capacity_req_per_s": cap, "workers": 4, "hbm_budget_gb": 200}
     for f in (0.5, 0.8, 1.1):
+        t0 = time.perf_counter()
         res = srv.run(S.make_trace(args.serving, cap * f, seed=args.seed + 8))
+        dt = time.perf_counter() - t0
         lvl = S.percentiles(res)
-        lvl.update({"offered_req_per_s": cap * f, "hits": sum(r["hit"] for r in res)})
+        # achieved throughput over the level's wall time, and every result's FP64 residual b - M x
+        # (relative, largest column; the refinement's tolerance is --batch-tol)
+        resid = [r["stats"]["residual"] for r in res if "residual" in r["stats"]]
+        lvl.update({"offered_req_per_s": cap * f, "achieved_req_per_s": len(res) / dt,
+                    "achieved_solves_per_s": sum(r["T"] for r in res) / dt, "hits": sum(r["hit"] for r in res),
+                    "max_residual": max(resid) if resid else None,
+                    "fallbacks": sum(1 for r in res if r["stats"].get("fallback"))})
         out["load_%.1fx" % f] = lvl
         print("# serving %.1fx: %s" % (f, lvl), file=sys.stderr, flush=True)
     return out

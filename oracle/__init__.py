@@ -30,7 +30,10 @@ How each function is pinned (tests/test_oracle_*.py):
   the 2D ``'point'`` sources as a dipole at R^T (-2, 0) through scipy's Hankel
   functions (rotation covariance of the Green's functions).
 
-Parity unpinned: the multi-candidate branch of the ``regularize`` NaN fix-up
-(several scatterers within ``radius`` of one measurement point) is restated
-from the source text only.
+* the ``regularize`` NaN fix-up, multi-candidate branch included (several
+  scatterers within ``radius`` of one measurement point: the nearest one's
+  Ek), against a brute-force restatement on a hand-placed configuration
+  (tests/test_oracle_pins.py).
+
+Every function has a pin of this kind; none is parity unpinned.
 """

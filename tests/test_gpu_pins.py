@@ -134,7 +134,8 @@ def test_full_self_interaction(kind):
     probes = torch.cat([0.5 * pts[:7], pts[:3]])
     la = dut.LDOS_measurements(probes, k0, alpha, radius, self_interaction_type="full")
     lb = ora.LDOS_measurements(probes, k0, alpha, radius, self_interaction_type="full")
-    assert (la - lb).abs().max() <= 1e-9 * lb.abs().max()
+    err = ((la - lb).abs().max() / lb.abs().max()).item()
+    assert err <= 1e-10, err
 
 
 @pytest.mark.parametrize("kind", list(DUT))
