@@ -2,6 +2,7 @@
 #include "cgemm.h"
 #include "zgemm_dma.h"
 #include "mg_internal.h"
+#include <cstdint>
 #include <cstdlib>
 
 namespace {
@@ -135,7 +136,10 @@ int dispatch(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t l
              int64_t ldb, int64_t sB, cd beta, void* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
   constexpr bool F64 = std::is_same<T, double>::value;
   const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64) * batch;
-  if (F64 && zdma_shape(M, N, K, batch) && !(alpha.re == 0 && alpha.im == 0))
+  // the DMA loads move 16-byte elements: operands at an address that is not a multiple of 16 (a
+  // caller's pointer into the middle of an element) take the register-staged kernel
+  const bool aligned = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
+  if (F64 && aligned && zdma_shape(M, N, K, batch) && !(alpha.re == 0 && alpha.im == 0))
     return launch_zdma<TRAIL>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
   if (tiles >= (F64 ? 1 : 2) * (int64_t)mg::cu_count())
     return launch<T, TRAIL, 128, 64, 16, 4, 2, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch,
