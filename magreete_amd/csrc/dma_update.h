@@ -8,7 +8,8 @@
 // 8 waves of 32 x 64 (two 16-row fragments by four 16-column fragments).
 // In LDS the 16 units of a row are XOR-swizzled by a permutation of the
 // row's low bits (swz) so that the reads of one fragment hit distinct banks.
-// The accumulators start from -C, loaded behind the first k-tile's operands;
+// The accumulators start from -C, loaded behind the first k-tile's operands,
+// or (LATE) from zero with C copied to LDS by DMA during the last k-tiles;
 // the k-loop is unrolled (the tile count is a template parameter) so that
 // the compiler can tell the two stages' DMA apart and waits only where the
 // loop asks for it.
@@ -116,6 +117,22 @@ __device__ inline void compute(const Stage& st, v4f (&cr)[TM][TN], v4f (&ci)[TM]
 constexpr unsigned vm_wait(int v) { return (unsigned)((v & 0xF) | ((v >> 4) << 14) | (0x7 << 4) | (0xF << 8)); }
 constexpr unsigned kLdsWait = 0xC07F;   // lgkmcnt(0)
 
+// Rows [64 h, 64 h + 64) of the C tile into stage st (row-major, 128
+// complex floats per row): one DMA instruction per row, 8 per wave.  Rows
+// past M read row M - 1 and units past N read column 0 (never used).  Needs
+// 16-byte aligned units: C and ldc even in complex floats, N even.
+__device__ inline void load_c_half(Stage& st, const float2* C, int64_t ldc, int64_t M, int64_t N, int h) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float2* L = reinterpret_cast<float2*>(&st);
+  const int64_t c = 2 * lane, cc = c < N ? c : 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int r = wave * 8 + i;
+    const int64_t gr = 64 * h + r, rr = gr < M ? gr : M - 1;
+    __builtin_amdgcn_global_load_lds((gvp)(C + rr * ldc + cc), (lvp)&L[r * BN], 16, 0, 0);
+  }
+}
+
 // C -= sum over k-tiles t < T of the tile's products; src(t, stage) issues
 // k-tile t's DMA (load_tile).  C points at the tile's origin (row-major,
 // ldc); M x N: the valid extent from there (the whole tile when both are at
@@ -124,7 +141,15 @@ constexpr unsigned kLdsWait = 0xC07F;   // lgkmcnt(0)
 // launch's descriptor table) would otherwise give flat accesses, which also
 // count against the LDS counter, so the k-loop's waits for its LDS reads
 // would wait on them too (measured: 199 against 305 TFLOP/s per launch).
-template <int T, typename Src>
+//
+// LATE = false: the accumulators start from -C, loaded behind the first
+// k-tile's operands, and C = -acc at the end.  LATE = true: they start from
+// zero, and C is copied by DMA into each stage as soon as its last k-tile has
+// been read (the first half during the last k-tile's products), then
+// C = C - acc from LDS: the tile's C read no longer holds up its first
+// products (standalone, 12288^2 x 512: the C read and write took a third of
+// the launch).  LATE needs load_c_half's alignment.
+template <int T, bool LATE = false, typename Src>
 __device__ inline void tile_update(Stage& s0, Stage& s1, Src src, float2* __restrict__ Cflat, int64_t ldc, int64_t M,
                                    int64_t N) {
   static_assert(T % 2 == 0 && T > 0, "k-tiles come in pairs (one per stage)");
@@ -146,9 +171,11 @@ __device__ inline void tile_update(Stage& s0, Stage& s1, Src src, float2* __rest
     for (int b = 0; b < TN; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = rbase + a * 16 + r, col = cbase + b * 16;
         float2 c = make_float2(0.f, 0.f);
-        if (whole || (row < M && col < N)) c = C[row * ldc + col];
+        if (!LATE) {
+          const int64_t row = rbase + a * 16 + r, col = cbase + b * 16;
+          if (whole || (row < M && col < N)) c = C[row * ldc + col];
+        }
         cr[a][b][r] = -c.x;
         ci[a][b][r] = -c.y;
       }
@@ -164,6 +191,9 @@ __device__ inline void tile_update(Stage& s0, Stage& s1, Src src, float2* __rest
     if (t + 2 < T) {
       src(t + 2, s0);
       __builtin_amdgcn_s_waitcnt(vm_wait(8));   // k-tile t + 1 has landed
+    } else if (LATE) {
+      load_c_half(s0, Cflat, ldc, M, N, 0);
+      __builtin_amdgcn_s_waitcnt(vm_wait(8));   // k-tile t + 1 has landed (C's first half may be in flight)
     } else {
       __builtin_amdgcn_s_waitcnt(vm_wait(0));
     }
@@ -172,6 +202,24 @@ __device__ inline void tile_update(Stage& s0, Stage& s1, Src src, float2* __rest
     __builtin_amdgcn_s_waitcnt(kLdsWait);
     __builtin_amdgcn_s_barrier();
     if (t + 3 < T) src(t + 3, s1);
+  }
+  if (LATE) {
+    load_c_half(s1, Cflat, ldc, M, N, 1);
+    __builtin_amdgcn_s_waitcnt(vm_wait(0));
+    __builtin_amdgcn_s_barrier();
+    const float2* L = reinterpret_cast<const float2*>(wm < 2 ? &s0 : &s1);
+    const int64_t lbase = rbase - (wm < 2 ? 0 : 64);
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = rbase + a * 16 + r, col = cbase + b * 16;
+          const float2 c = L[(lbase + a * 16 + r) * BN + col];
+          if (whole || (row < M && col < N)) C[row * ldc + col] = make_float2(c.x - cr[a][b][r], c.y - ci[a][b][r]);
+        }
+    return;
   }
 #pragma unroll
   for (int a = 0; a < TM; ++a)

@@ -476,15 +476,16 @@ int launch_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ld
 
 // ---- BF16x3 trailing update for large C: operand tiles copied to LDS by DMA --
 // 128 x 128 tiles (dma_update.h: the same per-wave MFMA work and accumulation
-// order as k_ldu_update, hence the same bits), the k-loop over NSRC steps of
-// 8 k-tiles.  Standalone, 12032^2 x 256: 1.03 ms against 1.19 for
-// k_ldu_update; x 512: 1.67 against 1.70 (the same bits).
+// order as k_ldu_update; C enters the sum first, as there, or last, in the
+// LATE form), the k-loop over NSRC steps of 8 k-tiles.  Standalone,
+// 12032^2 x 256 with C first: 1.03 ms against 1.19 for k_ldu_update; x 512:
+// 1.67 against 1.70 (the same bits).
 namespace dma {
 using mg_dma::BM;
 using mg_dma::BN;
 using mg_dma::U;
 
-template <int NSRC>
+template <int NSRC, bool LATE>
 __global__ __launch_bounds__(mg_dma::NT) void k_ldu_update_dma(int64_t M, int64_t N, UpdSrc src,
                                                               float2* __restrict__ C, int64_t ldc, int nbm, int nbn) {
   __shared__ mg_dma::Stage s0, s1;
@@ -496,13 +497,22 @@ __global__ __launch_bounds__(mg_dma::NT) void k_ldu_update_dma(int64_t M, int64_
     const int q = t / (KG / 4), g0 = (t % (KG / 4)) * 4;
     mg_dma::load_tile(st, src.A[q] + i0 * KG * U, src.B[q] + j0 * KG * U, KG, g0);
   };
-  mg_dma::tile_update<T>(s0, s1, tile, C + i0 * ldc + j0, ldc, M - i0, N - j0);
+  mg_dma::tile_update<T, LATE>(s0, s1, tile, C + i0 * ldc + j0, ldc, M - i0, N - j0);
 }
 
+// C is read late (by DMA, tile_update's LATE form) when its 16-byte units
+// are aligned: standalone 12288^2 x 512, 1.59 against 1.70 ms; x 256, 0.99
+// against 1.13 ms.
 template <int NSRC>
 int launch_dma(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
   const int nbm = (int)((M + BM - 1) / BM), nbn = (int)((N + BN - 1) / BN);
-  hipLaunchKernelGGL(k_ldu_update_dma<NSRC>, dim3(nbm * nbn), dim3(mg_dma::NT), 0, s, M, N, src, C, ldc, nbm, nbn);
+  const bool late = (reinterpret_cast<uintptr_t>(C) & 15) == 0 && ldc % 2 == 0 && N % 2 == 0;
+  if (late)
+    hipLaunchKernelGGL((k_ldu_update_dma<NSRC, true>), dim3(nbm * nbn), dim3(mg_dma::NT), 0, s, M, N, src, C, ldc,
+                       nbm, nbn);
+  else
+    hipLaunchKernelGGL((k_ldu_update_dma<NSRC, false>), dim3(nbm * nbn), dim3(mg_dma::NT), 0, s, M, N, src, C, ldc,
+                       nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }

@@ -44,7 +44,7 @@ def test_ldu_factor_solve(n, prec):
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "bf16x6"])
-@pytest.mark.parametrize("n,nrhs", [(300, 1), (1100, 130), (2900, 300), (4100, 256)])
+@pytest.mark.parametrize("n,nrhs", [(300, 1), (1100, 130), (2900, 300), (2901, 64), (4100, 256)])
 def test_ldu_sweep_solve(n, nrhs, prec):
     """The split-bf16 LDU solves run each triangular sweep as one dataflow
     launch: ragged last blocks, right-hand sides narrower or wider than a
