@@ -698,6 +698,250 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
     tile.store(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
 
+// The BF16x3 sweep with its operand k-tiles copied to LDS by DMA through
+// NSTG stages (k_ldu_sweep stages one k-tile ahead through registers, so
+// every k-tile waited about one memory latency; the chain of block rows is
+// made of such waits).  The same tiles (64 x 32, four waves of 32 x 16),
+// tickets, counters and stores as k_ldu_sweep.  x_j's units are read with
+// agent-scope (sc1) DMA: other workgroups of the launch wrote them.  Every
+// k-tile is fetched two ahead of its product; the counters of up to 8 block
+// rows are read at once (a counter load waits for the wave's earlier loads,
+// the DMA in flight included), when the first k-tile of a block row not yet
+// known complete is to be fetched.
+namespace sdma {
+constexpr int BM = 64, BN = 32, NT = 256, UPR = 16, DPT = 6;   // DPT: DMA instructions per wave and k-tile
+struct Stage {
+  u32x4 a[BM * UPR];
+  u32x4 b[BN * UPR];
+};
+using mg_dma::gvp;
+using mg_dma::kLdsWait;
+using mg_dma::lvp;
+using mg_dma::swz;
+using mg_dma::vm_wait;
+
+// One 16-byte-per-lane DMA (global_load_lds_dwordx4) of the wave's 64
+// lanes into LDS at dst + 16 lane; SC1: agent-scope coherent read.  Written
+// as inline assembly so that the compiler does not track it: it cannot tell
+// which stage a DMA writes when the stage rotates in a run-time loop, and
+// then waits for every DMA in flight before any LDS read, which undoes the
+// prefetch.  The loop's own waits (vm_wait before the barrier) order the
+// DMA and the reads; the instruction has no register results, and it reads
+// its address registers when it issues.  One wait state separates the M0
+// write from the instruction that reads it.
+template <bool SC1>
+__device__ inline void dma16(const u32x4* src, u32x4* dst) {
+  const unsigned lds =
+      __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)dst);
+  if constexpr (SC1)
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off sc1" ::"v"(src), "s"(lds)
+                 : "memory", "m0");
+  else
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds)
+                 : "memory", "m0");
+}
+
+// acc += the k-tile in st; wave (wm, wn) holds rows 32 wm + [0, 32) and
+// columns 16 wn + [0, 16) (the accumulator layout of PTile<2, 64, 32, 2, 2>).
+__device__ inline void compute(const Stage& st, v4f (&cr)[2][1], v4f (&ci)[2][1]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1, li = lane & 15, lg = lane >> 4;
+  const int sl = swz(li);
+  bf16x8 ar[2][2], ai[2][2], br[2], bi[2], bn[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int row = wm * 32 + a * 16 + li;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      ar[a][q] = mg_dma::unit(&st.a[row * UPR + ((lg * 4 + q) ^ sl)]);
+      ai[a][q] = mg_dma::unit(&st.a[row * UPR + ((lg * 4 + 2 + q) ^ sl)]);
+    }
+  }
+  const int col = wn * 16 + li;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    br[q] = mg_dma::unit(&st.b[col * UPR + ((lg * 4 + q) ^ sl)]);
+    bi[q] = mg_dma::unit(&st.b[col * UPR + ((lg * 4 + 2 + q) ^ sl)]);
+    bn[q] = mg_neg8(bi[q]);
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q + p < 2; ++q) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) cr[a][0] = mg_mfma_bf16(ar[a][p], br[q], cr[a][0]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) ci[a][0] = mg_mfma_bf16(ar[a][p], bi[q], ci[a][0]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) cr[a][0] = mg_mfma_bf16(ai[a][p], bn[q], cr[a][0]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) ci[a][0] = mg_mfma_bf16(ai[a][p], br[q], ci[a][0]);
+    }
+}
+
+template <int NSTG, bool FWD>
+__global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ SP, int64_t np, const float2* Bin,
+                                                      int64_t ldbin, float2* X, int64_t ldx, u32x4* XS, int64_t n,
+                                                      int64_t nrhs, int nblk, int ct, int* ctr, int* next,
+                                                      int nnext) {
+  static_assert(NSTG == 3, "three stages");
+  typedef mg_ps::PTile<2, BM, BN, 2, 2> Tile;
+  constexpr int U = 4, KG = kIB / 8, KT = KG / 4;
+  // three stages, each named at every DMA and read (the loop is unrolled by three)
+  __shared__ Stage sm0, sm1, sm2;
+  __shared__ int s_ticket, s_cnt;
+  if (blockIdx.x == 0)   // the next sweep's whole counter set (that launch follows this one on the stream)
+    for (int e = threadIdx.x; e < nnext; e += blockDim.x)
+      __hip_atomic_store(next + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int tpr = (kIB / BM) * ct;   // tiles per block row
+  const int blk = s_ticket / tpr, w = s_ticket % tpr;
+  const int i = FWD ? blk : nblk - 1 - blk;
+  int* done = ctr + 1;
+  const int64_t i0 = (int64_t)i * kIB + (int64_t)(w / ct) * BM, j0 = (int64_t)(w % ct) * BN;
+  const int64_t M = n - i0, N = nrhs - j0;   // valid extents (M <= 0: a row tile past a short last block)
+  const int64_t ldgx = np / 8;
+  const SplitLayout lay{np, nblk, U};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  Tile tile;
+  tile.init(Bin + i0 * ldbin + j0, ldbin, M, N, make_float2(-1.f, 0.f), M > 0);
+  // the start values have landed before the loop: the compiler does not see
+  // the loop's DMA, and would otherwise wait for all of it (vmcnt(0)) where
+  // it first uses them
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(vm_wait(0));
+  bool ok = true;
+  const int steps = M > 0 ? (FWD ? i : nblk - 1 - i) : 0;
+  const int Q = steps * KT;
+  auto row_of_step = [&](int st) { return FWD ? st : nblk - 1 - st; };
+  // Thread 0 reads the counters of the block rows of steps s .. s+7 (those
+  // that exist) until the leading complete ones are at least one; returns
+  // their count, -1 when the wait timed out.
+  auto poll = [&](int s) {
+    if (threadIdx.x == 0) {
+      const int nb = min(8, steps - s);
+      long long spins = 0;
+      int c = 0;
+      while (true) {
+        int v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+          v[b] = __hip_atomic_load(&done[row_of_step(min(s + b, steps - 1))], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        // every loaded value is used (a branch-free count): a load left in
+        // flight would be waited for later by the compiler, which does not
+        // see the DMA and so would wait for all of it
+        c = 0;
+        bool run = true;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          run = run && b < nb && v[b] >= tpr;
+          c += run ? 1 : 0;
+        }
+        if (c > 0) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSweepSpin) {
+          c = -1;
+          break;
+        }
+      }
+      s_cnt = c;
+    }
+    __builtin_amdgcn_s_waitcnt(kLdsWait);
+    __builtin_amdgcn_s_barrier();
+    const int c = s_cnt;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return c;
+  };
+  // k-tile q (step q / KT) into stage sg
+  auto issue = [&](int q, Stage& sg) {
+    const int st = q / KT, kt = q - st * KT, j = row_of_step(st);
+    const u32x4* Aj;
+    int64_t ldga;
+    if constexpr (FWD) {
+      ldga = KG;
+      Aj = SP + lay.off(j) + (i0 - (int64_t)(j + 1) * kIB) * KG * U;
+    } else {
+      ldga = lay.rows(i) / 8;
+      Aj = SP + lay.off(i) + ((i0 - (int64_t)i * kIB) * ldga + (int64_t)(j - i - 1) * KG) * U;
+    }
+    const u32x4* Xj = XS + (j0 * ldgx + (int64_t)j * KG) * U;
+    const int64_t g0 = 4 * kt;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r0 = wave * 16 + e * 4, r = r0 + (lane >> 4), u = (lane & 15) ^ swz(r);
+      dma16<false>(Aj + ((int64_t)r * ldga + g0) * U + u, &sg.a[r0 * UPR]);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c0 = wave * 8 + e * 4, c = c0 + (lane >> 4), u = (lane & 15) ^ swz(c);
+      const int64_t cc = c < N ? c : N - 1;
+      dma16<true>(Xj + (cc * ldgx + g0) * U + u, &sg.b[c0 * UPR]);
+    }
+  };
+  int known = 0;   // steps whose block rows are known complete
+  // k-tile q into sg once its block row is known complete (reading counters,
+  // and waiting, when it is not yet known)
+  auto fetch = [&](int q, Stage& sg) {
+    const int st = q / KT;
+    if (st >= known) {
+      const int c = poll(st);
+      if (c <= 0) {
+        ok = false;
+        return;
+      }
+      known = st + c;
+    }
+    issue(q, sg);
+  };
+  // k-tile q from stage cur; k-tile q + 2 goes to n2, the stage of k-tile
+  // q - 1 (free after the barrier).  Every k-tile is fetched two ahead of its
+  // product and no other vector-memory load is in flight in the loop (the
+  // counter loads are waited for where they are read), so "at most one
+  // k-tile in flight" means k-tile q has landed.
+  auto step = [&](int q, Stage& cur, Stage& n2) {
+    asm volatile("" ::: "memory");   // the previous product's LDS reads stay above the barrier
+    // k-tile q has landed when at most k-tile q + 1 is in flight
+    if (q + 1 < Q) __builtin_amdgcn_s_waitcnt(vm_wait(DPT));
+    else __builtin_amdgcn_s_waitcnt(vm_wait(0));
+    // every wave's k-tile q has landed and its reads of stage n2 are done
+    __builtin_amdgcn_s_waitcnt(kLdsWait);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");   // and this product's below it
+    if (q + 2 < Q) fetch(q + 2, n2);
+    compute(cur, tile.cr, tile.ci);
+  };
+  if (Q > 0) fetch(0, sm0);
+  if (Q > 1 && ok) fetch(1, sm1);
+  for (int q = 0; q < Q && ok; q += 3) {
+    step(q, sm0, sm2);
+    if (q + 1 < Q && ok) step(q + 1, sm1, sm0);
+    if (q + 2 < Q && ok) step(q + 2, sm2, sm1);
+  }
+  __builtin_amdgcn_s_waitcnt(0);   // nothing in flight into LDS past this point
+  __syncthreads();
+  if (!ok) {
+    const float nan = __builtin_nanf("");
+#pragma unroll
+    for (int a = 0; a < Tile::TM; ++a)
+#pragma unroll
+      for (int b = 0; b < Tile::TN; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tile.cr[a][b][r] = tile.ci[a][b][r] = nan;
+  }
+  tile.store_split_cols(XS + (j0 * ldgx + i0 / 8) * U, ldgx, M, N, make_float2(-1.f, 0.f));
+  // every wave's stores complete before the barrier after which the row
+  // counter is bumped (a workgroup fence emits no such wait on gfx950)
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[i], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (M > 0)
+    tile.store(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+}
+}  // namespace sdma
+
 // W = D^{-1} X block by block from the pre-split D inverses (DS: A-operand
 // rows, 32 groups each) and the forward sweep's pre-split result (XS:
 // B-operand columns, ldgx groups each); one workgroup per 64 x 64 tile of a
@@ -758,16 +1002,44 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
 // rows, not its MFMA work); with two LDS stages 64 x 64 took 1.08-1.10 ms (one
 // workgroup per CU) and 64 x 32 1.05-1.07 ms (63 KB: two per CU; the row's
 // last product 8.7 us against 10.7), 0.7 ms less per refined solve.
+// DMA-staged BF16x3 sweep launch (grid as launch_sweep's)
+template <int NSTG, bool FWD>
+int launch_sweep_dma(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X,
+                     int64_t ldx, u32x4* XS, int64_t n, int64_t nrhs, hipStream_t s) {
+  const int nblk = (int)((n + kIB - 1) / kIB);
+  const int ct = (int)((nrhs + sdma::BN - 1) / sdma::BN);
+  const int64_t grid = (int64_t)nblk * (kIB / sdma::BM) * ct;
+  if (grid > 0x7fffffffLL) return MG_ERR_ARG;
+  int* ctr = sc.take();
+  hipLaunchKernelGGL((sdma::k_ldu_sweep_dma<NSTG, FWD>), dim3((unsigned)grid), dim3(sdma::NT), 0, s, SP, np, Bin,
+                     ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr, sc.other(), (int)sc.cap);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
+// MG_SWEEP_DMA: 0 register-staged sweeps, 3 DMA stages (A/B timing)
+inline int sweep_stages() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("MG_SWEEP_DMA");
+    v = (e && e[0] == '0') ? 0 : 3;
+  }
+  return v;
+}
+
 template <int NS>
 int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, const u32x4* DS, int64_t np, float2* B,
                int64_t ldb, int64_t nrhs, float2* W, u32x4* XS, int64_t n, hipStream_t s) {
   constexpr int BN = 32;
   const int64_t nblk = (n + kIB - 1) / kIB;
   const int ct = (int)((nrhs + 63) / 64);
-  MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
+  const int dma = NS == 2 ? sweep_stages() : 0;
+  if (dma == 3) MG_TRY((launch_sweep_dma<3, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
+  else MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
   hipLaunchKernelGGL(k_d_product<NS>, dim3((unsigned)(nblk * (kIB / 64) * ct)), dim3(256), 0, s, DS, XS, np / 8, W,
                      nrhs, n, nrhs, ct);
   MG_HIP(hipGetLastError());
+  if (dma == 3) return launch_sweep_dma<3, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
   return launch_sweep<NS, 64, BN, 2, 2, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
 }
 
