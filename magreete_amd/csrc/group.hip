@@ -667,7 +667,12 @@ __global__ __launch_bounds__(mg_dma::NT) void k_grp_update_dma(const GrpProb* __
   const u32x4* B = reinterpret_cast<const u32x4*>(pb.SB) + j0 * KG * mg_dma::U;
   float2* C = reinterpret_cast<float2*>(pb.A) + (r0 + i0) * pb.lda + r0 + j0;
   auto tile = [&](int t, mg_dma::Stage& st) { mg_dma::load_tile(st, A, B, KG, 4 * t); };
-  mg_dma::tile_update<T>(s0, s1, tile, C, pb.lda, M - i0, cend - j0);
+  // C read late (by DMA) where its 16-byte units are aligned and whole
+  const int64_t Mt = M - i0, Nt = cend - j0;
+  if ((reinterpret_cast<uintptr_t>(C) & 15) == 0 && pb.lda % 2 == 0 && (Nt >= 128 || Nt % 2 == 0))
+    mg_dma::tile_update<T, true>(s0, s1, tile, C, pb.lda, Mt, Nt);
+  else
+    mg_dma::tile_update<T, false>(s0, s1, tile, C, pb.lda, Mt, Nt);
 }
 
 // ---- tournament-pivoting leaf (BF16x3 groups) --------------------------------
