@@ -1160,7 +1160,9 @@ __global__ void k_grp_clear(int* c, int cap) {
 // Measured on the 256-matrix ensemble (96 GB of complex128, 32 columns):
 // RF 2, KS 2 at two waves per SIMD 36 ms; RF 1 with KS 4 (three waves per
 // SIMD) 41 ms, with KS 2 44 ms; the rows staged through LDS by DMA in
-// 512-byte segments (three stages, one workgroup per CU) 59 ms.
+// 512-byte segments (three stages, one workgroup per CU) 59 ms; the FP64
+// DMA-staged GEMM tile of the Krylov product at 128 x 32 (three stages of
+// 40 KB, one workgroup per CU) 37 ms.
 constexpr int kResCols = 32;
 
 template <int RF, int KS>
