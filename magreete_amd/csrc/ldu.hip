@@ -549,6 +549,7 @@ int ldu_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, 
 }
 
 constexpr long long kSweepSpin = 1LL << 22;   // bound on one wait (~seconds)
+constexpr int kSweepTickets = 8;              // ticket counters ahead of a sweep's block-row counters
 
 // One block-triangular sweep of the LDU solve over the right-hand sides X
 // (n x nrhs, row-major, ldx), in one launch:
@@ -793,13 +794,22 @@ __global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ 
   if (blockIdx.x == 0)   // the next sweep's whole counter set (that launch follows this one on the stream)
     for (int e = threadIdx.x; e < nnext; e += blockDim.x)
       __hip_atomic_store(next + e, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // Tickets per XCD (workgroup b runs on XCD b mod 8): XCD x takes the row
+  // tiles g = 4 blk + (row tile) with g mod 8 = x, block rows in sweep
+  // order, each with its ct column tiles in a row, so that the column tiles
+  // of a row tile share one L2 for the panel block they all read.  Every
+  // class of workgroups hands out its tiles in dependency order and runs on
+  // its own XCD, so a workgroup only waits on tiles that are done or running.
+  const int xcd = blockIdx.x & 7;
+  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ctr + xcd, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int tpr = (kIB / BM) * ct;   // tiles per block row
-  const int blk = s_ticket / tpr, w = s_ticket % tpr;
+  const int g = xcd + 8 * (s_ticket / ct), ctile = s_ticket % ct;   // row tile (global), column tile
+  if (g >= (kIB / BM) * nblk) return;                               // a workgroup past its XCD's tiles
+  const int blk = g / (kIB / BM), rtile = g % (kIB / BM);
   const int i = FWD ? blk : nblk - 1 - blk;
-  int* done = ctr + 1;
-  const int64_t i0 = (int64_t)i * kIB + (int64_t)(w / ct) * BM, j0 = (int64_t)(w % ct) * BN;
+  int* done = ctr + kSweepTickets;
+  const int64_t i0 = (int64_t)i * kIB + (int64_t)rtile * BM, j0 = (int64_t)ctile * BN;
   const int64_t M = n - i0, N = nrhs - j0;   // valid extents (M <= 0: a row tile past a short last block)
   const int64_t ldgx = np / 8;
   const SplitLayout lay{np, nblk, U};
@@ -1008,7 +1018,10 @@ int launch_sweep_dma(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bi
                      int64_t ldx, u32x4* XS, int64_t n, int64_t nrhs, hipStream_t s) {
   const int nblk = (int)((n + kIB - 1) / kIB);
   const int ct = (int)((nrhs + sdma::BN - 1) / sdma::BN);
-  const int64_t grid = (int64_t)nblk * (kIB / sdma::BM) * ct;
+  // eight classes of workgroups (one per XCD), each with the column tiles of
+  // every eighth row tile
+  const int64_t row_tiles = (int64_t)nblk * (kIB / sdma::BM);
+  const int64_t grid = 8 * ((row_tiles + 7) / 8) * ct;
   if (grid > 0x7fffffffLL) return MG_ERR_ARG;
   int* ctr = sc.take();
   hipLaunchKernelGGL((sdma::k_ldu_sweep_dma<NSTG, FWD>), dim3((unsigned)grid), dim3(sdma::NT), 0, s, SP, np, Bin,
@@ -1331,13 +1344,13 @@ int getrs_ldu(Workspace* ws, const void* LDU, int64_t n, int64_t lda, const void
   const int64_t nb = kIB, nblk = (n + nb - 1) / nb, full = n / nb;
   if (sweep_ns == 2 || sweep_ns == 3) {
     if (!ws) return MG_ERR_ARG;
-    if (ws->sweep_cap < nblk + 1) {
+    if (ws->sweep_cap < nblk + kSweepTickets) {
       // grows rarely (a larger system than any before on this handle)
       MG_HIP(hipStreamSynchronize(s));
       if (ws->sweep) MG_HIP(hipFree(ws->sweep));
       ws->sweep = nullptr;
       ws->sweep_cap = 0;
-      const int64_t cap = ((nblk + 1 + 255) / 256) * 256;
+      const int64_t cap = ((nblk + kSweepTickets + 255) / 256) * 256;
       MG_HIP(hipMalloc(&ws->sweep, sizeof(int) * 2 * cap));
       hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((2 * cap + 255) / 256)), dim3(256), 0, s, ws->sweep,
                          (int)(2 * cap));
