@@ -700,19 +700,27 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)
 }
 
 // The BF16x3 sweep with its operand k-tiles copied to LDS by DMA through
-// NSTG stages (k_ldu_sweep stages one k-tile ahead through registers, so
+// three stages (k_ldu_sweep stages one k-tile ahead through registers, so
 // every k-tile waited about one memory latency; the chain of block rows is
-// made of such waits).  The same tiles (64 x 32, four waves of 32 x 16),
-// tickets, counters and stores as k_ldu_sweep.  x_j's units are read with
-// agent-scope (sc1) DMA: other workgroups of the launch wrote them.  Every
-// k-tile is fetched two ahead of its product; the counters of up to 8 block
-// rows are read at once (a counter load waits for the wave's earlier loads,
-// the DMA in flight included), when the first k-tile of a block row not yet
-// known complete is to be fetched.
+// made of such waits).  Tiles of RB rows (64 or 128) and 32 columns, waves
+// of 32 x 16; the same counters and stores as k_ldu_sweep, tickets per XCD.
+// x_j's units are read with agent-scope (sc1) DMA: other workgroups of the
+// launch wrote them.  Every k-tile is fetched two ahead of its product; the
+// counters of up to 8 block rows are read at once (a counter load waits for
+// the wave's earlier loads, the DMA in flight included), when the first
+// k-tile of a block row not yet known complete is to be fetched.
 namespace sdma {
-constexpr int BM = 64, BN = 32, NT = 256, UPR = 16, DPT = 6;   // DPT: DMA instructions per wave and k-tile
+// RB rows per tile (64 or 128), 32 columns; 2 RB / 32 waves of 32 x 16
+constexpr int BN = 32, UPR = 16;
+template <int RB>
+constexpr int nthreads() { return 4 * RB; }
+// DMA instructions per wave and k-tile: A 16 rows (four rows each), B 512 / RB
+// columns (four each)
+template <int RB>
+constexpr int dma_per_tile() { return 4 + 128 / RB; }
+template <int RB>
 struct Stage {
-  u32x4 a[BM * UPR];
+  u32x4 a[RB * UPR];
   u32x4 b[BN * UPR];
 };
 using mg_dma::gvp;
@@ -743,8 +751,9 @@ __device__ inline void dma16(const u32x4* src, u32x4* dst) {
 }
 
 // acc += the k-tile in st; wave (wm, wn) holds rows 32 wm + [0, 32) and
-// columns 16 wn + [0, 16) (the accumulator layout of PTile<2, 64, 32, 2, 2>).
-__device__ inline void compute(const Stage& st, v4f (&cr)[2][1], v4f (&ci)[2][1]) {
+// columns 16 wn + [0, 16) (the accumulator layout of PTile<2, RB, 32, RB / 32, 2>).
+template <int RB>
+__device__ inline void compute(const Stage<RB>& st, v4f (&cr)[2][1], v4f (&ci)[2][1]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1, li = lane & 15, lg = lane >> 4;
   const int sl = swz(li);
@@ -780,16 +789,17 @@ __device__ inline void compute(const Stage& st, v4f (&cr)[2][1], v4f (&ci)[2][1]
     }
 }
 
-template <int NSTG, bool FWD>
-__global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ SP, int64_t np, const float2* Bin,
+template <int RB, bool FWD>
+__global__ __launch_bounds__(nthreads<RB>()) void k_ldu_sweep_dma(const u32x4* __restrict__ SP, int64_t np, const float2* Bin,
                                                       int64_t ldbin, float2* X, int64_t ldx, u32x4* XS, int64_t n,
                                                       int64_t nrhs, int nblk, int ct, int* ctr, int* next,
                                                       int nnext) {
-  static_assert(NSTG == 3, "three stages");
-  typedef mg_ps::PTile<2, BM, BN, 2, 2> Tile;
+  constexpr int BM = RB, DPT = dma_per_tile<RB>();
+  typedef Stage<RB> Stg;
+  typedef mg_ps::PTile<2, BM, BN, BM / 32, 2> Tile;
   constexpr int U = 4, KG = kIB / 8, KT = KG / 4;
   // three stages, each named at every DMA and read (the loop is unrolled by three)
-  __shared__ Stage sm0, sm1, sm2;
+  __shared__ Stg sm0, sm1, sm2;
   __shared__ int s_ticket, s_cnt;
   if (blockIdx.x == 0)   // the next sweep's whole counter set (that launch follows this one on the stream)
     for (int e = threadIdx.x; e < nnext; e += blockDim.x)
@@ -865,7 +875,7 @@ __global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ 
     return c;
   };
   // k-tile q (step q / KT) into stage sg
-  auto issue = [&](int q, Stage& sg) {
+  auto issue = [&](int q, Stg& sg) {
     const int st = q / KT, kt = q - st * KT, j = row_of_step(st);
     const u32x4* Aj;
     int64_t ldga;
@@ -884,8 +894,8 @@ __global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ 
       dma16<false>(Aj + ((int64_t)r * ldga + g0) * U + u, &sg.a[r0 * UPR]);
     }
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int c0 = wave * 8 + e * 4, c = c0 + (lane >> 4), u = (lane & 15) ^ swz(c);
+    for (int e = 0; e < 128 / RB; ++e) {
+      const int c0 = wave * (512 / RB) + e * 4, c = c0 + (lane >> 4), u = (lane & 15) ^ swz(c);
       const int64_t cc = c < N ? c : N - 1;
       dma16<true>(Xj + (cc * ldgx + g0) * U + u, &sg.b[c0 * UPR]);
     }
@@ -893,7 +903,7 @@ __global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ 
   int known = 0;   // steps whose block rows are known complete
   // k-tile q into sg once its block row is known complete (reading counters,
   // and waiting, when it is not yet known)
-  auto fetch = [&](int q, Stage& sg) {
+  auto fetch = [&](int q, Stg& sg) {
     const int st = q / KT;
     if (st >= known) {
       const int c = poll(st);
@@ -910,7 +920,7 @@ __global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ 
   // product and no other vector-memory load is in flight in the loop (the
   // counter loads are waited for where they are read), so "at most one
   // k-tile in flight" means k-tile q has landed.
-  auto step = [&](int q, Stage& cur, Stage& n2) {
+  auto step = [&](int q, Stg& cur, Stg& n2) {
     asm volatile("" ::: "memory");   // the previous product's LDS reads stay above the barrier
     // k-tile q has landed when at most k-tile q + 1 is in flight
     if (q + 1 < Q) __builtin_amdgcn_s_waitcnt(vm_wait(DPT));
@@ -920,7 +930,7 @@ __global__ __launch_bounds__(NT) void k_ldu_sweep_dma(const u32x4* __restrict__ 
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");   // and this product's below it
     if (q + 2 < Q) fetch(q + 2, n2);
-    compute(cur, tile.cr, tile.ci);
+    compute<RB>(cur, tile.cr, tile.ci);
   };
   if (Q > 0) fetch(0, sm0);
   if (Q > 1 && ok) fetch(1, sm1);
@@ -1012,30 +1022,35 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
 // rows, not its MFMA work); with two LDS stages 64 x 64 took 1.08-1.10 ms (one
 // workgroup per CU) and 64 x 32 1.05-1.07 ms (63 KB: two per CU; the row's
 // last product 8.7 us against 10.7), 0.7 ms less per refined solve.
-// DMA-staged BF16x3 sweep launch (grid as launch_sweep's)
-template <int NSTG, bool FWD>
+// DMA-staged BF16x3 sweep launch: RB-row tiles
+template <int RB, bool FWD>
 int launch_sweep_dma(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X,
                      int64_t ldx, u32x4* XS, int64_t n, int64_t nrhs, hipStream_t s) {
   const int nblk = (int)((n + kIB - 1) / kIB);
   const int ct = (int)((nrhs + sdma::BN - 1) / sdma::BN);
   // eight classes of workgroups (one per XCD), each with the column tiles of
   // every eighth row tile
-  const int64_t row_tiles = (int64_t)nblk * (kIB / sdma::BM);
+  const int64_t row_tiles = (int64_t)nblk * (kIB / RB);
   const int64_t grid = 8 * ((row_tiles + 7) / 8) * ct;
   if (grid > 0x7fffffffLL) return MG_ERR_ARG;
   int* ctr = sc.take();
-  hipLaunchKernelGGL((sdma::k_ldu_sweep_dma<NSTG, FWD>), dim3((unsigned)grid), dim3(sdma::NT), 0, s, SP, np, Bin,
-                     ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr, sc.other(), (int)sc.cap);
+  hipLaunchKernelGGL((sdma::k_ldu_sweep_dma<RB, FWD>), dim3((unsigned)grid), dim3(sdma::nthreads<RB>()), 0, s, SP,
+                     np, Bin, ldbin, X, ldx, XS, n, nrhs, nblk, ct, ctr, sc.other(), (int)sc.cap);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }
 
-// MG_SWEEP_DMA: 0 register-staged sweeps, 3 DMA stages (A/B timing)
+// The BF16x3 sweeps' tiles: 128 rows (one workgroup of eight waves per CU;
+// the published rows x_j are read by half as many row tiles as with 64, two
+// workgroups per CU): standalone 1.52 against 1.61 ms per preconditioner
+// application.  MG_SWEEP_DMA: 64 or 128 rows, 0 the register-staged sweeps
+// (A/B timing).
 inline int sweep_stages() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("MG_SWEEP_DMA");
-    v = (e && e[0] == '0') ? 0 : 3;
+    v = e ? atoi(e) : 128;
+    if (v != 0 && v != 64 && v != 128) v = 128;
   }
   return v;
 }
@@ -1047,12 +1062,14 @@ int ldu_sweeps(SweepCtr& ctr, const u32x4* SPL, const u32x4* SPUr, const u32x4* 
   const int64_t nblk = (n + kIB - 1) / kIB;
   const int ct = (int)((nrhs + 63) / 64);
   const int dma = NS == 2 ? sweep_stages() : 0;
-  if (dma == 3) MG_TRY((launch_sweep_dma<3, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
+  if (dma == 64) MG_TRY((launch_sweep_dma<64, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
+  else if (dma == 128) MG_TRY((launch_sweep_dma<128, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
   else MG_TRY((launch_sweep<NS, 64, BN, 2, 2, true>(ctr, SPL, np, B, ldb, B, ldb, XS, n, nrhs, s)));
   hipLaunchKernelGGL(k_d_product<NS>, dim3((unsigned)(nblk * (kIB / 64) * ct)), dim3(256), 0, s, DS, XS, np / 8, W,
                      nrhs, n, nrhs, ct);
   MG_HIP(hipGetLastError());
-  if (dma == 3) return launch_sweep_dma<3, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
+  if (dma == 64) return launch_sweep_dma<64, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
+  if (dma == 128) return launch_sweep_dma<128, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
   return launch_sweep<NS, 64, BN, 2, 2, false>(ctr, SPUr, np, W, nrhs, B, ldb, XS, n, nrhs, s);
 }
 
