@@ -102,8 +102,11 @@ __device__ inline void inv_barrier(unsigned* bar, unsigned target, int* err) {
 //   X_i <- -C_i P^{-1}) as BF16x3 MFMA products.
 // One barrier per panel; zero pivots are reported in info (first, 1-based).
 // bar / err: this launch's barrier counter and failure flag (zero on entry);
-// next: the other set, cleared here for the next launch.
-__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
+// next: the other set, cleared here for the next launch.  One wave per SIMD
+// (its eight workgroups take eight CUs): the register budget of two spilled
+// 80 bytes per lane; with one, no spill and the 12288-row BF16x3
+// factorisation 20.4 -> 20.0 ms.
+__global__ __launch_bounds__(kMwThreads) __attribute__((amdgpu_waves_per_eu(1))) void k_block_inverse(const float2* __restrict__ A, int64_t lda, int h,
                                                               float2* __restrict__ X, int* info, int64_t row0,
                                                               float2* exch, unsigned* bar, int* err, int* next) {
   __shared__ InvSmem sm;
