@@ -537,6 +537,9 @@ inline bool ldu_update_uses_dma(int64_t M, int64_t N) {
   return ((M + dma::BM - 1) / dma::BM) * ((N + dma::BN - 1) / dma::BN) >= (int64_t)mg::cu_count();
 }
 
+bool small_update_dma();
+int launch_update_small(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s);
+
 template <int NS>
 int ldu_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
   if (M <= 0 || N <= 0) return MG_OK;
@@ -551,6 +554,7 @@ int ldu_update(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, 
         default: return dma::launch_dma<4>(M, N, src, C, ldc, s);
       }
     }
+    if (small_update_dma()) return launch_update_small(M, N, src, C, ldc, s);
     return launch_update<NS, 64, 128, 2, 2>(M, N, src, C, ldc, s);
   } else {
     return launch_update<NS, 64, 64, 2, 2>(M, N, src, C, ldc, s);
@@ -969,6 +973,71 @@ __global__ __launch_bounds__(nthreads<RB>()) void k_ldu_sweep_dma(const u32x4* _
   if (M > 0)
     tile.store(X + i0 * ldx + j0, ldx, M, N, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
 }
+
+// Trailing update C -= sum_s A_s B_s (as k_ldu_update, same operands) with
+// the sweep's tiles and pipeline: RB x 32 tiles, operand k-tiles DMA'd two
+// ahead through three stages; for the chain-side updates too small for a
+// 128 x 128 tile per CU (the next diagonal block and the strips of the next
+// super-step), which the register-staged tiles ran one memory latency per
+// k-tile.  The operands hold whole tiles (padded panels), so the DMA reads
+// need no bounds; C is loaded and stored within M x N.
+template <int RB>
+__global__ __launch_bounds__(nthreads<RB>()) void k_ldu_update_small(int64_t M, int64_t N, UpdSrc src,
+                                                                    float2* __restrict__ C, int64_t ldc, int nbm,
+                                                                    int nbn) {
+  constexpr int BM = RB, DPT = dma_per_tile<RB>(), U = 4, KG = kIB / 8, KT = KG / 4;
+  typedef Stage<RB> Stg;
+  typedef mg_ps::PTile<2, BM, BN, BM / 32, 2> Tile;
+  static_assert(kIB % BM == 0 && kIB % BN == 0, "tiles within the padded 256-row blocks");
+  __shared__ Stg sm0, sm1, sm2;
+  int tm, tn;
+  mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
+  const int64_t i0 = (int64_t)tm * BM, j0 = (int64_t)tn * BN;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  Tile tile;
+  tile.init(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), true);
+  // the start values have landed before the loop (the compiler does not see
+  // the DMA; see dma16)
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(vm_wait(0));
+  const int Q = src.n * KT;
+  auto issue = [&](int q, Stg& sg) {
+    const int sq = q / KT, kt = q - sq * KT;
+    const u32x4* A = src.A[sq] + i0 * KG * U;
+    const u32x4* B = src.B[sq] + j0 * KG * U;
+    const int64_t g0 = 4 * kt;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r0 = wave * 16 + e * 4, r = r0 + (lane >> 4), u = (lane & 15) ^ swz(r);
+      dma16<false>(A + ((int64_t)r * KG + g0) * U + u, &sg.a[r0 * UPR]);
+    }
+#pragma unroll
+    for (int e = 0; e < 128 / RB; ++e) {
+      const int c0 = wave * (512 / RB) + e * 4, c = c0 + (lane >> 4), u = (lane & 15) ^ swz(c);
+      dma16<false>(B + ((int64_t)c * KG + g0) * U + u, &sg.b[c0 * UPR]);
+    }
+  };
+  // k-tile q from stage cur; k-tile q + 2 into n2 (the stage of k-tile q - 1)
+  auto step = [&](int q, Stg& cur, Stg& n2) {
+    asm volatile("" ::: "memory");
+    if (q + 1 < Q) __builtin_amdgcn_s_waitcnt(vm_wait(DPT));
+    else __builtin_amdgcn_s_waitcnt(vm_wait(0));
+    __builtin_amdgcn_s_waitcnt(kLdsWait);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (q + 2 < Q) issue(q + 2, n2);
+    compute<RB>(cur, tile.cr, tile.ci);
+  };
+  if (Q > 0) issue(0, sm0);
+  if (Q > 1) issue(1, sm1);
+  for (int q = 0; q < Q; q += 3) {
+    step(q, sm0, sm2);
+    if (q + 1 < Q) step(q + 1, sm1, sm0);
+    if (q + 2 < Q) step(q + 2, sm2, sm1);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  tile.store(C + i0 * ldc + j0, ldc, M - i0, N - j0, make_float2(-1.f, 0.f), make_float2(1.f, 0.f), false);
+}
 }  // namespace sdma
 
 // W = D^{-1} X block by block from the pre-split D inverses (DS: A-operand
@@ -1031,6 +1100,27 @@ int launch_sweep(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, i
 // rows, not its MFMA work); with two LDS stages 64 x 64 took 1.08-1.10 ms (one
 // workgroup per CU) and 64 x 32 1.05-1.07 ms (63 KB: two per CU; the row's
 // last product 8.7 us against 10.7), 0.7 ms less per refined solve.
+// The chain-side BF16x3 updates (C too small for a 128 x 128 tile per CU):
+// 64 x 32 tiles staged by DMA (sdma::k_ldu_update_small); MG_SMALL_UPDATE=0
+// selects the register-staged 64 x 128 tiles (A/B timing).
+bool small_update_dma() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("MG_SMALL_UPDATE");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
+
+int launch_update_small(int64_t M, int64_t N, const UpdSrc& src, float2* C, int64_t ldc, hipStream_t s) {
+  constexpr int RB = 64;
+  const int nbm = (int)((M + RB - 1) / RB), nbn = (int)((N + sdma::BN - 1) / sdma::BN);
+  hipLaunchKernelGGL((sdma::k_ldu_update_small<RB>), dim3(nbm * nbn), dim3(sdma::nthreads<RB>()), 0, s, M, N, src,
+                     C, ldc, nbm, nbn);
+  MG_HIP(hipGetLastError());
+  return MG_OK;
+}
+
 // DMA-staged BF16x3 sweep launch: RB-row tiles
 template <int RB, bool FWD>
 int launch_sweep_dma(SweepCtr& sc, const u32x4* SP, int64_t np, const float2* Bin, int64_t ldbin, float2* X,
