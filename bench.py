@@ -36,8 +36,9 @@ KERNEL = {"fp64": {"dma": "k_zgemm_dma_trail<64> (pivoted LU trailing update; 12
           "bf16x3": {"dma": "k_ldu_update_dma<1|2, LATE> (block-LDU trailing update over pre-split panels, one or "
                             "two steps per launch; BF16x3 MFMA, 128x128 tiles, operands DMA'd to LDS, C read late "
                             "by DMA)",
-                     "reg": "k_ldu_update<2, 64, 128, 2, 2> (the chain-side updates too small for a 128x128 tile "
-                            "per CU: next diagonal blocks, block rows and columns of the next super-step)"},
+                     "reg": "sdma::k_ldu_update_small<64> (the chain-side updates too small for a 128x128 tile "
+                            "per CU: next diagonal blocks, block rows and columns of the next super-step; 64x32 "
+                            "tiles, operands DMA'd to LDS two k-tiles ahead)"},
           "bf16x6": {"reg": "k_ldu_update<3, 64, 64, 2, 2> (block-LDU trailing update over pre-split panels, one or "
                             "two steps per launch; BF16x6 MFMA, 64x64 tiles)"}}
 PRODUCT_KERNEL = ("k_zgemm_dma<64> + k_splitk_reduce<double> (the refinement's FP64 Krylov product M Z, "
@@ -50,7 +51,7 @@ PRODUCT_KERNEL = ("k_zgemm_dma<64> + k_splitk_reduce<double> (the refinement's F
 # block LDU: 8 B per complex64 element each way, 8 B per pre-split BF16x3 panel element and k); the
 # product: M and Z read once, the two K-split partials written and read once, the result written once.
 TRAFFIC = {("fp64", "dma"): 2.192e9, ("fp64", "reg"): 6.44e7, ("c64", "reg"): 4.410e8,
-           ("bf16x3", "dma"): 1.189e9, ("bf16x3", "reg"): 3.85e7, ("bf16x6", "reg"): 2.709e8, "product": 4.179e9}
+           ("bf16x3", "dma"): 1.189e9, ("bf16x3", "reg"): 3.95e7, ("bf16x6", "reg"): 2.709e8, "product": 4.179e9}
 ALGO_BYTES = {("fp64", "dma"): 1.345e9, ("fp64", "reg"): 4.93e7, ("bf16x3", "dma"): 6.7e8, "product": 2.718e9}
 
 
