@@ -38,7 +38,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_cgemm_trail(int64_t M, int64_t
 
 // FP64 GEMM with DMA-staged operands (zgemm_dma.h); blockIdx.y walks the
 // batch as in k_cgemm.  The _trail symbol serves the LU trailing update.
-template <int BN>
+template <int BN, bool LATE>
 __global__ __launch_bounds__(mg_zdma::NT) void k_zgemm_dma(int64_t M, int64_t N, int64_t K, double2 alpha,
                                                           const double2* A, int64_t lda, int64_t sA,
                                                           const double2* B, int64_t ldb, int64_t sB, double2 beta,
@@ -46,17 +46,17 @@ __global__ __launch_bounds__(mg_zdma::NT) void k_zgemm_dma(int64_t M, int64_t N,
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
   const int64_t b = blockIdx.y;
-  mg_zdma::zgemm_tile<BN>(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
+  mg_zdma::zgemm_tile<BN, LATE>(M, N, K, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc, tm, tn);
 }
 
-template <int BN>
+template <int BN, bool LATE>
 __global__ __launch_bounds__(mg_zdma::NT) void k_zgemm_dma_trail(int64_t M, int64_t N, int64_t K, double2 alpha,
                                                                 const double2* A, int64_t lda, const double2* B,
                                                                 int64_t ldb, double2 beta, double2* C, int64_t ldc,
                                                                 int nbm, int nbn) {
   int tm, tn;
   mg_tile_order(blockIdx.x, nbm, nbn, tm, tn);
-  mg_zdma::zgemm_tile<BN>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
+  mg_zdma::zgemm_tile<BN, LATE>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
 }
 
 template <typename T>
@@ -102,6 +102,16 @@ inline bool zdma_enabled() {
   return v == 1;
 }
 
+// MG_ZGEMM_LATE=0: C loaded before the first k-tile (A/B timing)
+inline bool zdma_late() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("MG_ZGEMM_LATE");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
+
 // FP64 with DMA-staged operands: 128 x 64 tiles, three stages (one workgroup per CU)
 template <bool TRAIL>
 int launch_zdma(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_t lda, int64_t sA, const void* B,
@@ -112,12 +122,21 @@ int launch_zdma(int64_t M, int64_t N, int64_t K, cd alpha, const void* A, int64_
   const double2* a = reinterpret_cast<const double2*>(A);
   const double2* b = reinterpret_cast<const double2*>(B);
   double2* c = reinterpret_cast<double2*>(C);
-  if (TRAIL)
-    hipLaunchKernelGGL((k_zgemm_dma_trail<BN>), dim3(nbm * nbn), dim3(mg_zdma::NT), 0, s, M, N, K, al, a, lda, b, ldb,
-                       be, c, ldc, nbm, nbn);
+  // C read late (zgemm_tile's LATE) when it is read at all and its elements are 16-byte aligned
+  const bool late = zdma_late() && !(beta.re == 0 && beta.im == 0) && K >= 3 * mg_zdma::BK &&
+                    ((reinterpret_cast<uintptr_t>(C) | (uintptr_t)(sC * 16)) & 15) == 0;
+  if (TRAIL && late)
+    hipLaunchKernelGGL((k_zgemm_dma_trail<BN, true>), dim3(nbm * nbn), dim3(mg_zdma::NT), 0, s, M, N, K, al, a, lda, b,
+                       ldb, be, c, ldc, nbm, nbn);
+  else if (TRAIL)
+    hipLaunchKernelGGL((k_zgemm_dma_trail<BN, false>), dim3(nbm * nbn), dim3(mg_zdma::NT), 0, s, M, N, K, al, a, lda,
+                       b, ldb, be, c, ldc, nbm, nbn);
+  else if (late)
+    hipLaunchKernelGGL((k_zgemm_dma<BN, true>), dim3(nbm * nbn, (unsigned)batch), dim3(mg_zdma::NT), 0, s, M, N, K,
+                       al, a, lda, sA, b, ldb, sB, be, c, ldc, sC, nbm, nbn);
   else
-    hipLaunchKernelGGL((k_zgemm_dma<BN>), dim3(nbm * nbn, (unsigned)batch), dim3(mg_zdma::NT), 0, s, M, N, K, al, a,
-                       lda, sA, b, ldb, sB, be, c, ldc, sC, nbm, nbn);
+    hipLaunchKernelGGL((k_zgemm_dma<BN, false>), dim3(nbm * nbn, (unsigned)batch), dim3(mg_zdma::NT), 0, s, M, N, K,
+                       al, a, lda, sA, b, ldb, sB, be, c, ldc, sC, nbm, nbn);
   MG_HIP(hipGetLastError());
   return MG_OK;
 }

@@ -47,7 +47,17 @@ constexpr int dma_per_tile() { return 4 + BN / 32; }
 // Issues k-tile kt of both operands into st.  A: the tile's first row;
 // B: column j0 of row 0.  rmax / cmax: the last valid row of A / column of
 // B relative to the tile.
-template <int BN>
+// One 16-byte-per-lane copy global -> LDS (lane l to dst + 16 l), written
+// as asm so that the compiler does not track it: the caller's s_waitcnt
+// before each barrier is then the only wait (the compiler otherwise waits for
+// every copy in flight before LDS reads whose stage it cannot tell apart).
+__device__ __attribute__((always_inline)) inline void dma16_asm(const void* src, lvp dst) {
+  const unsigned lds = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)dst);
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds)
+               : "memory", "m0");
+}
+
+template <int BN, bool ASM = false>
 __device__ __attribute__((always_inline)) inline void load_tile(Stage<BN>& st, gcptr<double2> A, int64_t lda,
                                                                 gcptr<double2> B, int64_t ldb, int64_t kt, int rmax,
                                                                 int cmax) {
@@ -58,14 +68,16 @@ __device__ __attribute__((always_inline)) inline void load_tile(Stage<BN>& st, g
     const int r0 = wave * 16 + i * 4;
     const int r = r0 + (lane >> 4), u = (lane & 15) ^ swz(r);
     const int rr = r < rmax ? r : rmax;
-    __builtin_amdgcn_global_load_lds((gvp)(A + (int64_t)rr * lda + k0 + u), (lvp)&st.a[r0 * BK], 16, 0, 0);
+    if constexpr (ASM) dma16_asm(A + (int64_t)rr * lda + k0 + u, (lvp)&st.a[r0 * BK]);
+    else __builtin_amdgcn_global_load_lds((gvp)(A + (int64_t)rr * lda + k0 + u), (lvp)&st.a[r0 * BK], 16, 0, 0);
   }
 #pragma unroll
   for (int i = 0; i < BN / 32; ++i) {
     const int q = wave * (BN / 32) + i;   // 64-element run q: row q / (BN / 64), columns from 64 (q % (BN / 64))
     const int k = q / (BN / 64), c0 = (q % (BN / 64)) * 64;
     const int c = c0 + (lane ^ swz(k >> 2)), cc = c < cmax ? c : cmax;   // slot c0 + lane holds column c
-    __builtin_amdgcn_global_load_lds((gvp)(B + (k0 + k) * ldb + cc), (lvp)&st.b[k * BN + c0], 16, 0, 0);
+    if constexpr (ASM) dma16_asm(B + (k0 + k) * ldb + cc, (lvp)&st.b[k * BN + c0]);
+    else __builtin_amdgcn_global_load_lds((gvp)(B + (k0 + k) * ldb + cc), (lvp)&st.b[k * BN + c0], 16, 0, 0);
   }
 }
 
@@ -120,25 +132,38 @@ constexpr unsigned kLdsWait = 0xC07F;   // lgkmcnt(0)
 // k-tile t + 2 (the stage that k-tile t - 1 occupied).  Every wave has
 // finished its reads of nxt before the barrier (lgkmcnt(0) after its
 // compute), and k-tile t has landed for every wave after it.
-template <int BN, int TN>
+// LATE: the stages freed by the last two k-tiles receive C's first two row
+// parts (cpart(part, stage)); k-tile T - 1 is then waited for with the first
+// part in flight (as many DMA instructions per wave as a k-tile).
+template <int BN, int TN, bool LATE = false, typename CPart = int>
 __device__ __attribute__((always_inline)) inline void kstep(Stage<BN>& cur, Stage<BN>& nxt, int64_t t, int64_t T,
                                                             gcptr<double2> A, int64_t lda, gcptr<double2> B,
                                                             int64_t ldb, int rmax, int cmax, v4d (&t1)[TM][TN],
-                                                            v4d (&t2)[TM][TN], v4d (&t3)[TM][TN]) {
-  if (t + 1 < T) __builtin_amdgcn_s_waitcnt(vm_wait(dma_per_tile<BN>()));   // k-tile t + 1 may be in flight
+                                                            v4d (&t2)[TM][TN], v4d (&t3)[TM][TN],
+                                                            CPart cpart = 0) {
+  if (t + 1 < T || LATE) __builtin_amdgcn_s_waitcnt(vm_wait(dma_per_tile<BN>()));   // k-tile t + 1 may be in flight
   else __builtin_amdgcn_s_waitcnt(vm_wait(0));
   __builtin_amdgcn_s_barrier();
-  if (t + 2 < T) load_tile<BN>(nxt, A, lda, B, ldb, t + 2, rmax, cmax);
+  if constexpr (LATE) asm volatile("" ::: "memory");   // no LDS read moves above the barrier
+  if (t + 2 < T) load_tile<BN, LATE>(nxt, A, lda, B, ldb, t + 2, rmax, cmax);
+  else if constexpr (LATE) cpart((int)(t + 2 - T), nxt);
   compute<BN, TN>(cur, t1, t2, t3);
   __builtin_amdgcn_s_waitcnt(kLdsWait);
 }
 
 // The tile (tm, tn) of C = alpha A B + beta C; K % BK == 0, alpha != 0.
-template <int BN>
+// LATE (BN = 64, K >= 3 BK, beta != 0, C 16-byte aligned): the accumulators
+// start at zero and C is copied into the stages by DMA as the last k-tiles
+// free them (rows 0-47 and 48-95 during the last two k-tiles' products, rows
+// 96-127 after them: one complex128 element per lane, a 64-element row per
+// instruction), then C = beta C + alpha acc from LDS.  With C loaded first,
+// its read held up the first products.
+template <int BN, bool LATE = false>
 __device__ __attribute__((always_inline)) inline void zgemm_tile(int64_t M, int64_t N, int64_t K, double2 alpha,
                                                                  const void* Ap, int64_t lda, const void* Bp,
                                                                  int64_t ldb, double2 beta, void* Cp, int64_t ldc,
                                                                  int tm, int tn) {
+  static_assert(!LATE || BN == 64, "C rows of 64 elements, one per lane");
   constexpr int TN = BN / 32;
   __shared__ Stage<BN> s0, s1, s2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -152,7 +177,7 @@ __device__ __attribute__((always_inline)) inline void zgemm_tile(int64_t M, int6
 
   // accumulators from (beta / alpha) C: t1 = cr, t2 = 0, t3 = cr + ci, so
   // that t1 - t2 and t3 - t1 - t2 carry it; loaded before the first DMA
-  const bool cload = !(beta.x == 0 && beta.y == 0);
+  const bool cload = !LATE && !(beta.x == 0 && beta.y == 0);
   double2 q;
   {
     const double d = alpha.x * alpha.x + alpha.y * alpha.y;
@@ -179,12 +204,70 @@ __device__ __attribute__((always_inline)) inline void zgemm_tile(int64_t M, int6
       }
     }
 
-  if (T > 0) load_tile<BN>(s0, A, lda, B, ldb, 0, rmax, cmax);
-  if (T > 1) load_tile<BN>(s1, A, lda, B, ldb, 1, rmax, cmax);
+  // LATE: C row part p (rows 48 p .. 48 p + 47, the last part 32 rows) into
+  // stage st: wave w takes 6 (4) rows, lane l column l
+  // (stages by LDS byte address: 32-bit, wave-uniform)
+  auto lds_of = [](Stage<BN>& st) { return (unsigned)(uintptr_t)(lvp)&st; };
+  auto cpart_at = [&](int part, unsigned base) {
+    const int rpw = part < 2 ? 6 : 4;
+    const int cc = lane < cmax ? lane : cmax;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      if (e < rpw) {
+        const int rl = wave * rpw + e, r = 48 * part + rl, rr = r < rmax ? r : rmax;
+        const unsigned lds = __builtin_amdgcn_readfirstlane(base + (unsigned)(rl * BN * 16));
+        asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(
+                         C + (i0 + rr) * ldc + j0 + cc),
+                     "s"(lds)
+                     : "memory", "m0");
+      }
+    }
+  };
+  auto cpart = [&](int part, Stage<BN>& st) { cpart_at(part, lds_of(st)); };
+  if (T > 0) load_tile<BN, LATE>(s0, A, lda, B, ldb, 0, rmax, cmax);
+  if (T > 1) load_tile<BN, LATE>(s1, A, lda, B, ldb, 1, rmax, cmax);
   for (int64_t t = 0; t < T; t += 3) {
-    kstep<BN, TN>(s0, s2, t, T, A, lda, B, ldb, rmax, cmax, t1, t2, t3);
-    if (t + 1 < T) kstep<BN, TN>(s1, s0, t + 1, T, A, lda, B, ldb, rmax, cmax, t1, t2, t3);
-    if (t + 2 < T) kstep<BN, TN>(s2, s1, t + 2, T, A, lda, B, ldb, rmax, cmax, t1, t2, t3);
+    kstep<BN, TN, LATE>(s0, s2, t, T, A, lda, B, ldb, rmax, cmax, t1, t2, t3, cpart);
+    if (t + 1 < T) kstep<BN, TN, LATE>(s1, s0, t + 1, T, A, lda, B, ldb, rmax, cmax, t1, t2, t3, cpart);
+    if (t + 2 < T) kstep<BN, TN, LATE>(s2, s1, t + 2, T, A, lda, B, ldb, rmax, cmax, t1, t2, t3, cpart);
+  }
+  if constexpr (LATE) {
+    // every wave is done with k-tile T - 1's stage: C's last part into it
+    __builtin_amdgcn_s_barrier();
+    const unsigned o0 = lds_of(s0), o1 = lds_of(s1), o2 = lds_of(s2);
+    auto stage_at = [&](int64_t idx) { return idx == 0 ? o0 : (idx == 1 ? o1 : o2); };
+    cpart_at(2, stage_at((T - 1) % 3));
+    __builtin_amdgcn_s_waitcnt(vm_wait(0));
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int64_t cl = wn * (BN / 2) + j * 16 + li, col = j0 + cl;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rl = wm * 32 + i * 16 + lk + 4 * r;
+          const int64_t row = i0 + rl;
+          const int part = rl < 48 ? 0 : (rl < 96 ? 1 : 2);
+#if __HIP_DEVICE_COMPILE__
+          typedef __attribute__((address_space(3))) const double* ldp;
+          const ldp L = (ldp)(uintptr_t)stage_at((T - 3 + part) % 3) + 2 * ((rl - 48 * part) * BN + cl);
+          const double2 c = make_double2(L[0], L[1]);
+#else
+          const double2 c = make_double2(0.0, 0.0);
+#endif
+          if (row < M && col < N) {
+            const double a = t1[i][j][r], b = t2[i][j][r], d = t3[i][j][r];
+            const double xr = a - b, xi = d - a - b;
+            double2 out;
+            out.x = beta.x * c.x - beta.y * c.y + alpha.x * xr - alpha.y * xi;
+            out.y = beta.x * c.y + beta.y * c.x + alpha.x * xi + alpha.y * xr;
+            C[row * ldc + col] = out;
+          }
+        }
+      }
+    return;
   }
 
 #pragma unroll
