@@ -29,8 +29,8 @@ PEAK_Z3 = 78.6 * 4.0 / 3.0
 PEAK_CLASS = {("fp64", "dma"): PEAK_Z3, ("fp64", "reg"): 78.6, ("c64", "reg"): 157.3 * 4.0 / 3.0,
               ("bf16x3", "dma"): PEAK["bf16x3"], ("bf16x3", "reg"): PEAK["bf16x3"], ("bf16x6", "reg"): PEAK["bf16x6"]}
 # the factorisation's trailing-update launches (timed with HIP events on their stream), per kernel class
-KERNEL = {"fp64": {"dma": "k_zgemm_dma_trail<64> (pivoted LU trailing update; 128x64 tiles, operands DMA'd to LDS "
-                          "through three stages, 3-product complex MFMA)",
+KERNEL = {"fp64": {"dma": "k_zgemm_dma_trail<64, true> (pivoted LU trailing update; 128x64 tiles, operands DMA'd to "
+                          "LDS through three stages, C DMA'd late into the freed stages, 3-product complex MFMA)",
                    "reg": "k_cgemm_trail<double, 64, 64> (the updates too small for a tile per CU)"},
           "c64": {"reg": "k_cgemm_trail<float> (block-LDU trailing update; 128x64 tiles, 3-product complex MFMA)"},
           "bf16x3": {"dma": "k_ldu_update_dma<1|2, LATE> (block-LDU trailing update over pre-split panels, one or "
@@ -50,7 +50,7 @@ PRODUCT_KERNEL = ("k_zgemm_dma<64> + k_splitk_reduce<double> (the refinement's F
 # and written once plus each operand panel read once (pivoted LU: 16 B per complex128 element each way;
 # block LDU: 8 B per complex64 element each way, 8 B per pre-split BF16x3 panel element and k); the
 # product: M and Z read once, the two K-split partials written and read once, the result written once.
-TRAFFIC = {("fp64", "dma"): 2.192e9, ("fp64", "reg"): 6.44e7, ("c64", "reg"): 4.410e8,
+TRAFFIC = {("fp64", "dma"): 2.181e9, ("fp64", "reg"): 6.44e7, ("c64", "reg"): 4.410e8,
            ("bf16x3", "dma"): 1.189e9, ("bf16x3", "reg"): 3.95e7, ("bf16x6", "reg"): 2.709e8, "product": 4.179e9}
 ALGO_BYTES = {("fp64", "dma"): 1.345e9, ("fp64", "reg"): 4.93e7, ("bf16x3", "dma"): 6.7e8, "product": 2.718e9}
 
