@@ -13,7 +13,10 @@ constexpr int kLeaf = 32;              // widest leaf of the triangular solves
 // (solve 160 ms against 158 ms)
 constexpr int kGetf2MaxW = 32;
 constexpr int kMaxGetf2Blocks = 256;   // workgroups of one leaf launch (<= one per CU)
-constexpr int kGetf2MinRows = 64;      // aim for at least this many rows per workgroup
+// aim for at least this many rows per workgroup (MG_GETF2_ROWS overrides it);
+// the 12288-row factorisation: 141.5 ms at 32 (256 workgroups), 138.9 at 48,
+// 137.0-137.4 at 64, 136.7-137.1 at 96 (128 workgroups): flat above 64
+constexpr int kGetf2MinRows = 64;
 // dynamic LDS per leaf workgroup: small enough that a leaf workgroup shares its
 // CU with a trailing-update workgroup (the look-ahead runs both at once);
 // measured on the 12288-row factorisation: 138 ms at 48 KB, 158 ms at 96 KB,

@@ -442,13 +442,25 @@ int getrs_dinv(const void* LU, int64_t n, int64_t lda, const int* ipiv, const vo
   return MG_OK;
 }
 
+// rows per leaf workgroup to aim for (MG_GETF2_ROWS overrides kGetf2MinRows)
+static int getf2_min_rows() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("MG_GETF2_ROWS");
+    const int r = e ? atoi(e) : 0;
+    v = r >= 16 && r <= 4096 ? r : kGetf2MinRows;
+  }
+  return v;
+}
+
 template <typename T>
 static int getf2(Workspace& ws, void* A, int64_t lda, int64_t k, int64_t m, int64_t w, int* ipiv, int* info,
                  hipStream_t s) {
   typedef typename C2<T>::t T2;
   const int64_t bytes_per_row = w * (int64_t)sizeof(T2);
   const int64_t max_rows = kGetf2Lds / bytes_per_row;
-  int G = (int)((m + kGetf2MinRows - 1) / kGetf2MinRows);
+  const int min_rows = getf2_min_rows();
+  int G = (int)((m + min_rows - 1) / min_rows);
   const int gmax = ws.coop_blocks;
   if (G > gmax) G = gmax;
   if (G < 1) G = 1;
