@@ -40,7 +40,8 @@ def test_cgemm(prec, mnk):
     assert err < {"c128": 1e-14, "c64": 2e-6, "bf16x3": 5e-5, "bf16x6": 2e-6}[prec], err
 
 
-@pytest.mark.parametrize("mnk", [(4200, 4300, 96), (4099, 4113, 64), (12288, 256, 512), (2100, 4113, 1024)])
+@pytest.mark.parametrize("mnk", [(4200, 4300, 96), (4099, 4113, 64), (4099, 4113, 48), (4200, 4300, 32),
+                                 (12288, 256, 512), (2100, 4113, 1024)])
 @pytest.mark.parametrize("ld_pad", [0, 24])
 def test_cgemm_fp64_dma_shapes(mnk, ld_pad):
     """Shapes that take the FP64 kernel with DMA-staged operands (tiles for
